@@ -1,0 +1,134 @@
+/*
+ * symhip.h - C ABI of the MI355X runtime that executes compiled expression programs.
+ *
+ * It replaces the five functions that the reference emits into every generated shared
+ * library and resolves with dlsym (src/StringResources.h:198-204, loaded at
+ * src/ComputeUnit.cpp:146-152):
+ *
+ *   reference                                  this ABI
+ *   init(indexData, outIndexData, constData)   symhip_create  (tables copied to HBM once)
+ *   (system("hipcc ...") + dlopen)             symhip_compile (hipRTC, in process)
+ *   setArg(d, i)                               symhip_set_arg / symhip_set_arg_device
+ *   run()                                      symhip_run     (all launches on one stream)
+ *   getResult(d, k)                            symhip_get_result / symhip_get_result_device
+ *   finish()                                   symhip_destroy (frees every device buffer)
+ *
+ * Differences in behaviour, all deliberate:
+ *  - state lives behind an opaque handle instead of file-scope globals, so several programs
+ *    can be live in one process;
+ *  - every call returns a status code (0 = success, > 0 = hipError_t, < 0 = SYMHIP_E_*)
+ *    and leaves a message for symhip_last_error(), instead of exit(code) on a HIP error
+ *    (src/StringResources.h:174-181) or a silent no-op when linking failed
+ *    (src/ComputeUnit.cpp:155-165);
+ *  - results are written in the caller's output order by the program itself: there is no
+ *    separate gather kernel (src/StringResources.h:218-221).
+ *
+ * Threading: one handle must not be used from two threads at once.  Calls on different
+ * handles are independent.
+ */
+#ifndef SYMHIP_H
+#define SYMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SYMHIP_SUCCESS = 0,
+    SYMHIP_E_INVALID = -1,    /* bad argument (null handle, index or size out of range) */
+    SYMHIP_E_NO_DEVICE = -2,  /* no HIP device visible */
+    SYMHIP_E_COMPILE = -3,    /* hipRTC rejected the source; see symhip_last_error() */
+    SYMHIP_E_NOT_COMPILED = -4, /* run before a successful compile / load */
+    SYMHIP_E_SYMBOL = -5,     /* a launch names a function the module does not contain */
+    SYMHIP_E_IO = -6          /* code-object cache could not be read or written */
+};
+
+typedef struct symhip_program* symhip_t;
+
+/* One kernel launch of a program; launches run in order on one stream. */
+typedef struct {
+    const char* name; /* extern "C" __global__ function in the compiled source */
+    uint32_t grid;    /* workgroups */
+    uint32_t block;   /* work-items per workgroup */
+} symhip_launch;
+
+/*
+ * Everything the runtime needs besides the source (the reference's generated size arrays,
+ * src/ComputeUnit.cpp:376-407, plus its three table files).  All arrays are copied.
+ *   values    : FP64 workspace x = [inputs | intermediates], num_values entries
+ *   arg_*     : where input container i lives inside x
+ *   out_*     : where output container k lives inside the result buffer
+ *   index     : u32 gather table (positions inside x)
+ *   consts    : FP64 per-instance constants
+ *   outpos    : u32 placement table (positions inside the result buffer)
+ * Generated kernels take (x, x, consts, index, outpos, result) in that order.
+ */
+typedef struct {
+    uint64_t num_values;
+    uint32_t num_args;
+    const uint64_t* arg_start;
+    const uint64_t* arg_length;
+    uint32_t num_outputs;
+    const uint64_t* out_start;
+    const uint64_t* out_length;
+    uint64_t num_index;
+    const uint32_t* index_data;
+    uint64_t num_const;
+    const double* const_data;
+    uint64_t num_outpos;
+    const uint32_t* outpos_data;
+    uint32_t num_launches;
+    const symhip_launch* launches;
+} symhip_program_desc;
+
+/* 1 when a HIP device is visible, 0 otherwise (never initialises more than the runtime). */
+int symhip_device_available(void);
+
+/* Allocate device buffers and upload the tables (reference: init). */
+int symhip_create(symhip_t* out, const symhip_program_desc* desc);
+
+/*
+ * Compile `source` with hipRTC for the current device's architecture (gfx950 when no device
+ * is present) and load it.  `options` is a space-separated list of extra compiler options
+ * or NULL.  Code objects are cached on disk by a hash of source, options and architecture.
+ */
+int symhip_compile(symhip_t h, const char* source, const char* options);
+
+/* Compile without a device and without a handle; writes the code object to `out_path`
+ * (used to fill the cache ahead of time).  `arch` e.g. "gfx950". */
+int symhip_compile_to_file(const char* source, const char* options, const char* arch, const char* out_path);
+
+/* Upload input container i from host memory (reference: setArg).  n must equal its length. */
+int symhip_set_arg(symhip_t h, int i, const double* host, size_t n);
+/* Same from device memory, asynchronously on `stream` (hipStream_t, NULL = default). */
+int symhip_set_arg_device(symhip_t h, int i, const double* dev, size_t n, void* stream);
+
+/* Enqueue every launch on `stream` (reference: run()).  Does not synchronise. */
+int symhip_run(symhip_t h, void* stream);
+/* Enqueue launch `idx` only (for per-kernel timing). */
+int symhip_launch_one(symhip_t h, uint32_t idx, void* stream);
+uint32_t symhip_num_launches(symhip_t h);
+
+/* Copy output container k to host memory after synchronising (reference: getResult). */
+int symhip_get_result(symhip_t h, int k, double* host, size_t n);
+/* Copy output container k to device memory, asynchronously on `stream`. */
+int symhip_get_result_device(symhip_t h, int k, double* dev, size_t n, void* stream);
+
+/* Device address of output container k / of the x workspace (valid until destroy). */
+const double* symhip_output_device_ptr(symhip_t h, int k);
+double* symhip_values_device_ptr(symhip_t h);
+
+/* Free everything (reference: finish).  Accepts NULL. */
+void symhip_destroy(symhip_t h);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* symhip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYMHIP_H */

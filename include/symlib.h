@@ -1,0 +1,92 @@
+/*
+ * symlib.h - C ABI over the C++ ComputeUnit for the three benchmark workloads, for callers
+ * that cannot instantiate the C++ templates (Python ctypes, other FFIs).
+ *
+ * Each symlib_unit_* constructor runs the workload's symbolic front-end exactly as the
+ * reference's example programs do and then constructs
+ *     Sym::ComputeUnit<double>(Sym::Device(Sym::UseHIP(), Sym::ThreadsPerBlock(t)), in, out)
+ * (reference surface: src/ComputeUnit.hpp:192-222):
+ *   symlib_unit_ata_plus_a   examples/Tutorial/main.cpp:36-68     in: A values, out: A^T A + A
+ *   symlib_unit_cotan        examples/BuildCotan/main.cpp:39-121  in: V (n x 3), out: L
+ *   symlib_unit_bw_hessian   examples/BaraffWitkinAD/main.cpp:88-113 (+ bending stiffness)
+ *                                                                  in: X (n x 3), out: H
+ * symlib_unit_compile / _execute / _get_results are ComputeUnit::compile / execute /
+ * getResults.  Dense n x 3 inputs are column-major (x of all vertices, then y, then z).
+ * Output containers are CSC matrices; their pattern is available before compilation.
+ *
+ * All functions return 0 on success and a non-zero code otherwise; symlib_last_error()
+ * then holds the message.  No function throws across the ABI.
+ */
+#ifndef SYMLIB_H
+#define SYMLIB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "symhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct symlib_unit* symlib_unit_t;
+
+typedef struct {
+    int threads_per_block;       /* 0 = default (256) */
+    int decomposition_threshold; /* < 0 = default (6) */
+    int simplify;                /* 1 = simplify kernel templates (default), 0 = skip */
+    int share_templates;         /* 1 = one differentiated template per element shape (default) */
+} symlib_options;
+
+typedef struct {
+    uint64_t num_inputs;        /* input values */
+    uint64_t num_intermediates; /* FP64 intermediates written and re-read */
+    uint64_t num_index;         /* u32 gather entries */
+    uint64_t num_const;         /* FP64 per-instance constants */
+    uint64_t num_outpos;        /* u32 placement entries */
+    uint64_t num_results;       /* output values */
+    uint32_t num_kernels;
+    uint32_t num_waves;
+    uint64_t source_bytes;
+    double frontend_seconds;    /* symbolic front-end (building the expressions) */
+    double build_seconds;       /* ComputeUnit construction (decompose .. emit) */
+    double compile_seconds;     /* hipRTC + table upload */
+} symlib_unit_info;
+
+void symlib_default_options(symlib_options* opt);
+
+int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt);
+int symlib_unit_cotan(symlib_unit_t* out, int num_vertices, int num_faces, const int* faces, const symlib_options* opt);
+int symlib_unit_bw_hessian(symlib_unit_t* out, int num_vertices, int num_faces, const int* faces, const double* rest,
+                           double k_stretch, double k_shear, double k_bend, const symlib_options* opt);
+
+int symlib_unit_compile(symlib_unit_t u);
+/* inputs[i] has lengths[i] values; one entry per input container, in order */
+int symlib_unit_execute(symlib_unit_t u, int num_inputs, const double* const* inputs, const size_t* lengths);
+/* enqueue the program on a stream without copying inputs (they are already on the device) */
+int symlib_unit_run(symlib_unit_t u, void* stream);
+int symlib_unit_get_results(symlib_unit_t u, int k, double* out, size_t n);
+
+int symlib_unit_num_outputs(symlib_unit_t u);
+/* rows, cols, nnz of output container k */
+int symlib_unit_output_shape(symlib_unit_t u, int k, int* rows, int* cols, uint64_t* nnz);
+/* CSC outer (cols + 1) and inner (nnz) arrays of output container k */
+int symlib_unit_output_pattern(symlib_unit_t u, int k, int* outer, int* inner);
+int symlib_unit_get_info(symlib_unit_t u, symlib_unit_info* info);
+/* emitted HIP source (valid until destroy) */
+const char* symlib_unit_source(symlib_unit_t u);
+/* runtime handle for device-side access (NULL before compile) */
+symhip_t symlib_unit_runtime(symlib_unit_t u);
+void symlib_unit_destroy(symlib_unit_t u);
+
+/* Known answers of examples/Tutorial/main.cpp:22-33: evaluate(a + b sqrt(ab)) and both
+ * derivatives at (a, b) = (1, 2); host only. */
+int symlib_tutorial_known_answers(double out[3]);
+
+const char* symlib_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYMLIB_H */

@@ -1,0 +1,214 @@
+"""FP64 oracle (TEST INFRASTRUCTURE ONLY).
+
+ctypes wrapper over oracle/liboracle.so (oracle/oracle.c).  Only tests/, the smoke check in
+__graft_entry__.py and the CPU-baseline leg of bench.py import this module; the product
+package symboliclib_amd never does.
+
+Every function returns CSC arrays (indptr, indices) plus values and per-value scales
+scale = sum of |terms|, the denominator of the relative error used by the parity tests.
+Reference files restated by the C code are cited in oracle.c.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        L.oracle_ata_plus_a_pattern.restype = ctypes.c_long
+        L.oracle_ata_plus_a_pattern.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_void_p]
+        L.oracle_ata_plus_a_values.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p, ctypes.c_void_p]
+        L.oracle_cotan_pattern.restype = ctypes.c_long
+        L.oracle_cotan_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, ctypes.c_void_p]
+        L.oracle_cotan_values.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _f64p, _f64p, ctypes.c_void_p]
+        L.oracle_bw_pattern.restype = ctypes.c_long
+        L.oracle_bw_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, ctypes.c_void_p]
+        L.oracle_bw_energy.restype = ctypes.c_double
+        L.oracle_bw_energy.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _f64p, _f64p, ctypes.c_double, ctypes.c_double]
+        L.oracle_bw_gradient.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _f64p, _f64p, ctypes.c_double,
+                                         ctypes.c_double, _f64p]
+        L.oracle_interior_edges.restype = ctypes.c_long
+        L.oracle_interior_edges.argtypes = [ctypes.c_int, _i32p, ctypes.c_void_p]
+        L.oracle_bend_Q.argtypes = [ctypes.c_int, _i32p, _f64p, _f64p]
+        L.oracle_bw_plan_create.restype = ctypes.c_void_p
+        L.oracle_bw_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _f64p, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double]
+        L.oracle_bw_plan_nnz.restype = ctypes.c_long
+        L.oracle_bw_plan_nnz.argtypes = [ctypes.c_void_p]
+        L.oracle_bw_plan_values.argtypes = [ctypes.c_void_p, _f64p, _f64p, ctypes.c_void_p]
+        L.oracle_bw_plan_free.argtypes = [ctypes.c_void_p]
+        L.oracle_cotan_plan_create.restype = ctypes.c_void_p
+        L.oracle_cotan_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, _i32p]
+        L.oracle_cotan_plan_values.argtypes = [ctypes.c_void_p, _f64p, _f64p, ctypes.c_void_p]
+        L.oracle_cotan_plan_free.argtypes = [ctypes.c_void_p]
+        L.oracle_num_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _colmajor(X):
+    """n x 3 array -> column-major flat (x of all vertices, then y, then z)."""
+    return _f64(np.asarray(X, dtype=np.float64).T.reshape(-1))
+
+
+def num_threads():
+    return lib().oracle_num_threads()
+
+
+def ata_plus_a(indptr, indices, values):
+    """B = A^T A + A for CSC A (n x n)."""
+    L = lib()
+    n = len(indptr) - 1
+    cp, ri, a = _i32(indptr), _i32(indices), _f64(values)
+    bcp = np.zeros(n + 1, np.int32)
+    nnz = L.oracle_ata_plus_a_pattern(n, cp, ri, bcp, None)
+    bri = np.zeros(nnz, np.int32)
+    L.oracle_ata_plus_a_pattern(n, cp, ri, bcp, bri.ctypes.data_as(ctypes.c_void_p))
+    bv = np.zeros(nnz)
+    sc = np.zeros(nnz)
+    L.oracle_ata_plus_a_values(n, cp, ri, a, bcp, bri, bv, sc.ctypes.data_as(ctypes.c_void_p))
+    return bcp, bri, bv, sc
+
+
+def cotan_pattern(nv, F):
+    L = lib()
+    F = _i32(F).reshape(-1)
+    cp = np.zeros(nv + 1, np.int32)
+    nnz = L.oracle_cotan_pattern(nv, len(F) // 3, F, cp, None)
+    ri = np.zeros(nnz, np.int32)
+    L.oracle_cotan_pattern(nv, len(F) // 3, F, cp, ri.ctypes.data_as(ctypes.c_void_p))
+    return cp, ri
+
+
+def cotan(V, F):
+    """Cotangent Laplacian of mesh (V: n x 3, F: m x 3)."""
+    L = lib()
+    nv = len(V)
+    cp, ri = cotan_pattern(nv, F)
+    vals = np.zeros(len(ri))
+    sc = np.zeros(len(ri))
+    Ff = _i32(F).reshape(-1)
+    L.oracle_cotan_values(nv, len(Ff) // 3, Ff, _colmajor(V), vals, sc.ctypes.data_as(ctypes.c_void_p))
+    return cp, ri, vals, sc
+
+
+def bw_pattern(nv, F, with_bend=True):
+    L = lib()
+    F = _i32(F).reshape(-1)
+    cp = np.zeros(3 * nv + 1, np.int32)
+    nnz = L.oracle_bw_pattern(nv, len(F) // 3, F, int(with_bend), cp, None)
+    ri = np.zeros(nnz, np.int32)
+    L.oracle_bw_pattern(nv, len(F) // 3, F, int(with_bend), cp, ri.ctypes.data_as(ctypes.c_void_p))
+    return cp, ri
+
+
+class BWPlan:
+    """Prepared Baraff-Witkin Hessian assembly; values() may be called for many X."""
+
+    def __init__(self, rest, F, k_stretch, k_shear, k_bend):
+        self._F = _i32(F).reshape(-1)
+        self._R = _colmajor(rest)
+        self.nv = len(rest)
+        self.nf = len(self._F) // 3
+        self._p = lib().oracle_bw_plan_create(self.nv, self.nf, self._F, self._R, k_stretch, k_shear, k_bend)
+        self.nnz = lib().oracle_bw_plan_nnz(self._p)
+
+    def values(self, X, with_scale=True):
+        vals = np.zeros(self.nnz)
+        sc = np.zeros(self.nnz) if with_scale else None
+        lib().oracle_bw_plan_values(self._p, _colmajor(X), vals,
+                                    sc.ctypes.data_as(ctypes.c_void_p) if with_scale else None)
+        return vals, sc
+
+    def close(self):
+        if self._p:
+            lib().oracle_bw_plan_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+
+class CotanPlan:
+    def __init__(self, nv, F):
+        self._F = _i32(F).reshape(-1)
+        self.nv = nv
+        self._p = lib().oracle_cotan_plan_create(nv, len(self._F) // 3, self._F)
+
+    def values(self, V, nnz, with_scale=False):
+        vals = np.zeros(nnz)
+        sc = np.zeros(nnz) if with_scale else None
+        lib().oracle_cotan_plan_values(self._p, _colmajor(V), vals,
+                                       sc.ctypes.data_as(ctypes.c_void_p) if with_scale else None)
+        return vals, sc
+
+    def close(self):
+        if self._p:
+            lib().oracle_cotan_plan_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+
+def bw_hessian(rest, F, X, k_stretch, k_shear, k_bend):
+    """Full symmetric Hessian of the stretch + shear energy plus k_bend * Q per interior edge."""
+    cp, ri = bw_pattern(len(rest), F, with_bend=k_bend != 0.0)
+    plan = BWPlan(rest, F, k_stretch, k_shear, k_bend)
+    vals, sc = plan.values(X)
+    plan.close()
+    assert len(vals) == len(ri)
+    return cp, ri, vals, sc
+
+
+def bw_energy(rest, F, X, k_stretch, k_shear):
+    Ff = _i32(F).reshape(-1)
+    return lib().oracle_bw_energy(len(rest), len(Ff) // 3, Ff, _colmajor(rest), _colmajor(X), k_stretch, k_shear)
+
+
+def bw_gradient(rest, F, X, k_stretch, k_shear):
+    """Gradient (length 3n, index vertex + coord * n) of the stretch + shear energy."""
+    Ff = _i32(F).reshape(-1)
+    g = np.zeros(3 * len(rest))
+    lib().oracle_bw_gradient(len(rest), len(Ff) // 3, Ff, _colmajor(rest), _colmajor(X), k_stretch, k_shear, g)
+    return g
+
+
+def interior_edges(F):
+    Ff = _i32(F).reshape(-1)
+    ne = lib().oracle_interior_edges(len(Ff) // 3, Ff, None)
+    E4 = np.zeros(4 * ne, np.int32)
+    lib().oracle_interior_edges(len(Ff) // 3, Ff, E4.ctypes.data_as(ctypes.c_void_p))
+    return E4.reshape(-1, 4)
+
+
+def bend_Q(rest, quad):
+    Q = np.zeros(16)
+    lib().oracle_bend_Q(len(rest), _i32(quad), _colmajor(rest), Q)
+    return Q.reshape(4, 4)
+
+
+def rel_errors(values, ref_values, scale):
+    """Per-value |v - v_ref| / scale (scale 0 -> absolute error)."""
+    scale = np.where(scale > 0, scale, 1.0)
+    return np.abs(np.asarray(values) - np.asarray(ref_values)) / scale

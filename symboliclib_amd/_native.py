@@ -1,0 +1,108 @@
+"""ctypes bindings of the native library (symboliclib_amd/lib/libsymlib.so).
+
+The library is loaded from this package's lib/ folder only; when it is missing the import
+fails with an error instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsymlib.so")
+
+
+class SymlibError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("threads_per_block", ctypes.c_int), ("decomposition_threshold", ctypes.c_int),
+                ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int)]
+
+
+class UnitInfo(ctypes.Structure):
+    _fields_ = [("num_inputs", ctypes.c_uint64), ("num_intermediates", ctypes.c_uint64),
+                ("num_index", ctypes.c_uint64), ("num_const", ctypes.c_uint64), ("num_outpos", ctypes.c_uint64),
+                ("num_results", ctypes.c_uint64), ("num_kernels", ctypes.c_uint32), ("num_waves", ctypes.c_uint32),
+                ("source_bytes", ctypes.c_uint64), ("frontend_seconds", ctypes.c_double),
+                ("build_seconds", ctypes.c_double), ("compile_seconds", ctypes.c_double)]
+
+
+class Launch(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32)]
+
+
+class ProgramDesc(ctypes.Structure):
+    _fields_ = [("num_values", ctypes.c_uint64), ("num_args", ctypes.c_uint32),
+                ("arg_start", ctypes.POINTER(ctypes.c_uint64)), ("arg_length", ctypes.POINTER(ctypes.c_uint64)),
+                ("num_outputs", ctypes.c_uint32), ("out_start", ctypes.POINTER(ctypes.c_uint64)),
+                ("out_length", ctypes.POINTER(ctypes.c_uint64)), ("num_index", ctypes.c_uint64),
+                ("index_data", ctypes.POINTER(ctypes.c_uint32)), ("num_const", ctypes.c_uint64),
+                ("const_data", ctypes.POINTER(ctypes.c_double)), ("num_outpos", ctypes.c_uint64),
+                ("outpos_data", ctypes.POINTER(ctypes.c_uint32)), ("num_launches", ctypes.c_uint32),
+                ("launches", ctypes.POINTER(Launch))]
+
+
+# symbol -> (restype, argtypes); every exported C function of include/symhip.h and include/symlib.h
+_VP, _I, _U32, _U64, _SZ, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double
+_IP = ctypes.POINTER(ctypes.c_int)
+_DP = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    # symhip.h
+    "symhip_device_available": (_I, []),
+    "symhip_create": (_I, [ctypes.POINTER(_VP), ctypes.POINTER(ProgramDesc)]),
+    "symhip_compile": (_I, [_VP, ctypes.c_char_p, ctypes.c_char_p]),
+    "symhip_compile_to_file": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "symhip_set_arg": (_I, [_VP, _I, _DP, _SZ]),
+    "symhip_set_arg_device": (_I, [_VP, _I, _VP, _SZ, _VP]),
+    "symhip_run": (_I, [_VP, _VP]),
+    "symhip_launch_one": (_I, [_VP, _U32, _VP]),
+    "symhip_num_launches": (_U32, [_VP]),
+    "symhip_get_result": (_I, [_VP, _I, _DP, _SZ]),
+    "symhip_get_result_device": (_I, [_VP, _I, _VP, _SZ, _VP]),
+    "symhip_output_device_ptr": (_VP, [_VP, _I]),
+    "symhip_values_device_ptr": (_VP, [_VP]),
+    "symhip_destroy": (None, [_VP]),
+    "symhip_last_error": (ctypes.c_char_p, []),
+    # symlib.h
+    "symlib_default_options": (None, [ctypes.POINTER(Options)]),
+    "symlib_unit_ata_plus_a": (_I, [ctypes.POINTER(_VP), _I, _VP, _VP, ctypes.POINTER(Options)]),
+    "symlib_unit_cotan": (_I, [ctypes.POINTER(_VP), _I, _I, _VP, ctypes.POINTER(Options)]),
+    "symlib_unit_bw_hessian": (_I, [ctypes.POINTER(_VP), _I, _I, _VP, _VP, _D, _D, _D, ctypes.POINTER(Options)]),
+    "symlib_unit_compile": (_I, [_VP]),
+    "symlib_unit_execute": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_SZ)]),
+    "symlib_unit_run": (_I, [_VP, _VP]),
+    "symlib_unit_get_results": (_I, [_VP, _I, _VP, _SZ]),
+    "symlib_unit_num_outputs": (_I, [_VP]),
+    "symlib_unit_output_shape": (_I, [_VP, _I, _IP, _IP, ctypes.POINTER(_U64)]),
+    "symlib_unit_output_pattern": (_I, [_VP, _I, _VP, _VP]),
+    "symlib_unit_get_info": (_I, [_VP, ctypes.POINTER(UnitInfo)]),
+    "symlib_unit_source": (ctypes.c_char_p, [_VP]),
+    "symlib_unit_runtime": (_VP, [_VP]),
+    "symlib_unit_destroy": (None, [_VP]),
+    "symlib_tutorial_known_answers": (_I, [_DP]),
+    "symlib_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded native library; raises SymlibError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SymlibError(f"native library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        L = lib()
+        msg = (L.symlib_last_error() or b"").decode() or (L.symhip_last_error() or b"").decode()
+        raise SymlibError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,245 @@
+// C ABI over ComputeUnit for the benchmark workloads (symlib.h).
+#include "symlib.h"
+
+#include <chrono>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../sym/ComputeUnit.hpp"
+#include "../sym/SymbolicDifferentiation.hpp"
+#include "../workloads/Workloads.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+using Clock = std::chrono::steady_clock;
+double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    } catch (...) {
+        g_err = "unknown C++ exception";
+        return 1;
+    }
+}
+
+Sym::Device makeDevice(const symlib_options* opt) {
+    symlib_options o;
+    symlib_default_options(&o);
+    if (opt) o = *opt;
+    Sym::Device d(Sym::UseHIP(), Sym::ThreadsPerBlock(o.threads_per_block > 0 ? o.threads_per_block : 256));
+    if (o.decomposition_threshold >= 0) d.decompositionThreshold = (unsigned)o.decomposition_threshold;
+    d.simplifyTemplates = o.simplify != 0;
+    return d;
+}
+
+}  // namespace
+
+struct symlib_unit {
+    std::unique_ptr<Sym::ComputeUnit<double>> unit;
+    std::vector<Sym::SparseMatrix<double>> patterns;  // output patterns (values unused)
+    symlib_unit_info info{};
+};
+
+namespace {
+
+template <class... Containers>
+void finishUnit(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, Clock::time_point t1,
+                const Containers&... cs) {
+    u->unit.reset(new Sym::ComputeUnit<double>(dev, cs...));
+    const Clock::time_point t2 = Clock::now();
+    const auto& P = u->unit->getProgram();
+    u->info.num_inputs = P.numInputs;
+    u->info.num_intermediates = P.numIntermediates;
+    u->info.num_index = P.index.size();
+    u->info.num_const = P.consts.size();
+    u->info.num_outpos = P.outpos.size();
+    u->info.num_results = P.numResults();
+    u->info.num_kernels = (uint32_t)P.kernels.size();
+    u->info.num_waves = (uint32_t)P.numWaves;
+    u->info.source_bytes = u->unit->getCode().size();
+    u->info.frontend_seconds = seconds(t0, t1);
+    u->info.build_seconds = seconds(t1, t2);
+}
+
+Sym::workloads::Faces toFaces(int nf, const int* faces) {
+    Sym::workloads::Faces F((size_t)nf);
+    for (int i = 0; i < nf; ++i) F[i] = {faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]};
+    return F;
+}
+
+Sym::SparseMatrix<double> patternOf(const Sym::SparseMatrix<Sym::Symbolic>& S) {
+    Sym::SparseMatrix<double> P(S.rows(), S.cols());
+    P.setPattern(S.outer(), S.inner());
+    return P;
+}
+
+void checkFaces(int nv, int nf, const int* faces) {
+    if (nv <= 0 || nf <= 0 || !faces) throw std::invalid_argument("empty mesh");
+    for (long i = 0; i < 3L * nf; ++i)
+        if (faces[i] < 0 || faces[i] >= nv) throw std::invalid_argument("face index out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* symlib_last_error(void) { return g_err.c_str(); }
+
+void symlib_default_options(symlib_options* opt) {
+    if (!opt) return;
+    opt->threads_per_block = 256;
+    opt->decomposition_threshold = -1;
+    opt->simplify = 1;
+    opt->share_templates = 1;
+}
+
+int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {
+    if (!out) return 1;
+    *out = nullptr;
+    return guarded([&] {
+        if (n <= 0 || !colptr || !rowidx) throw std::invalid_argument("empty matrix");
+        for (int j = 0; j < n; ++j)
+            if (colptr[j + 1] < colptr[j]) throw std::invalid_argument("column pointers must not decrease");
+        std::unique_ptr<symlib_unit> u(new symlib_unit());
+        const auto t0 = Clock::now();
+        Sym::SparseMatrix<double> A(n, n, colptr, rowidx, nullptr);
+        for (int j = 0; j < n; ++j)
+            for (int p = colptr[j]; p < colptr[j + 1]; ++p)
+                if (rowidx[p] < 0 || rowidx[p] >= n || (p > colptr[j] && rowidx[p] <= rowidx[p - 1]))
+                    throw std::invalid_argument("row indices must be in range, sorted and unique per column");
+        auto AS = Sym::makeSymbolic(A, 0);
+        auto BS = Sym::workloads::ataPlusA(AS);
+        const auto t1 = Clock::now();
+        finishUnit(u.get(), makeDevice(opt), t0, t1, AS, BS);
+        u->patterns.push_back(patternOf(BS));
+        *out = u.release();
+    });
+}
+
+int symlib_unit_cotan(symlib_unit_t* out, int nv, int nf, const int* faces, const symlib_options* opt) {
+    if (!out) return 1;
+    *out = nullptr;
+    return guarded([&] {
+        checkFaces(nv, nf, faces);
+        std::unique_ptr<symlib_unit> u(new symlib_unit());
+        const auto t0 = Clock::now();
+        Sym::DenseMatrix<double> V(nv, 3);
+        auto VS = Sym::makeSymbolic(V, 0);
+        Sym::SparseMatrix<Sym::Symbolic> LS;
+        Sym::workloads::buildCotan(VS, toFaces(nf, faces), LS);
+        const auto t1 = Clock::now();
+        finishUnit(u.get(), makeDevice(opt), t0, t1, VS, LS);
+        u->patterns.push_back(patternOf(LS));
+        *out = u.release();
+    });
+}
+
+int symlib_unit_bw_hessian(symlib_unit_t* out, int nv, int nf, const int* faces, const double* rest, double k_stretch,
+                           double k_shear, double k_bend, const symlib_options* opt) {
+    if (!out) return 1;
+    *out = nullptr;
+    return guarded([&] {
+        checkFaces(nv, nf, faces);
+        if (!rest) throw std::invalid_argument("rest positions missing");
+        std::unique_ptr<symlib_unit> u(new symlib_unit());
+        const auto t0 = Clock::now();
+        Sym::DenseMatrix<double> R(nv, 3, rest);
+        Sym::DenseMatrix<double> X(nv, 3);
+        auto XS = Sym::makeSymbolic(X, 0);
+        const bool share = opt ? opt->share_templates != 0 : true;
+        auto HS = Sym::workloads::baraffWitkinHessian(XS, R, toFaces(nf, faces), k_stretch, k_shear, k_bend, share);
+        const auto t1 = Clock::now();
+        finishUnit(u.get(), makeDevice(opt), t0, t1, XS, HS);
+        u->patterns.push_back(patternOf(HS));
+        *out = u.release();
+    });
+}
+
+int symlib_unit_compile(symlib_unit_t u) {
+    if (!u) return 1;
+    return guarded([&] {
+        const auto t0 = Clock::now();
+        u->unit->compile();
+        u->info.compile_seconds = seconds(t0, Clock::now());
+    });
+}
+
+int symlib_unit_execute(symlib_unit_t u, int num_inputs, const double* const* inputs, const size_t* lengths) {
+    if (!u) return 1;
+    return guarded([&] {
+        if (num_inputs != (int)u->unit->numInputs()) throw std::invalid_argument("wrong number of input containers");
+        for (int i = 0; i < num_inputs; ++i) u->unit->setInput(i, inputs[i], lengths[i]);
+        u->unit->run(nullptr);
+    });
+}
+
+int symlib_unit_run(symlib_unit_t u, void* stream) {
+    if (!u) return 1;
+    return guarded([&] { u->unit->run(stream); });
+}
+
+int symlib_unit_get_results(symlib_unit_t u, int k, double* out, size_t n) {
+    if (!u) return 1;
+    return guarded([&] { u->unit->getOutput(k, out, n); });
+}
+
+int symlib_unit_num_outputs(symlib_unit_t u) { return u ? (int)u->patterns.size() : 0; }
+
+int symlib_unit_output_shape(symlib_unit_t u, int k, int* rows, int* cols, uint64_t* nnz) {
+    if (!u || k < 0 || k >= (int)u->patterns.size()) {
+        g_err = "bad output index";
+        return 1;
+    }
+    const auto& P = u->patterns[k];
+    if (rows) *rows = P.rows();
+    if (cols) *cols = P.cols();
+    if (nnz) *nnz = P.nonZeros();
+    return 0;
+}
+
+int symlib_unit_output_pattern(symlib_unit_t u, int k, int* outer, int* inner) {
+    if (!u || k < 0 || k >= (int)u->patterns.size()) {
+        g_err = "bad output index";
+        return 1;
+    }
+    const auto& P = u->patterns[k];
+    if (outer) std::copy(P.outer().begin(), P.outer().end(), outer);
+    if (inner) std::copy(P.inner().begin(), P.inner().end(), inner);
+    return 0;
+}
+
+int symlib_unit_get_info(symlib_unit_t u, symlib_unit_info* info) {
+    if (!u || !info) return 1;
+    *info = u->info;
+    return 0;
+}
+
+const char* symlib_unit_source(symlib_unit_t u) { return u ? u->unit->getCode().c_str() : ""; }
+
+symhip_t symlib_unit_runtime(symlib_unit_t u) { return u ? u->unit->runtime() : nullptr; }
+
+void symlib_unit_destroy(symlib_unit_t u) { delete u; }
+
+int symlib_tutorial_known_answers(double out[3]) {
+    return guarded([&] {
+        using namespace Sym;
+        Symbolic a(0, 0), b(1, 0);
+        Symbolic c = a + b * sqrt(a * b);
+        std::vector<double> vals{1., 2.};
+        out[0] = evaluate(c, vals);
+        auto dc = differentiate(c, std::vector<Symbolic>{a, b});
+        out[1] = evaluate(dc[0], vals);
+        out[2] = evaluate(dc[1], vals);
+    });
+}
+
+}  // extern "C"

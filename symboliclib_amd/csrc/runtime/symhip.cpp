@@ -1,0 +1,333 @@
+// MI355X runtime for compiled expression programs: device buffers, hipRTC compilation with
+// an on-disk code-object cache, and ordered kernel launches.  C ABI declared in symhip.h.
+#include "symhip.h"
+
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../sym/Hashing.hpp"
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string& msg) {
+    g_lastError = msg;
+    return code;
+}
+
+int hipFail(hipError_t e, const char* what) {
+    g_lastError = std::string(what) + ": " + hipGetErrorString(e);
+    return (int)e;
+}
+
+#define HIP_TRY(call, what)                         \
+    do {                                            \
+        hipError_t e_ = (call);                     \
+        if (e_ != hipSuccess) return hipFail(e_, what); \
+    } while (0)
+
+std::string defaultArch() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return "gfx950";
+    }
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "gfx950";
+    std::string a = prop.gcnArchName;
+    auto colon = a.find(':');
+    if (colon != std::string::npos) a = a.substr(0, colon);
+    return a.empty() ? std::string("gfx950") : a;
+}
+
+std::vector<std::string> splitOptions(const char* options) {
+    std::vector<std::string> out;
+    if (!options) return out;
+    std::istringstream ss(options);
+    std::string tok;
+    while (ss >> tok) out.push_back(tok);
+    return out;
+}
+
+// Directory of the code-object cache: $SYMLIB_CACHE_DIR, else .codecache next to the
+// directory holding this library.
+std::string cacheDir() {
+    if (const char* env = std::getenv("SYMLIB_CACHE_DIR")) return env;
+    Dl_info info;
+    if (dladdr((void*)&cacheDir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        auto slash = p.rfind('/');
+        if (slash != std::string::npos) {
+            std::string dir = p.substr(0, slash);  // .../symboliclib_amd/lib
+            auto up = dir.rfind('/');
+            if (up != std::string::npos) return dir.substr(0, up) + "/.codecache";
+        }
+    }
+    return ".codecache";
+}
+
+int compileSource(const std::string& src, const std::vector<std::string>& extra, const std::string& arch,
+                  std::vector<char>& code) {
+    hiprtcProgram prog;
+    hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "symlib_program.hip", 0, nullptr, nullptr);
+    if (r != HIPRTC_SUCCESS) return fail(SYMHIP_E_COMPILE, std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r));
+
+    std::vector<std::string> opts = {"--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                     "-Xclang", "-packed-fp32-ops"};
+    opts.insert(opts.end(), extra.begin(), extra.end());
+    std::vector<const char*> copts;
+    for (auto& o : opts) copts.push_back(o.c_str());
+
+    r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    size_t logSize = 0;
+    hiprtcGetProgramLogSize(prog, &logSize);
+    std::string log(logSize, '\0');
+    if (logSize) hiprtcGetProgramLog(prog, &log[0]);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(SYMHIP_E_COMPILE, std::string("hipRTC compilation failed: ") + hiprtcGetErrorString(r) + "\n" + log);
+    }
+    size_t codeSize = 0;
+    hiprtcGetCodeSize(prog, &codeSize);
+    code.resize(codeSize);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return SYMHIP_SUCCESS;
+}
+
+std::string cacheKey(const std::string& src, const char* options, const std::string& arch) {
+    std::string all = src + "\n//opts:" + (options ? options : "") + "\n//arch:" + arch + "\n//v1";
+    char buf[32];
+    std::snprintf(buf, sizeof(buf), "%016llx", (unsigned long long)Sym::CRC64_hash(all.data(), all.size()));
+    return std::string(buf) + "_" + std::to_string(all.size());
+}
+
+bool readFile(const std::string& path, std::vector<char>& data) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    data.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return !data.empty();
+}
+
+bool writeFileAtomic(const std::string& path, const std::vector<char>& data) {
+    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return false;
+        f.write(data.data(), (std::streamsize)data.size());
+        if (!f) return false;
+    }
+    return std::rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+}  // namespace
+
+struct symhip_program {
+    std::vector<uint64_t> argStart, argLength, outStart, outLength;
+    uint64_t numValues = 0, numResults = 0;
+    double* x = nullptr;
+    double* result = nullptr;
+    double* c = nullptr;
+    uint32_t* p = nullptr;
+    uint32_t* o = nullptr;
+    std::vector<symhip_launch> launches;
+    std::vector<std::string> names;
+    hipModule_t module = nullptr;
+    std::vector<hipFunction_t> functions;
+};
+
+extern "C" {
+
+int symhip_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n > 0 ? 1 : 0;
+}
+
+const char* symhip_last_error(void) { return g_lastError.c_str(); }
+
+void symhip_destroy(symhip_t h) {
+    if (!h) return;
+    if (h->module) hipModuleUnload(h->module);
+    hipFree(h->x);
+    hipFree(h->result);
+    hipFree(h->c);
+    hipFree(h->p);
+    hipFree(h->o);
+    delete h;
+}
+
+int symhip_create(symhip_t* out, const symhip_program_desc* d) {
+    if (!out || !d) return fail(SYMHIP_E_INVALID, "symhip_create: null argument");
+    *out = nullptr;
+    if (!symhip_device_available()) return fail(SYMHIP_E_NO_DEVICE, "symhip_create: no HIP device visible");
+
+    auto* h = new symhip_program();
+    h->numValues = d->num_values;
+    h->argStart.assign(d->arg_start, d->arg_start + d->num_args);
+    h->argLength.assign(d->arg_length, d->arg_length + d->num_args);
+    h->outStart.assign(d->out_start, d->out_start + d->num_outputs);
+    h->outLength.assign(d->out_length, d->out_length + d->num_outputs);
+    for (uint32_t i = 0; i < d->num_args; ++i)
+        if (h->argStart[i] + h->argLength[i] > h->numValues) {
+            delete h;
+            return fail(SYMHIP_E_INVALID, "symhip_create: input container outside the value buffer");
+        }
+    for (uint32_t k = 0; k < d->num_outputs; ++k) h->numResults = std::max(h->numResults, h->outStart[k] + h->outLength[k]);
+    for (uint32_t i = 0; i < d->num_launches; ++i) {
+        h->names.push_back(d->launches[i].name ? d->launches[i].name : "");
+        h->launches.push_back(d->launches[i]);
+    }
+
+    auto upload = [&](void** dst, const void* src, size_t bytes, const char* what) -> int {
+        if (bytes == 0) return SYMHIP_SUCCESS;
+        hipError_t e = hipMalloc(dst, bytes);
+        if (e != hipSuccess) return hipFail(e, what);
+        if (src) {
+            e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hipFail(e, what);
+        } else {
+            e = hipMemset(*dst, 0, bytes);
+            if (e != hipSuccess) return hipFail(e, what);
+        }
+        return SYMHIP_SUCCESS;
+    };
+    int rc;
+    if ((rc = upload((void**)&h->x, nullptr, sizeof(double) * h->numValues, "hipMalloc(values)")) ||
+        (rc = upload((void**)&h->result, nullptr, sizeof(double) * h->numResults, "hipMalloc(results)")) ||
+        (rc = upload((void**)&h->p, d->index_data, sizeof(uint32_t) * d->num_index, "upload(index)")) ||
+        (rc = upload((void**)&h->c, d->const_data, sizeof(double) * d->num_const, "upload(consts)")) ||
+        (rc = upload((void**)&h->o, d->outpos_data, sizeof(uint32_t) * d->num_outpos, "upload(outpos)"))) {
+        symhip_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_compile_to_file(const char* source, const char* options, const char* arch, const char* out_path) {
+    if (!source || !out_path) return fail(SYMHIP_E_INVALID, "symhip_compile_to_file: null argument");
+    std::vector<char> code;
+    int rc = compileSource(source, splitOptions(options), arch ? arch : "gfx950", code);
+    if (rc) return rc;
+    if (!writeFileAtomic(out_path, code)) return fail(SYMHIP_E_IO, std::string("cannot write ") + out_path);
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_compile(symhip_t h, const char* source, const char* options) {
+    if (!h || !source) return fail(SYMHIP_E_INVALID, "symhip_compile: null argument");
+    const std::string arch = defaultArch();
+    const std::string src = source;
+    const std::string dir = cacheDir();
+    const std::string path = dir + "/" + cacheKey(src, options, arch) + "." + arch + ".co";
+
+    std::vector<char> code;
+    if (!readFile(path, code)) {
+        int rc = compileSource(src, splitOptions(options), arch, code);
+        if (rc) return rc;
+        mkdir(dir.c_str(), 0755);
+        writeFileAtomic(path, code);  // cache failures are not fatal
+    }
+
+    if (h->module) {
+        hipModuleUnload(h->module);
+        h->module = nullptr;
+    }
+    h->functions.clear();
+    HIP_TRY(hipModuleLoadData(&h->module, code.data()), "hipModuleLoadData");
+    for (auto& name : h->names) {
+        hipFunction_t f;
+        hipError_t e = hipModuleGetFunction(&f, h->module, name.c_str());
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SYMHIP_E_SYMBOL, "symhip_compile: module has no function " + name);
+        }
+        h->functions.push_back(f);
+    }
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_set_arg(symhip_t h, int i, const double* host, size_t n) {
+    if (!h || i < 0 || (size_t)i >= h->argStart.size()) return fail(SYMHIP_E_INVALID, "symhip_set_arg: bad argument index");
+    if (n != h->argLength[i]) return fail(SYMHIP_E_INVALID, "symhip_set_arg: length does not match the input container");
+    if (!n) return SYMHIP_SUCCESS;
+    if (!host) return fail(SYMHIP_E_INVALID, "symhip_set_arg: null data");
+    HIP_TRY(hipMemcpy(h->x + h->argStart[i], host, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy(setArg)");
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_set_arg_device(symhip_t h, int i, const double* dev, size_t n, void* stream) {
+    if (!h || i < 0 || (size_t)i >= h->argStart.size()) return fail(SYMHIP_E_INVALID, "symhip_set_arg_device: bad argument index");
+    if (n != h->argLength[i]) return fail(SYMHIP_E_INVALID, "symhip_set_arg_device: length mismatch");
+    if (!n) return SYMHIP_SUCCESS;
+    HIP_TRY(hipMemcpyAsync(h->x + h->argStart[i], dev, sizeof(double) * n, hipMemcpyDeviceToDevice, (hipStream_t)stream),
+            "hipMemcpyAsync(setArg)");
+    return SYMHIP_SUCCESS;
+}
+
+uint32_t symhip_num_launches(symhip_t h) { return h ? (uint32_t)h->launches.size() : 0; }
+
+int symhip_launch_one(symhip_t h, uint32_t idx, void* stream) {
+    if (!h || idx >= h->launches.size()) return fail(SYMHIP_E_INVALID, "symhip_launch_one: bad launch index");
+    if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
+    const symhip_launch& L = h->launches[idx];
+    if (L.grid == 0) return SYMHIP_SUCCESS;
+    void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result};
+    HIP_TRY(hipModuleLaunchKernel(h->functions[idx], L.grid, 1, 1, L.block, 1, 1, 0, (hipStream_t)stream, args, nullptr),
+            "hipModuleLaunchKernel");
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_run(symhip_t h, void* stream) {
+    if (!h) return fail(SYMHIP_E_INVALID, "symhip_run: null handle");
+    for (uint32_t i = 0; i < h->launches.size(); ++i) {
+        int rc = symhip_launch_one(h, i, stream);
+        if (rc) return rc;
+    }
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_get_result(symhip_t h, int k, double* host, size_t n) {
+    if (!h || k < 0 || (size_t)k >= h->outStart.size()) return fail(SYMHIP_E_INVALID, "symhip_get_result: bad output index");
+    if (n != h->outLength[k]) return fail(SYMHIP_E_INVALID, "symhip_get_result: length does not match the output container");
+    if (!n) return SYMHIP_SUCCESS;
+    if (!host) return fail(SYMHIP_E_INVALID, "symhip_get_result: null data");
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    HIP_TRY(hipMemcpy(host, h->result + h->outStart[k], sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy(getResult)");
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_get_result_device(symhip_t h, int k, double* dev, size_t n, void* stream) {
+    if (!h || k < 0 || (size_t)k >= h->outStart.size()) return fail(SYMHIP_E_INVALID, "symhip_get_result_device: bad output index");
+    if (n != h->outLength[k]) return fail(SYMHIP_E_INVALID, "symhip_get_result_device: length mismatch");
+    if (!n) return SYMHIP_SUCCESS;
+    HIP_TRY(hipMemcpyAsync(dev, h->result + h->outStart[k], sizeof(double) * n, hipMemcpyDeviceToDevice, (hipStream_t)stream),
+            "hipMemcpyAsync(getResult)");
+    return SYMHIP_SUCCESS;
+}
+
+const double* symhip_output_device_ptr(symhip_t h, int k) {
+    if (!h || k < 0 || (size_t)k >= h->outStart.size()) return nullptr;
+    return h->result + h->outStart[k];
+}
+
+double* symhip_values_device_ptr(symhip_t h) { return h ? h->x : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+#include "ComputeUnit.hpp"
+
+#include <string>
+
+namespace Sym {
+
+ComputeUnitBase::ComputeUnitBase(const Device& d) : device(d) {
+    if (!device.hipDevice)
+        throw std::invalid_argument("ComputeUnit: this build implements the UseHIP device only (add UseHIP() to the Device)");
+    if (device.singlePrecision) throw std::invalid_argument("ComputeUnit: this build evaluates in FP64 only");
+}
+
+ComputeUnitBase::~ComputeUnitBase() { close(); }
+
+void ComputeUnitBase::close() {
+    if (rt) {
+        symhip_destroy(rt);
+        rt = nullptr;
+    }
+}
+
+void ComputeUnitBase::check(int rc, const char* what) const {
+    if (rc != SYMHIP_SUCCESS)
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + symhip_last_error());
+}
+
+// Inputs are containers of variables {0..len-1, g} of one group g; every other container is
+// an output whose k-th value becomes ASSIGN(VAR(k, -10 - container), expr)
+// (src/ComputeUnit.cpp:171-228).
+void ComputeUnitBase::addExpressions(const Symbolic* expr, size_t len, int& id) {
+    if (!len) return;
+    bool isInput = true;
+    for (size_t i = 0; i < len; ++i)
+        if (expr[i].op() != VAR) {
+            isInput = false;
+            break;
+        }
+
+    if (isInput) {
+        auto v0 = expr[0].variable();
+        for (size_t i = 0; i < len; ++i) {
+            if (expr[i].variable()[0] != (int)i)
+                throw std::invalid_argument("ComputeUnit: input variables must be numbered 0..len-1 in container order");
+            if (expr[i].variable()[1] != v0[1])
+                throw std::invalid_argument("ComputeUnit: all variables of an input container must share one group");
+        }
+        if (io.inputOffsets.count(v0[1])) throw std::invalid_argument("ComputeUnit: two input containers use the same group");
+        io.inputSizes.push_back(len);
+        io.inputOffsets[v0[1]] = io.inputDataSize;
+        io.inputLengths[v0[1]] = len;
+        io.inputDataSize += len;
+    } else {
+        for (size_t i = 0; i < len; ++i) {
+            if (expr[i].op() == NOOP) throw std::invalid_argument("ComputeUnit: output container holds an empty expression");
+            outputExpressions.push_back(Symbolic(ASSIGN, Symbolic((int)i, -10 - id), expr[i]));
+        }
+        io.outputSizes.push_back(len);
+        ++id;
+    }
+}
+
+void ComputeUnitBase::init() {
+    if (io.outputSizes.empty()) throw std::invalid_argument("ComputeUnit: no output expressions");
+    BuildOptions opt;
+    opt.decompositionThreshold = (unsigned char)std::min(255u, device.decompositionThreshold);
+    opt.simplifyTemplates = device.simplifyTemplates;
+    program = buildProgram(outputExpressions, io, opt);
+    EmitOptions eo;
+    eo.threadsPerBlock = device.threadsPerBlock();
+    code = emitHip(program, eo, launches);
+    outputExpressions.clear();  // the program holds everything needed from here on
+    outputExpressions.shrink_to_fit();
+}
+
+void ComputeUnitBase::compileProgram() {
+    close();
+    std::vector<uint64_t> argStart(io.inputSizes.size()), argLength(io.inputSizes.size());
+    {
+        size_t acc = 0;
+        for (size_t i = 0; i < io.inputSizes.size(); ++i) {
+            argStart[i] = acc;
+            argLength[i] = io.inputSizes[i];
+            acc += io.inputSizes[i];
+        }
+    }
+    std::vector<symhip_launch> ls;
+    for (auto& L : launches) ls.push_back({L.name.c_str(), L.grid, L.block});
+
+    symhip_program_desc d{};
+    d.num_values = program.numValues();
+    d.num_args = (uint32_t)argStart.size();
+    d.arg_start = argStart.data();
+    d.arg_length = argLength.data();
+    d.num_outputs = (uint32_t)program.outStart.size();
+    d.out_start = program.outStart.data();
+    d.out_length = program.outLength.data();
+    d.num_index = program.index.size();
+    d.index_data = program.index.data();
+    d.num_const = program.consts.size();
+    d.const_data = program.consts.data();
+    d.num_outpos = program.outpos.size();
+    d.outpos_data = program.outpos.data();
+    d.num_launches = (uint32_t)ls.size();
+    d.launches = ls.data();
+
+    symhip_t h = nullptr;
+    check(symhip_create(&h, &d), "symhip_create");
+    int rc = symhip_compile(h, code.c_str(), nullptr);
+    if (rc) {
+        const std::string msg = symhip_last_error();
+        symhip_destroy(h);
+        throw std::runtime_error("symhip_compile failed (" + std::to_string(rc) + "): " + msg);
+    }
+    rt = h;
+}
+
+void ComputeUnitBase::setArgImpl(int i, const double* d, size_t n) {
+    if (!rt) throw std::runtime_error("ComputeUnit: execute() before compile()");
+    if (i < 0 || (size_t)i >= io.inputSizes.size()) throw std::invalid_argument("ComputeUnit: too many input containers");
+    if (n != io.inputSizes[i]) throw std::invalid_argument("ComputeUnit: input container " + std::to_string(i) + " has the wrong size");
+    check(symhip_set_arg(rt, i, d, n), "symhip_set_arg");
+}
+
+void ComputeUnitBase::runImpl(void* stream) {
+    if (!rt) throw std::runtime_error("ComputeUnit: run before compile()");
+    check(symhip_run(rt, stream), "symhip_run");
+}
+
+void ComputeUnitBase::getResultImpl(int k, double* d, size_t n) {
+    if (!rt) throw std::runtime_error("ComputeUnit: getResults() before compile()");
+    if (k < 0 || (size_t)k >= io.outputSizes.size()) throw std::invalid_argument("ComputeUnit: too many output containers");
+    if (n != io.outputSizes[k]) throw std::invalid_argument("ComputeUnit: output container " + std::to_string(k) + " has the wrong size");
+    check(symhip_get_result(rt, k, d, n), "symhip_get_result");
+}
+
+}  // namespace Sym

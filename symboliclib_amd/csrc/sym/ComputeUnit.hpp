@@ -1,0 +1,211 @@
+// ComputeUnit: compile a set of symbolic output expressions over symbolic input containers
+// into an MI355X program, then evaluate it for numeric inputs.
+//
+// Drop-in for the reference's UseHIP path (src/ComputeUnit.hpp:14-223):
+//
+//   Sym::ComputeUnit<double> unit(Sym::Device(Sym::UseHIP(), Sym::ThreadsPerBlock(256)), AS, BS);
+//   unit.compile().execute(A).getResults(B);
+//
+// Constructor arguments after the Device are containers in order; containers holding only
+// variables {0..len-1, group} are inputs, all others are outputs.  execute() takes the
+// numeric inputs in the same order, getResults() fills the outputs in the same order.
+// Underneath, the unit calls the C runtime declared in symhip.h: hipRTC compilation in
+// process, device-resident tables, one launch per wave of independent kernels.
+//
+// Differences from the reference: errors throw (std::invalid_argument, std::runtime_error)
+// instead of printing and degrading to no-ops; only the UseHIP device with RealT = double is
+// implemented (the CPU and CUDA back-ends are outside this build).
+#pragma once
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "ContainerSupport.hpp"
+#include "Program.hpp"
+#include "Symbolic.hpp"
+#include "symhip.h"
+
+namespace Sym {
+
+template <class T, class Tag>
+struct NamedType {
+    T x;
+    explicit NamedType(const T& x_) : x(x_) {}
+};
+
+using ThreadsPerBlock = NamedType<int, struct ThreadsPerBlockTag>;
+using NumThreads = NamedType<int, struct NumThreadsTag>;
+using VecWidth = NamedType<int, struct VecWidthTag>;
+using DecompositionThreshold = NamedType<int, struct DecompositionThresholdTag>;
+
+struct UseHIP {};
+struct UseCuda {};
+struct UseSinglePrecision {};
+// Skip algebraic simplification of kernel templates (faster compile of very large templates).
+struct NoSimplify {};
+
+class Device {
+public:
+    unsigned int vectorWidth = 1;
+    unsigned int numThreads = 1;
+    unsigned int decompositionThreshold = 6;
+    bool singlePrecision = false;
+    bool cudaDevice = false;
+    bool hipDevice = false;
+    bool simplifyTemplates = true;
+
+private:
+    void set(const DecompositionThreshold& nt) { decompositionThreshold = (unsigned)nt.x; }
+    void set(const ThreadsPerBlock& nt) { numThreads = (unsigned)nt.x; }
+    void set(const VecWidth& nt) { vectorWidth = (unsigned)nt.x; }
+    void set(const NumThreads& nt) { numThreads = (unsigned)nt.x; }
+    void set(const UseSinglePrecision&) { singlePrecision = true; }
+    void set(const NoSimplify&) { simplifyTemplates = false; }
+    void set(const UseHIP&) {
+        hipDevice = true;
+        cudaDevice = false;
+        vectorWidth = 0;
+    }
+    void set(const UseCuda&) {
+        cudaDevice = true;
+        hipDevice = false;
+        vectorWidth = 0;
+    }
+    template <class T>
+    void set(const T&) {
+        throw std::invalid_argument("Device: parameter not understood");
+    }
+    void init() {}
+    template <class T, class... Args>
+    void init(const T& t, const Args&... args) {
+        set(t);
+        init(args...);
+    }
+
+public:
+    template <class... Args>
+    Device(const Args&... args) {
+        init(args...);
+    }
+
+    int threadsPerBlock() const { return numThreads <= 1 ? 256 : (int)numThreads; }
+};
+
+// Everything of ComputeUnit that does not depend on the scalar type.
+class ComputeUnitBase {
+protected:
+    Device device;
+    std::vector<Symbolic> outputExpressions;
+    ProgramInputs io;
+    Program program;
+    std::string code;
+    std::vector<LaunchDesc> launches;
+    symhip_t rt = nullptr;
+    int numOutputContainers = 0;
+
+    void addExpressions(const Symbolic* expr, size_t len, int& id);
+    void init();
+    void setArgImpl(int i, const double* d, size_t n);
+    void getResultImpl(int k, double* d, size_t n);
+    void runImpl(void* stream);
+    void check(int rc, const char* what) const;
+
+    explicit ComputeUnitBase(const Device& d);
+
+public:
+    ComputeUnitBase(const ComputeUnitBase&) = delete;
+    ComputeUnitBase& operator=(const ComputeUnitBase&) = delete;
+    virtual ~ComputeUnitBase();
+
+    void close();
+    bool compiled() const { return rt != nullptr; }
+
+    const Program& getProgram() const { return program; }
+    const std::string& getCode() const { return code; }
+    const std::vector<LaunchDesc>& getLaunches() const { return launches; }
+    symhip_t runtime() const { return rt; }
+    size_t numInputs() const { return io.inputSizes.size(); }
+    size_t inputSize(int i) const { return io.inputSizes.at(i); }
+    size_t numOutputs() const { return io.outputSizes.size(); }
+    size_t outputSize(int k) const { return io.outputSizes.at(k); }
+
+    // compile (hipRTC), upload tables
+    void compileProgram();
+    // run on a stream without synchronising (inputs already on the device)
+    void run(void* stream = nullptr) { runImpl(stream); }
+};
+
+template <class RealT>
+class ComputeUnit : public ComputeUnitBase {
+    static_assert(std::is_same<RealT, double>::value, "this build evaluates programs in FP64 only (RealT = double)");
+
+    template <class... TArgs>
+    void setExpressions(int) {}
+
+    template <class T, class... TArgs>
+    void setExpressions(int id, const T& A, const TArgs&... args) {
+        addExpressions(contPtr(A), contSize(A), id);
+        setExpressions(id, args...);
+    }
+
+    void setArgs(int) {}
+    template <class T, class... TArg>
+    void setArgs(int id, const T& arg, const TArg&... rest) {
+        setArgImpl(id, contPtr(arg), contSize(arg));
+        setArgs(id + 1, rest...);
+    }
+
+    void getRes(int) {}
+    template <class T, class... TArg>
+    void getRes(int id, T& arg, TArg&... rest) {
+        getResultImpl(id, contPtr(arg), contSize(arg));
+        getRes(id + 1, rest...);
+    }
+
+public:
+    template <class... TArg>
+    ComputeUnit(Device device_, const TArg&... expressions) : ComputeUnitBase(device_) {
+        int id = 0;
+        setExpressions(id, expressions...);
+        init();
+    }
+
+    ComputeUnit& compile() {
+        compileProgram();
+        return *this;
+    }
+
+    template <class... TArg>
+    ComputeUnit& execute(const TArg&... inputs) {
+        setArgs(0, inputs...);
+        runImpl(nullptr);
+        return *this;
+    }
+
+    template <class TArg>
+    ComputeUnit& getResults(TArg& out, const int i) {
+        getResultImpl(i, contPtr(out), contSize(out));
+        return *this;
+    }
+
+    template <class... TArg>
+    ComputeUnit& getResults(TArg&... outs) {
+        getRes(0, outs...);
+        return *this;
+    }
+
+    // pointer variants for bindings
+    ComputeUnit& setInput(int i, const double* d, size_t n) {
+        setArgImpl(i, d, n);
+        return *this;
+    }
+    ComputeUnit& getOutput(int k, double* d, size_t n) {
+        getResultImpl(k, d, n);
+        return *this;
+    }
+};
+
+}  // namespace Sym

@@ -1,0 +1,142 @@
+"""Python mirror of ComputeUnit for the three workloads, over the C ABI of include/symlib.h.
+
+    unit = Unit.ata_plus_a(indptr, indices)          # construct: symbolic front-end + ComputeUnit
+    unit.compile()                                   # hipRTC for the GPU, tables to HBM
+    unit.execute(a_values)                           # inputs in constructor order
+    values = unit.get_results(0)                     # outputs in constructor order
+
+These calls map one to one onto ComputeUnit<double>(Device(UseHIP(), ThreadsPerBlock(t)),
+in, out).compile().execute(in).getResults(out) of the reference (src/ComputeUnit.hpp:192-222).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .workloads import to_colmajor
+
+
+def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True):
+    o = _native.Options()
+    _native.lib().symlib_default_options(ctypes.byref(o))
+    o.threads_per_block = int(threads_per_block)
+    o.decomposition_threshold = int(decomposition_threshold)
+    o.simplify = int(bool(simplify))
+    o.share_templates = int(bool(share_templates))
+    return o
+
+
+class Unit:
+    def __init__(self, handle, input_lengths):
+        self._h = handle
+        self.input_lengths = list(input_lengths)
+        self._keep = []
+
+    # ---- constructors (symbolic front-ends of the reference examples)
+    @classmethod
+    def ata_plus_a(cls, indptr, indices, **opt):
+        L = _native.lib()
+        cp = np.ascontiguousarray(indptr, np.int32)
+        ri = np.ascontiguousarray(indices, np.int32)
+        h = ctypes.c_void_p()
+        o = _opts(**opt)
+        _native.check(L.symlib_unit_ata_plus_a(ctypes.byref(h), len(cp) - 1, cp.ctypes.data, ri.ctypes.data,
+                                               ctypes.byref(o)), "symlib_unit_ata_plus_a")
+        return cls(h, [len(ri)])
+
+    @classmethod
+    def cotan(cls, num_vertices, faces, **opt):
+        L = _native.lib()
+        F = np.ascontiguousarray(faces, np.int32).reshape(-1)
+        h = ctypes.c_void_p()
+        o = _opts(**opt)
+        _native.check(L.symlib_unit_cotan(ctypes.byref(h), num_vertices, len(F) // 3, F.ctypes.data, ctypes.byref(o)),
+                      "symlib_unit_cotan")
+        return cls(h, [3 * num_vertices])
+
+    @classmethod
+    def bw_hessian(cls, rest, faces, k_stretch, k_shear, k_bend, **opt):
+        L = _native.lib()
+        F = np.ascontiguousarray(faces, np.int32).reshape(-1)
+        R = to_colmajor(rest)
+        h = ctypes.c_void_p()
+        o = _opts(**opt)
+        _native.check(L.symlib_unit_bw_hessian(ctypes.byref(h), len(rest), len(F) // 3, F.ctypes.data, R.ctypes.data,
+                                               k_stretch, k_shear, k_bend, ctypes.byref(o)), "symlib_unit_bw_hessian")
+        return cls(h, [3 * len(rest)])
+
+    # ---- ComputeUnit surface
+    def compile(self):
+        _native.check(_native.lib().symlib_unit_compile(self._h), "symlib_unit_compile")
+        return self
+
+    def execute(self, *inputs):
+        arrs = [np.ascontiguousarray(x, np.float64).reshape(-1) for x in inputs]
+        ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        lens = (ctypes.c_size_t * len(arrs))(*[a.size for a in arrs])
+        _native.check(_native.lib().symlib_unit_execute(self._h, len(arrs), ptrs, lens), "symlib_unit_execute")
+        return self
+
+    def run(self, stream=None):
+        _native.check(_native.lib().symlib_unit_run(self._h, stream), "symlib_unit_run")
+        return self
+
+    def get_results(self, k=0):
+        n = self.output_shape(k)[2]
+        out = np.empty(n, np.float64)
+        _native.check(_native.lib().symlib_unit_get_results(self._h, k, out.ctypes.data, n), "symlib_unit_get_results")
+        return out
+
+    # ---- introspection
+    def output_shape(self, k=0):
+        r, c, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        _native.check(_native.lib().symlib_unit_output_shape(self._h, k, ctypes.byref(r), ctypes.byref(c),
+                                                             ctypes.byref(nnz)), "symlib_unit_output_shape")
+        return r.value, c.value, nnz.value
+
+    def output_pattern(self, k=0):
+        r, c, nnz = self.output_shape(k)
+        outer = np.empty(c + 1, np.int32)
+        inner = np.empty(nnz, np.int32)
+        _native.check(_native.lib().symlib_unit_output_pattern(self._h, k, outer.ctypes.data, inner.ctypes.data),
+                      "symlib_unit_output_pattern")
+        return outer, inner
+
+    def info(self):
+        i = _native.UnitInfo()
+        _native.check(_native.lib().symlib_unit_get_info(self._h, ctypes.byref(i)), "symlib_unit_get_info")
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def source(self):
+        return _native.lib().symlib_unit_source(self._h).decode()
+
+    def runtime(self):
+        return _native.lib().symlib_unit_runtime(self._h)
+
+    def num_launches(self):
+        rt = self.runtime()
+        return _native.lib().symhip_num_launches(rt) if rt else 0
+
+    def launch_one(self, idx, stream=None):
+        _native.check(_native.lib().symhip_launch_one(self.runtime(), idx, stream), "symhip_launch_one")
+
+    def close(self):
+        if self._h:
+            _native.lib().symlib_unit_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def device_available():
+    return bool(_native.lib().symhip_device_available())

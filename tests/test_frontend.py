@@ -1,0 +1,132 @@
+"""Host-side checks of the symbolic front-end, compiler middle-end, emitter and C ABI (no GPU).
+
+Output patterns are compared bit-exactly with the oracle; emitted programs are compiled by
+hipRTC for gfx950 (no device needed); error paths must return codes with messages.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from symboliclib_amd import _native
+from symboliclib_amd import workloads as W
+from symboliclib_amd.unit import Unit
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _native.lib()
+    declared = []
+    for h in ("symhip.h", "symlib.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        declared += re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(sym\w+)\s*\(", text, flags=re.M)
+    assert len(declared) >= 30
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(declared) <= set(_native.SIGNATURES), set(declared) - set(_native.SIGNATURES)
+
+
+def test_tutorial_known_answers():
+    # examples/Tutorial/main.cpp:27-33: 3.82843, 2.41421, 2.12132 = 1 + 2 sqrt 2, 1 + sqrt 2, 3 / sqrt 2
+    out = (ctypes.c_double * 3)()
+    _native.check(_native.lib().symlib_tutorial_known_answers(out), "known answers")
+    assert out[0] == pytest.approx(1 + 2 * 2 ** 0.5, rel=1e-15)
+    assert out[1] == pytest.approx(1 + 2 ** 0.5, rel=1e-15)
+    assert out[2] == pytest.approx(3 / 2 ** 0.5, rel=1e-15)
+
+
+@pytest.mark.parametrize("subdiv", [0, 1, 3])
+def test_ata_plus_a_pattern(subdiv):
+    n, cp, ri, a = W.c1_matrix(subdiv=subdiv)
+    with Unit.ata_plus_a(cp, ri) as u:
+        outer, inner = u.output_pattern()
+        bcp, bri, _, _ = O.ata_plus_a(cp, ri, a)
+        assert np.array_equal(outer, bcp) and np.array_equal(inner, bri)
+
+
+def test_sphere_mtx_pattern():
+    n, cp, ri, a = W.load_mtx(os.path.join(ROOT, "tests", "golden", "sphere.mtx"))
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sphere_ata_plus_a.npz"))
+    with Unit.ata_plus_a(cp, ri) as u:
+        outer, inner = u.output_pattern()
+        assert np.array_equal(outer, g["indptr"]) and np.array_equal(inner, g["indices"])
+
+
+@pytest.mark.parametrize("side", [3, 8, 33])
+def test_cotan_pattern(side):
+    V, F = W.c2_mesh(side=side)
+    with Unit.cotan(len(V), F) as u:
+        outer, inner = u.output_pattern()
+        cp, ri = O.cotan_pattern(len(V), F)
+        assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
+        info = u.info()
+        # per-face area and per-corner weight become shared intermediates: three waves
+        assert info["num_waves"] == 3
+        assert info["num_intermediates"] == len(F) + 3 * len(F)
+
+
+@pytest.mark.parametrize("side,share", [(3, True), (5, False), (12, True)])
+def test_bw_pattern(side, share):
+    rest, F, X = W.c3_cloth(side=side)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, share_templates=share) as u:
+        outer, inner = u.output_pattern()
+        cp, ri = O.bw_pattern(len(rest), F)
+        assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
+        info = u.info()
+        if share:
+            # one 45-entry upper-triangle block per face
+            assert info["num_intermediates"] == 45 * len(F)
+
+
+def test_emitted_sources_compile_for_gfx950(tmp_path):
+    L = _native.lib()
+    n, cp, ri, a = W.c1_matrix(subdiv=1)
+    V, F2 = W.c2_mesh(side=5)
+    rest, F3, X = W.c3_cloth(side=4)
+    units = [Unit.ata_plus_a(cp, ri), Unit.cotan(len(V), F2), Unit.bw_hessian(rest, F3, 1e3, 1e2, 1e-2)]
+    for k, u in enumerate(units):
+        src = u.source()
+        out = str(tmp_path / f"prog{k}.co")
+        rc = L.symhip_compile_to_file(src.encode(), None, b"gfx950", out.encode())
+        assert rc == 0, L.symhip_last_error().decode()
+        assert os.path.getsize(out) > 1000
+        u.close()
+
+
+def test_errors_are_reported_not_thrown():
+    L = _native.lib()
+    h = ctypes.c_void_p()
+    bad_faces = np.array([[0, 1, 7]], np.int32)
+    rc = L.symlib_unit_cotan(ctypes.byref(h), 3, 1, bad_faces.ctypes.data, None)
+    assert rc != 0 and b"out of range" in L.symlib_last_error()
+    cp = np.array([0, 2, 3], np.int32)
+    ri = np.array([1, 0, 1], np.int32)  # unsorted rows in column 0
+    rc = L.symlib_unit_ata_plus_a(ctypes.byref(h), 2, cp.ctypes.data, ri.ctypes.data, None)
+    assert rc != 0 and b"sorted" in L.symlib_last_error()
+    # compile errors come back with the compiler log
+    rc = L.symhip_compile_to_file(b"this is not C++", None, b"gfx950", b"/tmp/never_written.co")
+    assert rc == -3 and len(L.symhip_last_error()) > 0
+    # runtime entry points reject null handles
+    assert L.symhip_run(None, None) == -1
+    assert L.symhip_set_arg(None, 0, None, 0) == -1
+
+
+def test_execute_before_compile_fails():
+    n, cp, ri, a = W.c1_matrix(subdiv=0)
+    with Unit.ata_plus_a(cp, ri) as u:
+        with pytest.raises(_native.SymlibError, match="before compile"):
+            u.execute(a)
+
+
+def test_template_sharing_matches_direct_tables():
+    rest, F, X = W.c3_cloth(side=4)
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, share_templates=True) as a, \
+            Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, share_templates=False) as b:
+        ia, ib = a.info(), b.info()
+        assert ia["num_results"] == ib["num_results"]
+        assert ia["num_outpos"] == ib["num_outpos"]
