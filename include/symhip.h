@@ -53,6 +53,7 @@ typedef struct {
     const char* name; /* extern "C" __global__ function in the compiled source */
     uint32_t grid;    /* workgroups */
     uint32_t block;   /* work-items per workgroup */
+    uint32_t shared;  /* dynamic LDS bytes per workgroup */
 } symhip_launch;
 
 /*
@@ -64,7 +65,8 @@ typedef struct {
  *   index     : u32 gather table (positions inside x)
  *   consts    : FP64 per-instance constants
  *   outpos    : u32 placement table (positions inside the result buffer)
- * Generated kernels take (x, x, consts, index, outpos, result) in that order.
+ *   tiles     : u32 tile pool of the tile-fused form (may be empty)
+ * Generated kernels take (x, x, consts, index, outpos, result, tiles) in that order.
  */
 typedef struct {
     uint64_t num_values;
@@ -80,6 +82,8 @@ typedef struct {
     const double* const_data;
     uint64_t num_outpos;
     const uint32_t* outpos_data;
+    uint64_t num_tiles_data;
+    const uint32_t* tiles_data;
     uint32_t num_launches;
     const symhip_launch* launches;
 } symhip_program_desc;

@@ -36,6 +36,8 @@ typedef struct {
     int decomposition_threshold; /* < 0 = default (6) */
     int simplify;                /* 1 = simplify kernel templates (default), 0 = skip */
     int share_templates;         /* 1 = one differentiated template per element shape (default) */
+    int tiled;                   /* 1 = LDS-staged work tiles when possible (default), 0 = wave launches */
+    int tile_lds_bytes;          /* LDS budget per tile, 0 = default (64 KiB) */
 } symlib_options;
 
 typedef struct {
@@ -51,6 +53,9 @@ typedef struct {
     double frontend_seconds;    /* symbolic front-end (building the expressions) */
     double build_seconds;       /* ComputeUnit construction (decompose .. emit) */
     double compile_seconds;     /* hipRTC + table upload */
+    uint32_t tiled;             /* 1 = tile-fused form */
+    uint64_t tile_pool_entries; /* u32 entries of the tile tables */
+    char tiling[128];           /* tile count, LDS bytes and recomputation factor, or why untiled */
 } symlib_unit_info;
 
 void symlib_default_options(symlib_options* opt);

@@ -244,8 +244,9 @@ long oracle_cotan_pattern(int nv, int nf, const int* F, int* out_cp, int* out_ri
     return nnz;
 }
 
-/* corner weights w = (e0 . e1) / |e0 x e1| (BuildCotan/main.cpp:49-70) */
-static void cotan_triplet_values(int nv, int nf, const int* F, const double* V, double* tv) {
+/* corner weights w = (e0 . e1) / |e0 x e1| (BuildCotan/main.cpp:49-70); tm (optional) gets the
+ * magnitude sum |e0_k e1_k| / |e0 x e1| of every weight */
+static void cotan_triplet_values(int nv, int nf, const int* F, const double* V, double* tv, double* tm) {
 #pragma omp parallel for schedule(static)
     for (long f = 0; f < nf; ++f) {
         const int* fv = F + 3 * f;
@@ -261,6 +262,10 @@ static void cotan_triplet_values(int nv, int nf, const int* F, const double* V, 
             tv[t + 1] = -w;
             tv[t + 2] = w;
             tv[t + 3] = w;
+            if (tm) {
+                const double m = (fabs(d0.x * d1.x) + fabs(d0.y * d1.y) + fabs(d0.z * d1.z)) / a;
+                tm[t] = tm[t + 1] = tm[t + 2] = tm[t + 3] = m;
+            }
         }
     }
 }
@@ -272,6 +277,7 @@ typedef struct {
     long* segStart; /* nnz + 1: triplets of slot s are perm[segStart[s] .. segStart[s+1]) */
     long* perm;
     double* tv;
+    double* tm; /* optional magnitude of each triplet value (NULL: |tv|) */
 } assembly;
 
 static void assembly_init(assembly* A, long nt, const int* tr, const int* tc, int ncols, int* cp) {
@@ -281,6 +287,7 @@ static void assembly_init(assembly* A, long nt, const int* tr, const int* tc, in
     A->segStart = (long*)calloc((size_t)A->nnz + 1, sizeof(long));
     A->perm = (long*)malloc(sizeof(long) * (size_t)(nt > 0 ? nt : 1));
     A->tv = (double*)malloc(sizeof(double) * (size_t)(nt > 0 ? nt : 1));
+    A->tm = NULL;
     for (long t = 0; t < nt; ++t) A->segStart[A->slot[t] + 1]++;
     for (long s = 0; s < A->nnz; ++s) A->segStart[s + 1] += A->segStart[s];
     long* fill = (long*)malloc(sizeof(long) * ((size_t)A->nnz + 1));
@@ -294,9 +301,10 @@ static void assembly_sum(const assembly* A, double* vals, double* scale) {
     for (long s = 0; s < A->nnz; ++s) {
         double v = 0.0, sc = 0.0;
         for (long q = A->segStart[s]; q < A->segStart[s + 1]; ++q) {
-            const double t = A->tv[A->perm[q]];
+            const long k = A->perm[q];
+            const double t = A->tv[k];
             v += t;
-            sc += fabs(t);
+            sc += A->tm ? A->tm[k] : fabs(t);
         }
         vals[s] = v;
         if (scale) scale[s] = sc;
@@ -308,6 +316,7 @@ static void assembly_free(assembly* A) {
     free(A->segStart);
     free(A->perm);
     free(A->tv);
+    free(A->tm);
 }
 
 void oracle_cotan_values(int nv, int nf, const int* F, const double* V, double* vals, double* scale) {
@@ -318,7 +327,8 @@ void oracle_cotan_values(int nv, int nf, const int* F, const double* V, double* 
     cotan_triplets_rc(nf, F, tr, tc);
     assembly A;
     assembly_init(&A, nt, tr, tc, nv, cp);
-    cotan_triplet_values(nv, nf, F, V, A.tv);
+    if (scale) A.tm = (double*)malloc(sizeof(double) * (size_t)nt);
+    cotan_triplet_values(nv, nf, F, V, A.tv, A.tm);
     assembly_sum(&A, vals, scale);
     assembly_free(&A);
     free(tr);
@@ -347,7 +357,13 @@ static void bw_rest(int nv, const int* fv, const double* R, double* area, double
 
 /* Hand-derived 9x9 stretch+shear stiffness K9 of one face (stretch_shear_forces_impl.hpp:99-203),
  * row-major, index 3*vertex + coordinate. */
-static void bw_face_K9(int nv, const int* fv, const double* R, const double* X, double ks, double ksh, double K9[81]) {
+/* K9 and, when M9 != NULL, the magnitude of every entry: the same closed form evaluated with
+ * the absolute value of every factor and every additive term (from the edge vectors
+ * D0 = x1 - x0, D1 = x2 - x0 on), i.e. the sum of the absolute magnitudes of the terms the
+ * entry is made of.  Entries such as (1 - Jn_x^2) / |J| cancel internally on flat parts of the
+ * cloth; their magnitude keeps both terms. */
+static void bw_face_K9(int nv, const int* fv, const double* R, const double* X, double ks, double ksh, double K9[81],
+                       double* M9) {
     double a, Vi[4];
     bw_rest(nv, fv, R, &a, Vi);
     double B[3][2];
@@ -367,6 +383,17 @@ static void bw_face_K9(int nv, const int* fv, const double* R, const double* X, 
     const double Jn0[3] = {J0.x / n0, J0.y / n0, J0.z / n0};
     const double Jn1[3] = {J1.x / n1, J1.y / n1, J1.z / n1};
     const double J0a[3] = {J0.x, J0.y, J0.z}, J1a[3] = {J1.x, J1.y, J1.z};
+    /* magnitudes: |D| Vinv with absolute values, norms of those, normalised by the actual norm */
+    const double D0a[3] = {D0.x, D0.y, D0.z}, D1a[3] = {D1.x, D1.y, D1.z};
+    double J0m[3], J1m[3], Jn0m[3], Jn1m[3];
+    for (int r = 0; r < 3; ++r) {
+        J0m[r] = fabs(D0a[r] * Vi[0]) + fabs(D1a[r] * Vi[2]);
+        J1m[r] = fabs(D0a[r] * Vi[1]) + fabs(D1a[r] * Vi[3]);
+        Jn0m[r] = J0m[r] / n0;
+        Jn1m[r] = J1m[r] / n1;
+    }
+    const double n0m = sqrt(J0m[0] * J0m[0] + J0m[1] * J0m[1] + J0m[2] * J0m[2]);
+    const double n1m = sqrt(J1m[0] * J1m[0] + J1m[1] * J1m[1] + J1m[2] * J1m[2]);
 
     /* stretch */
     const double C0 = a * (n0 - 1.0), C1 = a * (n1 - 1.0);
@@ -399,6 +426,23 @@ static void bw_face_K9(int nv, const int* fv, const double* R, const double* X, 
                     double sh = Cdv[r][i] * Cdv[s][j];
                     if (r == s) sh += ddCs * Cs;
                     K9[(3 * i + r) * 9 + 3 * j + s] = -ks * st - ksh * sh;
+                    if (M9) {
+                        const double id = r == s ? 1.0 : 0.0;
+                        const double IJxm = a * (id + Jn0m[r] * Jn0m[s]) / n0;
+                        const double IJym = a * (id + Jn1m[r] * Jn1m[s]) / n1;
+                        const double C0m = a * (n0m + 1.0), C1m = a * (n1m + 1.0);
+                        const double stm = a * a * fabs(B[i][0] * B[j][0]) * Jn0m[r] * Jn0m[s] +
+                                           a * a * fabs(B[i][1] * B[j][1]) * Jn1m[r] * Jn1m[s] +
+                                           fabs(B[i][0] * B[j][0]) * IJxm * C0m + fabs(B[i][1] * B[j][1]) * IJym * C1m;
+                        const double Cdvm_ri = a * (fabs(B[i][0]) * J1m[r] + fabs(B[i][1]) * J0m[r]);
+                        const double Cdvm_sj = a * (fabs(B[j][0]) * J1m[s] + fabs(B[j][1]) * J0m[s]);
+                        double shm = Cdvm_ri * Cdvm_sj;
+                        if (r == s) {
+                            const double Csm = a * (J0m[0] * J1m[0] + J0m[1] * J1m[1] + J0m[2] * J1m[2]);
+                            shm += a * (fabs(B[i][0] * B[j][1]) + fabs(B[i][1] * B[j][0])) * Csm;
+                        }
+                        M9[(3 * i + r) * 9 + 3 * j + s] = fabs(ks) * stm + fabs(ksh) * shm;
+                    }
                 }
         }
 }
@@ -581,9 +625,14 @@ void* oracle_bw_plan_create(int nv, int nf, const int* F, const double* R, doubl
     free(tr);
     free(tc);
     /* bending values are constant */
+    P->A.tm = (double*)malloc(sizeof(double) * (size_t)(nt > 0 ? nt : 1));
     for (long e = 0; e < P->ne; ++e)
         for (int dim = 0; dim < 3; ++dim)
-            for (int k = 0; k < 16; ++k) P->A.tv[81L * nf + 48 * e + 16 * dim + k] = P->Q[16 * e + k];
+            for (int k = 0; k < 16; ++k) {
+                const long t = 81L * nf + 48 * e + 16 * dim + k;
+                P->A.tv[t] = P->Q[16 * e + k];
+                P->A.tm[t] = fabs(P->Q[16 * e + k]);
+            }
     return P;
 }
 
@@ -593,14 +642,18 @@ void oracle_bw_plan_values(void* plan, const double* X, double* vals, double* sc
     bw_plan* P = (bw_plan*)plan;
 #pragma omp parallel for schedule(static)
     for (long f = 0; f < P->nf; ++f) {
-        double K9[81];
-        bw_face_K9(P->nv, P->F + 3 * f, P->R, X, P->ks, P->ksh, K9);
+        double K9[81], M9[81];
+        bw_face_K9(P->nv, P->F + 3 * f, P->R, X, P->ks, P->ksh, K9, scale ? M9 : NULL);
         double* tv = P->A.tv + 81 * f;
+        double* tm = P->A.tm + 81 * f;
         /* triplet (i, j, p, q) -> 2 K9(3i + p, 3j + q) */
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
                 for (int p = 0; p < 3; ++p)
-                    for (int q = 0; q < 3; ++q) tv[27 * i + 9 * j + 3 * p + q] = 2.0 * K9[(3 * i + p) * 9 + 3 * j + q];
+                    for (int q = 0; q < 3; ++q) {
+                        tv[27 * i + 9 * j + 3 * p + q] = 2.0 * K9[(3 * i + p) * 9 + 3 * j + q];
+                        if (scale) tm[27 * i + 9 * j + 3 * p + q] = 2.0 * M9[(3 * i + p) * 9 + 3 * j + q];
+                    }
     }
     assembly_sum(&P->A, vals, scale);
 }
@@ -641,7 +694,8 @@ void* oracle_cotan_plan_create(int nv, int nf, const int* F) {
 
 void oracle_cotan_plan_values(void* plan, const double* V, double* vals, double* scale) {
     cotan_plan* P = (cotan_plan*)plan;
-    cotan_triplet_values(P->nv, P->nf, P->F, V, P->A.tv);
+    if (scale && !P->A.tm) P->A.tm = (double*)malloc(sizeof(double) * (size_t)P->A.nt);
+    cotan_triplet_values(P->nv, P->nf, P->F, V, P->A.tv, scale ? P->A.tm : NULL);
     assembly_sum(&P->A, vals, scale);
 }
 

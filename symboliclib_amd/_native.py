@@ -16,7 +16,8 @@ class SymlibError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("threads_per_block", ctypes.c_int), ("decomposition_threshold", ctypes.c_int),
-                ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int)]
+                ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int), ("tiled", ctypes.c_int),
+                ("tile_lds_bytes", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):
@@ -24,11 +25,13 @@ class UnitInfo(ctypes.Structure):
                 ("num_index", ctypes.c_uint64), ("num_const", ctypes.c_uint64), ("num_outpos", ctypes.c_uint64),
                 ("num_results", ctypes.c_uint64), ("num_kernels", ctypes.c_uint32), ("num_waves", ctypes.c_uint32),
                 ("source_bytes", ctypes.c_uint64), ("frontend_seconds", ctypes.c_double),
-                ("build_seconds", ctypes.c_double), ("compile_seconds", ctypes.c_double)]
+                ("build_seconds", ctypes.c_double), ("compile_seconds", ctypes.c_double), ("tiled", ctypes.c_uint32),
+                ("tile_pool_entries", ctypes.c_uint64), ("tiling", ctypes.c_char * 128)]
 
 
 class Launch(ctypes.Structure):
-    _fields_ = [("name", ctypes.c_char_p), ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32)]
+    _fields_ = [("name", ctypes.c_char_p), ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("shared", ctypes.c_uint32)]
 
 
 class ProgramDesc(ctypes.Structure):
@@ -38,7 +41,8 @@ class ProgramDesc(ctypes.Structure):
                 ("out_length", ctypes.POINTER(ctypes.c_uint64)), ("num_index", ctypes.c_uint64),
                 ("index_data", ctypes.POINTER(ctypes.c_uint32)), ("num_const", ctypes.c_uint64),
                 ("const_data", ctypes.POINTER(ctypes.c_double)), ("num_outpos", ctypes.c_uint64),
-                ("outpos_data", ctypes.POINTER(ctypes.c_uint32)), ("num_launches", ctypes.c_uint32),
+                ("outpos_data", ctypes.POINTER(ctypes.c_uint32)), ("num_tiles_data", ctypes.c_uint64),
+                ("tiles_data", ctypes.POINTER(ctypes.c_uint32)), ("num_launches", ctypes.c_uint32),
                 ("launches", ctypes.POINTER(Launch))]
 
 

@@ -2,6 +2,7 @@
 #include "symlib.h"
 
 #include <chrono>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -38,6 +39,8 @@ Sym::Device makeDevice(const symlib_options* opt) {
     Sym::Device d(Sym::UseHIP(), Sym::ThreadsPerBlock(o.threads_per_block > 0 ? o.threads_per_block : 256));
     if (o.decomposition_threshold >= 0) d.decompositionThreshold = (unsigned)o.decomposition_threshold;
     d.simplifyTemplates = o.simplify != 0;
+    d.tiled = o.tiled != 0;
+    if (o.tile_lds_bytes > 0) d.tileLdsBytes = (unsigned)o.tile_lds_bytes;
     return d;
 }
 
@@ -66,6 +69,9 @@ void finishUnit(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, Cl
     u->info.num_kernels = (uint32_t)P.kernels.size();
     u->info.num_waves = (uint32_t)P.numWaves;
     u->info.source_bytes = u->unit->getCode().size();
+    u->info.tiled = u->unit->isTiled() ? 1 : 0;
+    u->info.tile_pool_entries = u->unit->tilePoolSize();
+    std::snprintf(u->info.tiling, sizeof(u->info.tiling), "%s", u->unit->tilingInfo().c_str());
     u->info.frontend_seconds = seconds(t0, t1);
     u->info.build_seconds = seconds(t1, t2);
 }
@@ -100,6 +106,8 @@ void symlib_default_options(symlib_options* opt) {
     opt->decomposition_threshold = -1;
     opt->simplify = 1;
     opt->share_templates = 1;
+    opt->tiled = 1;
+    opt->tile_lds_bytes = 0;
 }
 
 int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {

@@ -144,6 +144,7 @@ struct symhip_program {
     double* c = nullptr;
     uint32_t* p = nullptr;
     uint32_t* o = nullptr;
+    uint32_t* tiles = nullptr;
     std::vector<symhip_launch> launches;
     std::vector<std::string> names;
     hipModule_t module = nullptr;
@@ -171,6 +172,7 @@ void symhip_destroy(symhip_t h) {
     hipFree(h->c);
     hipFree(h->p);
     hipFree(h->o);
+    hipFree(h->tiles);
     delete h;
 }
 
@@ -214,7 +216,8 @@ int symhip_create(symhip_t* out, const symhip_program_desc* d) {
         (rc = upload((void**)&h->result, nullptr, sizeof(double) * h->numResults, "hipMalloc(results)")) ||
         (rc = upload((void**)&h->p, d->index_data, sizeof(uint32_t) * d->num_index, "upload(index)")) ||
         (rc = upload((void**)&h->c, d->const_data, sizeof(double) * d->num_const, "upload(consts)")) ||
-        (rc = upload((void**)&h->o, d->outpos_data, sizeof(uint32_t) * d->num_outpos, "upload(outpos)"))) {
+        (rc = upload((void**)&h->o, d->outpos_data, sizeof(uint32_t) * d->num_outpos, "upload(outpos)")) ||
+        (rc = upload((void**)&h->tiles, d->tiles_data, sizeof(uint32_t) * d->num_tiles_data, "upload(tiles)"))) {
         symhip_destroy(h);
         return rc;
     }
@@ -261,6 +264,11 @@ int symhip_compile(symhip_t h, const char* source, const char* options) {
         }
         h->functions.push_back(f);
     }
+    for (size_t i = 0; i < h->launches.size(); ++i)
+        if (h->launches[i].shared > 65536)
+            HIP_TRY(hipFuncSetAttribute((const void*)h->functions[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)h->launches[i].shared),
+                    "hipFuncSetAttribute(max dynamic LDS)");
     return SYMHIP_SUCCESS;
 }
 
@@ -289,8 +297,9 @@ int symhip_launch_one(symhip_t h, uint32_t idx, void* stream) {
     if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
     const symhip_launch& L = h->launches[idx];
     if (L.grid == 0) return SYMHIP_SUCCESS;
-    void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result};
-    HIP_TRY(hipModuleLaunchKernel(h->functions[idx], L.grid, 1, 1, L.block, 1, 1, 0, (hipStream_t)stream, args, nullptr),
+    void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result, &h->tiles};
+    HIP_TRY(hipModuleLaunchKernel(h->functions[idx], L.grid, 1, 1, L.block, 1, 1, L.shared, (hipStream_t)stream, args,
+                                  nullptr),
             "hipModuleLaunchKernel");
     return SYMHIP_SUCCESS;
 }

@@ -1,6 +1,9 @@
 #include "ComputeUnit.hpp"
 
+#include <sstream>
 #include <string>
+
+#include "Tiling.hpp"
 
 namespace Sym {
 
@@ -67,7 +70,25 @@ void ComputeUnitBase::init() {
     program = buildProgram(outputExpressions, io, opt);
     EmitOptions eo;
     eo.threadsPerBlock = device.threadsPerBlock();
-    code = emitHip(program, eo, launches);
+    tiledForm = false;
+    if (device.tiled) {
+        TileOptions to;
+        to.ldsBytes = device.tileLdsBytes;
+        to.threadsPerBlock = eo.threadsPerBlock;
+        TiledProgram tp = buildTiles(program, to);
+        if (tp.valid) {
+            code = emitHipTiled(program, tp, eo, launches);
+            tilePool = std::move(tp.pool);
+            tiledForm = true;
+            std::ostringstream note;
+            note << "tiles=" << tp.numTiles << " lds=" << tp.ldsBytes << " recompute="
+                 << (tp.uniqueIntermediates ? (double)tp.tiledIntermediates / tp.uniqueIntermediates : 1.0);
+            tilingNote = note.str();
+        } else {
+            tilingNote = "untiled: " + tp.reason;
+        }
+    }
+    if (!tiledForm) code = emitHip(program, eo, launches);
     outputExpressions.clear();  // the program holds everything needed from here on
     outputExpressions.shrink_to_fit();
 }
@@ -84,7 +105,7 @@ void ComputeUnitBase::compileProgram() {
         }
     }
     std::vector<symhip_launch> ls;
-    for (auto& L : launches) ls.push_back({L.name.c_str(), L.grid, L.block});
+    for (auto& L : launches) ls.push_back({L.name.c_str(), L.grid, L.block, L.shared});
 
     symhip_program_desc d{};
     d.num_values = program.numValues();
@@ -98,8 +119,15 @@ void ComputeUnitBase::compileProgram() {
     d.index_data = program.index.data();
     d.num_const = program.consts.size();
     d.const_data = program.consts.data();
-    d.num_outpos = program.outpos.size();
+    d.num_outpos = tiledForm ? 0 : program.outpos.size();
     d.outpos_data = program.outpos.data();
+    d.num_tiles_data = tilePool.size();
+    d.tiles_data = tilePool.data();
+    if (tiledForm) {
+        // tiles read inputs, constants and their own tables only
+        d.num_values = program.numInputs;
+        d.num_index = 0;
+    }
     d.num_launches = (uint32_t)ls.size();
     d.launches = ls.data();
 

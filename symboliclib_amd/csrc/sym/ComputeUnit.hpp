@@ -46,6 +46,10 @@ struct UseCuda {};
 struct UseSinglePrecision {};
 // Skip algebraic simplification of kernel templates (faster compile of very large templates).
 struct NoSimplify {};
+// Execution form: LDS-staged work tiles (default when every output's closure fits a tile),
+// or one launch per wave of kernels with intermediates in HBM.
+struct UseWaves {};
+using TileLDSBytes = NamedType<int, struct TileLDSBytesTag>;
 
 class Device {
 public:
@@ -56,6 +60,8 @@ public:
     bool cudaDevice = false;
     bool hipDevice = false;
     bool simplifyTemplates = true;
+    bool tiled = true;
+    unsigned int tileLdsBytes = 64 * 1024;
 
 private:
     void set(const DecompositionThreshold& nt) { decompositionThreshold = (unsigned)nt.x; }
@@ -64,6 +70,8 @@ private:
     void set(const NumThreads& nt) { numThreads = (unsigned)nt.x; }
     void set(const UseSinglePrecision&) { singlePrecision = true; }
     void set(const NoSimplify&) { simplifyTemplates = false; }
+    void set(const UseWaves&) { tiled = false; }
+    void set(const TileLDSBytes& b) { tileLdsBytes = (unsigned)b.x; }
     void set(const UseHIP&) {
         hipDevice = true;
         cudaDevice = false;
@@ -103,6 +111,9 @@ protected:
     Program program;
     std::string code;
     std::vector<LaunchDesc> launches;
+    std::vector<uint32_t> tilePool;
+    bool tiledForm = false;
+    std::string tilingNote;
     symhip_t rt = nullptr;
     int numOutputContainers = 0;
 
@@ -126,6 +137,9 @@ public:
     const Program& getProgram() const { return program; }
     const std::string& getCode() const { return code; }
     const std::vector<LaunchDesc>& getLaunches() const { return launches; }
+    bool isTiled() const { return tiledForm; }
+    const std::string& tilingInfo() const { return tilingNote; }
+    size_t tilePoolSize() const { return tilePool.size(); }
     symhip_t runtime() const { return rt; }
     size_t numInputs() const { return io.inputSizes.size(); }
     size_t inputSize(int i) const { return io.inputSizes.at(i); }

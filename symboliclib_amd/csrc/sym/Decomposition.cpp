@@ -1,10 +1,14 @@
 #include "Decomposition.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <set>
 #include <unordered_map>
 
+#include "IdMap.hpp"
 #include "Template.hpp"
 #include "Traverse.hpp"
 #include "Utilities.hpp"
@@ -16,6 +20,63 @@ namespace {
 constexpr unsigned char MERGED = 1;
 constexpr unsigned char EXPLICITLY_MERGED = 2;
 constexpr unsigned char FIXED_NODE = 3;
+
+// Children of a node keyed by algebraic hash, kept sorted (iteration in key order, as an
+// ordered map); nodes have few children, so a flat vector beats a tree.
+template <class T>
+class FlatMap {
+    std::vector<std::pair<hash_t, T>> v_;
+
+public:
+    using iterator = typename std::vector<std::pair<hash_t, T>>::iterator;
+    iterator begin() { return v_.begin(); }
+    iterator end() { return v_.end(); }
+    iterator find(hash_t k) {
+        auto it = std::lower_bound(v_.begin(), v_.end(), k, [](const auto& e, hash_t key) { return e.first < key; });
+        return (it != v_.end() && it->first == k) ? it : v_.end();
+    }
+    bool insert(const std::pair<hash_t, T>& e) {
+        auto it = std::lower_bound(v_.begin(), v_.end(), e.first, [](const auto& x, hash_t key) { return x.first < key; });
+        if (it != v_.end() && it->first == e.first) return false;
+        v_.insert(it, e);
+        return true;
+    }
+    void erase(hash_t k) {
+        auto it = find(k);
+        if (it != v_.end()) v_.erase(it);
+    }
+    size_t size() const { return v_.size(); }
+};
+
+// Small identity cache: linear scan while short, hash map beyond.
+class ExtractCache {
+    std::vector<std::pair<long long, std::pair<Symbolic, hash_t>>> small_;
+    std::unordered_map<long long, std::pair<Symbolic, hash_t>, IdentityHash<long long>> big_;
+    bool useBig_ = false;
+
+public:
+    const std::pair<Symbolic, hash_t>* find(long long k) const {
+        if (useBig_) {
+            auto it = big_.find(k);
+            return it == big_.end() ? nullptr : &it->second;
+        }
+        for (auto& e : small_)
+            if (e.first == k) return &e.second;
+        return nullptr;
+    }
+    void put(long long k, const std::pair<Symbolic, hash_t>& v) {
+        if (!useBig_ && small_.size() < 32) {
+            small_.emplace_back(k, v);
+            return;
+        }
+        if (!useBig_) {
+            for (auto& e : small_) big_.emplace(e.first, e.second);
+            small_.clear();
+            useBig_ = true;
+        }
+        big_[k] = v;
+    }
+};
 
 // Rebuild node `y` with new children (keeps the template id of INSTANCE nodes).
 Symbolic rebuild(const Symbolic& y, const std::vector<Symbolic>& childs) {
@@ -33,7 +94,7 @@ public:
     unsigned char complexity = 0;
     hash_t structureHash = 0;
     unsigned int numParents = 0;
-    std::map<hash_t, ExpressionNode*> childs;
+    FlatMap<ExpressionNode*> childs;
 
     ExpressionNode(const Symbolic& fx, int id_) : id(id_), fullx(fx) {}
 
@@ -56,7 +117,7 @@ public:
     }
 
     void addChild(const std::pair<hash_t, ExpressionNode*>& c) {
-        if (childs.insert(c).second) ++c.second->numParents;
+        if (childs.insert(c)) ++c.second->numParents;
     }
 
     // absorb child n into this node's computation
@@ -67,7 +128,7 @@ public:
             n.numParents = 0;
             n.status = EXPLICITLY_MERGED;
         }
-        for (auto& c : n.childs) addChild(c);
+        for (auto& c : n.childs) addChild(c);  // n is never this node
         complexity = (unsigned char)std::min(255, complexity + n.complexity);
     }
 
@@ -75,12 +136,12 @@ public:
         if (x.op() != NOOP) return x;
 
         std::vector<std::pair<Symbolic, hash_t>> stack;
-        std::unordered_map<long long, std::pair<Symbolic, hash_t>, IdentityHash<long long>> cache;
+        ExtractCache cache;
+        std::vector<hash_t> hashes;
 
         auto preFun = [&](const Symbolic& y) {
-            auto itc = cache.find(y.id());
-            if (itc != cache.end()) {
-                stack.push_back(itc->second);
+            if (auto* hit = cache.find(y.id())) {
+                stack.push_back(*hit);
                 return false;
             }
             if (y.op() == FIXED) {
@@ -109,19 +170,19 @@ public:
             if (OpInfos[y.op()].commutative)
                 sortBy(stack.end() - nc, stack.end(), [](const auto& e) { return e.second; });
 
-            std::vector<hash_t> hashes;
-            hashes.reserve(nc + 2);
-            std::vector<Symbolic> ch;
-            ch.reserve(nc);
-            for (size_t i = stack.size() - nc; i < stack.size(); ++i) {
-                ch.push_back(std::move(stack[i].first));
-                hashes.push_back(stack[i].second);
-            }
+            hashes.clear();
+            const size_t first = stack.size() - nc;
+            for (size_t i = first; i < stack.size(); ++i) hashes.push_back(stack[i].second);
             hashes.push_back(OpInfos[y.op()].hash);
             if (y.op() == INSTANCE) hashes.push_back((hash_t)y.tag());
+            static_assert(sizeof(std::pair<Symbolic, hash_t>) == 2 * sizeof(void*), "layout");
+            // children are the first members of consecutive stack entries
+            std::vector<Symbolic> ch;
+            ch.reserve(nc);
+            for (size_t i = first; i < stack.size(); ++i) ch.push_back(std::move(stack[i].first));
             stack.erase(stack.end() - nc, stack.end());
             stack.push_back(std::make_pair(rebuild(y, ch), Sym::hash(hashes)));
-            cache[y.id()] = stack.back();
+            cache.put(y.id(), stack.back());
         };
 
         prePostOrderTraverse(fullx, preFun, postFun);
@@ -149,9 +210,16 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
     std::vector<ExpressionNode> allNodes;
     allNodes.reserve(2 * expr.size() + 16);
     // node identity -> (occurrence count, index + 1 in allNodes)
-    std::unordered_map<long long, std::pair<unsigned, unsigned>, IdentityHash<long long>> exprCount;
-    exprCount.reserve(2 * expr.size() + 16);
+    IdMap<std::pair<unsigned, unsigned>> exprCount(4 * expr.size() + 16);
     int id = 0;
+    const bool timing = std::getenv("SYMLIB_TIMING") != nullptr && expr.size() > 100000;
+    auto tprev = std::chrono::steady_clock::now();
+    auto phase = [&](const char* what) {
+        if (!timing) return;
+        auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[symlib]   decompose: %-18s %8.3f s\n", what, std::chrono::duration<double>(now - tprev).count());
+        tprev = now;
+    };
 
     // 1. count occurrences of every sufficiently complex sub-expression
     for (const auto& x : expr) {
@@ -177,6 +245,7 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
         });
     }
 
+    phase("count");
     // 2. every output is a node
     std::vector<unsigned> outputNodes;
     outputNodes.reserve(expr.size());
@@ -187,6 +256,7 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
         exprCount[x.id()].second = (unsigned)allNodes.size();
     }
 
+    phase("outputs");
     // 3. link nodes to the nodes they contain and estimate their own cost
     for (auto& node : allNodes) {
         if (node.status == FIXED_NODE) {
@@ -204,8 +274,8 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
                 comp += y.complexity();
                 return false;
             }
-            auto it = exprCount.find(y.id());
-            const unsigned second = it == exprCount.end() ? 0 : it->second.second;
+            auto* it = exprCount.find(y.id());
+            const unsigned second = it ? it->second : 0;
             if (second != 0 && y.ahash() != node.hash()) {
                 node.addChild(std::make_pair(y.ahash(), allNodes.data() + second - 1));
                 return false;
@@ -216,9 +286,17 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
         node.complexity = (unsigned char)std::min(255, comp);
     }
 
+    phase("children");
     // 4. merge cheap or single-parent children into their parents and assign levels
     std::vector<ExpressionNode*> stack;
-    std::set<ExpressionNode*> reachableNodes;
+    // membership flags by node index; iteration in index order = allNodes order
+    std::vector<char> reachable(allNodes.size(), 0);
+    auto reach = [&](ExpressionNode* n) {
+        char& r = reachable[n - allNodes.data()];
+        const bool fresh = !r;
+        r = 1;
+        return fresh;
+    };
     std::vector<ExpressionNode*> childStack;
 
     for (auto i : outputNodes) {
@@ -232,8 +310,8 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
             stack.pop_back();
 
             if (node->status == FIXED_NODE) {
-                reachableNodes.insert(node);
-            } else if (node->status != 0 || reachableNodes.insert(node).second) {
+                reach(node);
+            } else if (node->status != 0 || reach(node)) {
                 childStack.clear();
                 for (auto& c : node->childs) childStack.push_back(c.second);
 
@@ -258,10 +336,13 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
         }
     }
 
+    phase("merge/levels");
     // 5. extract one block per remaining node
     std::vector<int> blockIdMap(allNodes.size(), -1);
     std::vector<ExpressionBlock> blocks;
-    for (auto x : reachableNodes) {
+    for (size_t xi = 0; xi < allNodes.size(); ++xi) {
+        if (!reachable[xi]) continue;
+        ExpressionNode* x = allNodes.data() + xi;
         if (x->status != MERGED) {
             blockIdMap[x->id] = (int)blocks.size();
             ExpressionBlock block;
@@ -273,6 +354,7 @@ std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const 
         }
     }
 
+    phase("extract");
     for (auto& b : blocks)
         for (auto& i : b.childs) i = blockIdMap[i];
 

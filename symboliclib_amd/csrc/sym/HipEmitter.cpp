@@ -17,6 +17,7 @@
 #include <stdexcept>
 
 #include "Program.hpp"
+#include "Tiling.hpp"
 #include "Traverse.hpp"
 
 namespace Sym {
@@ -43,8 +44,42 @@ struct KernelEmitter {
     const Program& prog;
     const ProgramKernel& K;
     bool wide;
+    // tiled form: per-tile table D (operands | ids | outpos), cnt instances, LDS base L
+    const TiledProgram* tiled = nullptr;
+    size_t kernelIndex = 0;
+
+    std::string tiledLeaf(const std::array<int, 2>& v) const {
+        const int slot = v[0];
+        const size_t V = K.numVarSlots;
+        switch (v[1]) {
+            case -2: {
+                if (K.userOutputs) {
+                    const size_t row = V + (K.numConstSlots ? 1 : 0) + (size_t)slot;
+                    return "out[D[" + std::to_string(row) + "u * cnt + j]]";
+                }
+                return "lds[L + " + std::to_string(slot) + "u * cnt + j]";
+            }
+            case -3: {
+                const auto& va = K.access.at((size_t)slot);
+                const int kind = tiled->colKind[tiled->colOffset[kernelIndex] + va.baseVariableId];
+                const std::string e = "D[" + std::to_string(va.baseVariableId) + "u * cnt + j]";
+                const std::string off = va.offset ? ((va.offset > 0 ? " + " : " - ") + std::to_string(std::abs(va.offset)) + "u") : "";
+                if (kind == 0) return "x[" + e + off + "]";
+                if (kind == 1) return "lds[" + e + " & 0x7fffffffu]";
+                return "gather(x, lds, " + e + ", " + std::to_string(va.offset) + ")";
+            }
+            case -4:
+                return "c[" + std::to_string(K.constOffset + (size_t)slot * K.n) + "u + D[" + std::to_string(K.numVarSlots) +
+                       "u * cnt + j]]";
+            case LOCAL_VAR:
+                return "r" + std::to_string(slot);
+            default:
+                throw std::logic_error("emitHipTiled: unexpected variable group " + std::to_string(v[1]));
+        }
+    }
 
     std::string leaf(const std::array<int, 2>& v) const {
+        if (tiled) return tiledLeaf(v);
         const int slot = v[0];
         switch (v[1]) {
             case -2: {
@@ -170,7 +205,8 @@ std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<Lau
         const std::string name = "symlib_wave" + std::to_string(w);
         os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name
            << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
-           << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out) {\n";
+           << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
+           << "    const INDEX* __restrict__ tiles) {\n";
         os << "    const unsigned b = blockIdx.x;\n";
         size_t blocks = 0;
         for (const auto& K : prog.kernels) {
@@ -187,8 +223,57 @@ std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<Lau
         }
         os << "}\n\n";
         if (blocks > 0xffffffffull) throw std::length_error("emitHip: grid too large");
-        launches.push_back({name, (uint32_t)blocks, (uint32_t)T});
+        launches.push_back({name, (uint32_t)blocks, (uint32_t)T, 0});
     }
+    return os.str();
+}
+
+std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const EmitOptions& opt,
+                         std::vector<LaunchDesc>& launches) {
+    if (!tp.valid) throw std::invalid_argument("emitHipTiled: program is not tiled");
+    const int T = opt.threadsPerBlock;
+    std::ostringstream os;
+    os << "// generated by symboliclib_amd: tile-fused program, one workgroup per tile, FP64\n";
+    os << "typedef unsigned int INDEX;\n\n";
+    os << "__device__ __forceinline__ double gather(const double* __restrict__ x, const double* lds, INDEX v, int off) {\n"
+       << "    return (v & 0x80000000u) ? lds[v & 0x7fffffffu] : x[(int)v + off];\n}\n\n";
+    os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") symlib_tiled"
+       << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+       << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
+       << "    const INDEX* __restrict__ tiles) {\n";
+    os << "    extern __shared__ double lds[];\n";
+    os << "    const INDEX* __restrict__ H = tiles + tiles[blockIdx.x];\n";
+    for (int w = 0; w < prog.numWaves; ++w) {
+        std::vector<size_t> ks;
+        for (size_t r = 0; r < prog.kernels.size(); ++r)
+            if (prog.kernels[r].wave == w && tp.kernelUsed[r] && prog.kernels[r].n) ks.push_back(r);
+        if (ks.empty()) continue;
+        if (w > 0) os << "    __syncthreads();\n";
+        os << "    {\n";
+        std::string total;
+        for (size_t q = 0; q < ks.size(); ++q) {
+            os << "        const unsigned n" << q << " = H[" << 3 * ks[q] << "];\n";
+            total += (q ? " + n" : "n") + std::to_string(q);
+        }
+        os << "        const unsigned total = " << total << ";\n";
+        os << "        for (unsigned g = threadIdx.x; g < total; g += " << T << "u) {\n";
+        os << "            unsigned j = g;\n";
+        for (size_t q = 0; q < ks.size(); ++q) {
+            const size_t r = ks[q];
+            os << "            if (j < n" << q << ") {\n";
+            os << "                const unsigned cnt = n" << q << ";\n";
+            os << "                const INDEX* __restrict__ D = tiles + H[" << 3 * r + 1 << "];\n";
+            if (!prog.kernels[r].userOutputs) os << "                const unsigned L = H[" << 3 * r + 2 << "];\n";
+            KernelEmitter ke{prog, prog.kernels[r], false, &tp, r};
+            ke.statements(prog.kernels[r].code, os, "                ");
+            os << "                continue;\n            }\n";
+            os << "            j -= n" << q << ";\n";
+        }
+        os << "        }\n    }\n";
+    }
+    os << "}\n";
+    launches.clear();
+    launches.push_back({"symlib_tiled", (uint32_t)tp.numTiles, (uint32_t)T, (uint32_t)tp.ldsBytes});
     return os.str();
 }
 

@@ -1,5 +1,8 @@
 #include "Program.hpp"
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <set>
 #include <stdexcept>
@@ -27,6 +30,19 @@ std::vector<std::vector<int>> group(const std::vector<ExpressionBlock>& blocks) 
     return ret;
 }
 
+// Phase timer printed to stderr when SYMLIB_TIMING is set (the reference prints its phases
+// through Timer, src/Timer.cpp:71-98).
+struct PhaseTimer {
+    bool on = std::getenv("SYMLIB_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void operator()(const char* what) {
+        if (!on) return;
+        auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[symlib] %-28s %8.3f s\n", what, std::chrono::duration<double>(now - t).count());
+        t = now;
+    }
+};
+
 inline long long varKey(const std::array<int, 2>& v) { return ((long long)v[1] << 32) ^ (unsigned)v[0]; }
 
 }  // namespace
@@ -34,8 +50,10 @@ inline long long varKey(const std::array<int, 2>& v) { return ((long long)v[1] <
 Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& io, const BuildOptions& opt) {
     Program prog;
     prog.numInputs = io.inputDataSize;
+    PhaseTimer timer;
 
     auto blocks = decompose(outputs, opt.decompositionThreshold, GLOBAL_INTERMEDIATE, true);
+    timer("decompose");
     auto groupIds = group(blocks);
     std::vector<int> blockGroup(blocks.size());
     std::vector<Kernel> kernels;
@@ -48,6 +66,7 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
         }
         kernels.emplace_back(exprs, 0, opt.simplifyTemplates);
     }
+    timer("group + kernel templates");
 
     // dependencies: the group computing a child must run before its readers
     const size_t nk = kernels.size();
@@ -118,6 +137,7 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
     }
     for (size_t pos = 0; pos < acc; ++pos)
         if (!written[pos]) throw std::logic_error("buildProgram: an output value is never computed");
+    timer("layout");
 
     for (size_t r = 0; r < nk; ++r) {
         Kernel& K = kernels[order[r]];
@@ -153,6 +173,7 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
         }
     }
     for (auto& P : prog.kernels) prog.numWaves = std::max(prog.numWaves, P.wave + 1);
+    timer("tables");
     return prog;
 }
 

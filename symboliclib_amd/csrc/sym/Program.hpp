@@ -71,6 +71,7 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
 struct LaunchDesc {
     std::string name;
     uint32_t grid = 0, block = 0;
+    uint32_t shared = 0;  // dynamic LDS bytes
 };
 
 struct EmitOptions {

@@ -23,7 +23,18 @@ struct IdentityHash {
 template <class F>
 void preOrderTraverse(const Symbolic& x, F&& fun, const bool unique = false) {
     std::unordered_set<long long> visited;
-    std::vector<const Symbolic*> stack{&x};
+    thread_local std::vector<const Symbolic*> stackStore;
+    std::vector<const Symbolic*> stack;
+    stack.swap(stackStore);  // reuse capacity; nested calls get a fresh vector
+    stack.clear();
+    stack.push_back(&x);
+    struct Restore {
+        std::vector<const Symbolic*>& s;
+        std::vector<const Symbolic*>& store;
+        ~Restore() {
+            if (s.capacity() > store.capacity()) s.swap(store);
+        }
+    } restore{stack, stackStore};
     while (!stack.empty()) {
         const Symbolic* y = stack.back();
         stack.pop_back();

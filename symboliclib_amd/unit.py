@@ -16,13 +16,16 @@ from . import _native
 from .workloads import to_colmajor
 
 
-def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True):
+def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True, tiled=True,
+          tile_lds_bytes=0):
     o = _native.Options()
     _native.lib().symlib_default_options(ctypes.byref(o))
     o.threads_per_block = int(threads_per_block)
     o.decomposition_threshold = int(decomposition_threshold)
     o.simplify = int(bool(simplify))
     o.share_templates = int(bool(share_templates))
+    o.tiled = int(bool(tiled))
+    o.tile_lds_bytes = int(tile_lds_bytes)
     return o
 
 
@@ -105,7 +108,9 @@ class Unit:
     def info(self):
         i = _native.UnitInfo()
         _native.check(_native.lib().symlib_unit_get_info(self._h, ctypes.byref(i)), "symlib_unit_get_info")
-        return {f: getattr(i, f) for f, _ in i._fields_}
+        d = {f: getattr(i, f) for f, _ in i._fields_}
+        d["tiling"] = d["tiling"].decode()
+        return d
 
     def source(self):
         return _native.lib().symlib_unit_source(self._h).decode()

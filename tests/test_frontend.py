@@ -130,3 +130,35 @@ def test_template_sharing_matches_direct_tables():
         ia, ib = a.info(), b.info()
         assert ia["num_results"] == ib["num_results"]
         assert ia["num_outpos"] == ib["num_outpos"]
+
+
+@pytest.mark.parametrize("side", [8, 24])
+def test_tiled_form_is_default_and_compiles(tmp_path, side):
+    L = _native.lib()
+    rest, F, X = W.c3_cloth(side=side)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND) as u:
+        info = u.info()
+        assert info["tiled"] == 1, info["tiling"]
+        fields = dict(kv.split("=") for kv in info["tiling"].split())
+        assert int(fields["lds"]) <= 64 * 1024
+        assert 1.0 <= float(fields["recompute"]) < 4.0
+        src = u.source()
+        assert "symlib_tiled" in src and "__syncthreads" in src
+        out = str(tmp_path / "tiled.co")
+        assert L.symhip_compile_to_file(src.encode(), None, b"gfx950", out.encode()) == 0, L.symhip_last_error()
+
+
+def test_waves_form_on_request():
+    V, F = W.c2_mesh(side=6)
+    with Unit.cotan(len(V), F, tiled=False) as u:
+        info = u.info()
+        assert info["tiled"] == 0
+        assert "symlib_wave2" in u.source()
+
+
+def test_tiling_with_tiny_budget_falls_back():
+    # an output that needs more intermediates than one tile holds cannot be tiled
+    rest, F, X = W.c3_cloth(side=4)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tile_lds_bytes=256) as u:
+        info = u.info()
+        assert info["tiled"] == 0 and info["tiling"].startswith("untiled")

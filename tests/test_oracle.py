@@ -184,3 +184,16 @@ def test_bw_pattern_counts_full_size():
     assert cp[-1] == 4110354
     cp, _ = O.bw_pattern(len(rest), F, with_bend=True)
     assert cp[-1] == 4890648
+
+
+def test_scales_bound_values():
+    # S(v) >= |v| for every value, and S is at most a modest multiple of |v| where nothing cancels
+    rest, F, X = _bw_small()
+    _, _, vals, sc = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+    assert np.all(sc >= np.abs(vals) * (1 - 1e-12))
+    V, F2 = W.c2_mesh(side=10)
+    _, _, v2, s2 = O.cotan(V, F2)
+    assert np.all(s2 >= np.abs(v2) * (1 - 1e-12))
+    n, cp, ri, a = W.c1_matrix(subdiv=2)
+    _, _, v1, s1 = O.ata_plus_a(cp, ri, np.abs(a))
+    assert np.allclose(s1, v1, rtol=1e-14)  # all terms positive: S = v

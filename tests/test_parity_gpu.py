@@ -2,7 +2,11 @@
 FP64 oracle.
 
 Bar: output patterns (CSC outer/inner arrays) bit-exact; every value within relative error
-1e-12 of the oracle, the error of a value v being |v - v_oracle| / sum of |terms| of v.  The
+1e-12 of the oracle, the error of a value v being |v - v_oracle| / S(v) where S(v) is the sum of
+the absolute magnitudes of the terms v is made of: the oracle's closed form evaluated with the
+absolute value of every factor and of every additive term (products A(k,i) A(k,j) and A(i,j);
+|e0_k e1_k| / |e0 x e1| per corner weight; for the cloth, the stretch and shear terms of every
+per-face Hessian entry from the edge vectors on, plus |k_bend Q| per bending constant).  The
 maximum and median errors are printed for every case.
 """
 import os
@@ -39,10 +43,14 @@ def test_sphere_mtx_golden(require_device):
         _check("sphere.mtx A^T A + A", u, (g["indptr"], g["indices"], g["values"], sc), u.get_results())
 
 
+FORMS = [pytest.param(True, id="tiled"), pytest.param(False, id="waves")]
+
+
+@pytest.mark.parametrize("tiled", FORMS)
 @pytest.mark.parametrize("subdiv", [1, 5])
-def test_ata_plus_a(require_device, subdiv):
+def test_ata_plus_a(require_device, subdiv, tiled):
     n, cp, ri, a = W.c1_matrix(subdiv=subdiv)
-    with Unit.ata_plus_a(cp, ri) as u:
+    with Unit.ata_plus_a(cp, ri, tiled=tiled) as u:
         u.compile()
         u.execute(a)
         _check(f"A^T A + A icosphere({subdiv})", u, O.ata_plus_a(cp, ri, a), u.get_results())
@@ -52,18 +60,21 @@ def test_ata_plus_a(require_device, subdiv):
         _check(f"A^T A + A icosphere({subdiv}) second values", u, O.ata_plus_a(cp, ri, a2), u.get_results())
 
 
-@pytest.mark.parametrize("side", [4, 64, 512])
-def test_cotan(require_device, side):
+@pytest.mark.parametrize("side,tiled", [(4, True), (4, False), (64, True), (64, False), (512, True), (512, False)])
+def test_cotan(require_device, side, tiled):
     V, F = W.c2_mesh(side=side)
-    with Unit.cotan(len(V), F) as u:
+    with Unit.cotan(len(V), F, tiled=tiled) as u:
+        assert u.info()["tiled"] == int(tiled), u.info()["tiling"]
         u.compile().execute(W.to_colmajor(V))
         _check(f"cotan grid {side}x{side}", u, O.cotan(V, F), u.get_results())
 
 
-@pytest.mark.parametrize("side,share", [(4, False), (4, True), (32, True), (256, True)])
-def test_bw_hessian(require_device, side, share):
+@pytest.mark.parametrize("side,share,tiled", [(4, False, True), (4, True, True), (4, True, False), (32, True, True),
+                                              (32, True, False), (256, True, True)])
+def test_bw_hessian(require_device, side, share, tiled):
     rest, F, X = W.c3_cloth(side=side)
-    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, share_templates=share) as u:
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, share_templates=share, tiled=tiled) as u:
+        assert u.info()["tiled"] == int(tiled), u.info()["tiling"]
         u.compile().execute(W.to_colmajor(X))
         ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
         _check(f"Baraff-Witkin {side}x{side} share={share}", u, ref, u.get_results())
@@ -76,10 +87,11 @@ def test_bw_hessian_stretch_shear_only(require_device):
         _check("Baraff-Witkin 9x9 no bend", u, O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, 0.0), u.get_results())
 
 
+@pytest.mark.parametrize("tiled", FORMS)
 @pytest.mark.parametrize("tpb", [64, 1024])
-def test_threads_per_block(require_device, tpb):
+def test_threads_per_block(require_device, tpb, tiled):
     V, F = W.c2_mesh(side=20)
-    with Unit.cotan(len(V), F, threads_per_block=tpb) as u:
+    with Unit.cotan(len(V), F, threads_per_block=tpb, tiled=tiled) as u:
         u.compile().execute(W.to_colmajor(V))
         _check(f"cotan tpb={tpb}", u, O.cotan(V, F), u.get_results())
 
@@ -90,6 +102,16 @@ def test_single_entry_matrix(require_device):
     with Unit.ata_plus_a(cp, ri) as u:
         u.compile().execute(np.array([3.0]))
         assert u.get_results()[0] == 12.0  # 3 * 3 + 3
+
+
+@pytest.mark.parametrize("lds", [16 * 1024, 160 * 1024])
+def test_bw_hessian_tile_budgets(require_device, lds):
+    rest, F, X = W.c3_cloth(side=24)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tile_lds_bytes=lds) as u:
+        assert u.info()["tiled"] == 1, u.info()["tiling"]
+        u.compile().execute(W.to_colmajor(X))
+        ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+        _check(f"Baraff-Witkin 24x24 tile LDS {lds}", u, ref, u.get_results())
 
 
 def test_matrix_with_empty_columns(require_device):
