@@ -30,13 +30,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_workload(name):
+def build_workload(name, **opt):
     from symboliclib_amd import workloads as W
     from symboliclib_amd.unit import Unit
 
     if name == "c1":
         n, cp, ri, a = W.c1_matrix()
-        unit = Unit.ata_plus_a(cp, ri)
+        unit = Unit.ata_plus_a(cp, ri, **opt)
         inputs = a
         counts = {"n_in": len(ri), "n_const": 0, "n_conn": len(ri) + n + 1}
         desc = {"workload": "A^T A + A, icosphere subdivision 5", "rows": n, "nnz_A": int(len(ri))}
@@ -53,7 +53,7 @@ def build_workload(name):
 
     elif name == "c2":
         V, F = W.c2_mesh()
-        unit = Unit.cotan(len(V), F)
+        unit = Unit.cotan(len(V), F, **opt)
         inputs = W.to_colmajor(V)
         counts = {"n_in": 3 * len(V), "n_const": 0, "n_conn": 3 * len(F)}
         desc = {"workload": "cotangent Laplacian, 512x512 jittered grid", "vertices": len(V), "faces": len(F)}
@@ -75,7 +75,7 @@ def build_workload(name):
 
     else:
         rest, F, X = W.c3_cloth()
-        unit = Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+        unit = Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, **opt)
         inputs = W.to_colmajor(X)
         import oracle as O
 
@@ -105,6 +105,10 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--form", default="tiled", choices=["tiled", "waves"],
+                    help="tiled: LDS-staged work tiles (default); waves: one launch per wave, intermediates in HBM")
+    ap.add_argument("--tile-lds", type=int, default=0, help="LDS bytes per tile (0 = default)")
+    ap.add_argument("--tpb", type=int, default=256, help="threads per block")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,7 +127,8 @@ def main():
 
     log(f"[bench] building workload {args.workload}")
     t0 = time.perf_counter()
-    unit, inputs, counts, desc, oracle_fn, cpu_eval = build_workload(args.workload)
+    unit, inputs, counts, desc, oracle_fn, cpu_eval = build_workload(
+        args.workload, tiled=args.form == "tiled", tile_lds_bytes=args.tile_lds, threads_per_block=args.tpb)
     t1 = time.perf_counter()
     unit.compile()
     t2 = time.perf_counter()
@@ -221,7 +226,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": dict(desc, output_nnz=n_out, form="plain", launches=nl, parallelism="single GPU"),
+        "config": dict(desc, output_nnz=n_out, form=f"plain single-instance, {args.form}", tiling=info["tiling"],
+                       threads_per_block=args.tpb, launches=nl, parallelism="single GPU"),
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
