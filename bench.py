@@ -105,8 +105,8 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--form", default="tiled", choices=["tiled", "waves"],
-                    help="tiled: LDS-staged work tiles (default); waves: one launch per wave, intermediates in HBM")
+    ap.add_argument("--form", default="waves", choices=["tiled", "waves"],
+                    help="waves: one launch per wave, intermediates in HBM (default); tiled: LDS-staged work tiles")
     ap.add_argument("--tile-lds", type=int, default=0, help="LDS bytes per tile (0 = default)")
     ap.add_argument("--tpb", type=int, default=256, help="threads per block")
     args = ap.parse_args()

@@ -38,6 +38,8 @@ typedef struct {
     int share_templates;         /* 1 = one differentiated template per element shape (default) */
     int tiled;                   /* 1 = LDS-staged work tiles when possible (default), 0 = wave launches */
     int tile_lds_bytes;          /* LDS budget per tile, 0 = default (64 KiB) */
+    int direct_assembly;         /* cloth Hessian: 1 = HessianAssembly (no per-nonzero expressions),
+                                    0 = symbolic SparseMatrix through ComputeUnit (default) */
 } symlib_options;
 
 typedef struct {
@@ -64,6 +66,13 @@ int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const i
 int symlib_unit_cotan(symlib_unit_t* out, int num_vertices, int num_faces, const int* faces, const symlib_options* opt);
 int symlib_unit_bw_hessian(symlib_unit_t* out, int num_vertices, int num_faces, const int* faces, const double* rest,
                            double k_stretch, double k_shear, double k_bend, const symlib_options* opt);
+
+/* Batched cloth Hessians: `count` independent cloths in one program (always directly
+ * assembled).  Cloth k has nv[k] vertices, nf[k] faces faces[k] (3 ints per face) and rest
+ * positions rest[k] (column-major nv x 3); it is input container k and output container k. */
+int symlib_unit_bw_hessian_batch(symlib_unit_t* out, int count, const int* num_vertices, const int* num_faces,
+                                 const int* const* faces, const double* const* rest, double k_stretch, double k_shear,
+                                 double k_bend, const symlib_options* opt);
 
 int symlib_unit_compile(symlib_unit_t u);
 /* inputs[i] has lengths[i] values; one entry per input container, in order */

@@ -17,7 +17,7 @@ class SymlibError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("threads_per_block", ctypes.c_int), ("decomposition_threshold", ctypes.c_int),
                 ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int), ("tiled", ctypes.c_int),
-                ("tile_lds_bytes", ctypes.c_int)]
+                ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):
@@ -72,6 +72,8 @@ SIGNATURES = {
     "symlib_unit_ata_plus_a": (_I, [ctypes.POINTER(_VP), _I, _VP, _VP, ctypes.POINTER(Options)]),
     "symlib_unit_cotan": (_I, [ctypes.POINTER(_VP), _I, _I, _VP, ctypes.POINTER(Options)]),
     "symlib_unit_bw_hessian": (_I, [ctypes.POINTER(_VP), _I, _I, _VP, _VP, _D, _D, _D, ctypes.POINTER(Options)]),
+    "symlib_unit_bw_hessian_batch": (_I, [ctypes.POINTER(_VP), _I, _VP, _VP, _VP, _VP, _D, _D, _D,
+                                          ctypes.POINTER(Options)]),
     "symlib_unit_compile": (_I, [_VP]),
     "symlib_unit_execute": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_SZ)]),
     "symlib_unit_run": (_I, [_VP, _VP]),

@@ -76,6 +76,28 @@ void finishUnit(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, Cl
     u->info.build_seconds = seconds(t1, t2);
 }
 
+void finishAssembly(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, const Sym::HessianAssembly& A) {
+    const Clock::time_point t1 = Clock::now();
+    u->unit.reset(new Sym::ComputeUnit<double>(dev, A));
+    const Clock::time_point t2 = Clock::now();
+    const auto& P = u->unit->getProgram();
+    u->info.num_inputs = P.numInputs;
+    u->info.num_intermediates = P.numIntermediates;
+    u->info.num_index = P.index.size();
+    u->info.num_const = P.consts.size();
+    u->info.num_outpos = P.outpos.size();
+    u->info.num_results = P.numResults();
+    u->info.num_kernels = (uint32_t)P.kernels.size();
+    u->info.num_waves = (uint32_t)P.numWaves;
+    u->info.source_bytes = u->unit->getCode().size();
+    u->info.tiled = u->unit->isTiled() ? 1 : 0;
+    u->info.tile_pool_entries = u->unit->tilePoolSize();
+    std::snprintf(u->info.tiling, sizeof(u->info.tiling), "%s", u->unit->tilingInfo().c_str());
+    u->info.frontend_seconds = seconds(t0, t1);
+    u->info.build_seconds = seconds(t1, t2);
+    u->patterns = u->unit->outputPatterns();
+}
+
 Sym::workloads::Faces toFaces(int nf, const int* faces) {
     Sym::workloads::Faces F((size_t)nf);
     for (int i = 0; i < nf; ++i) F[i] = {faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]};
@@ -106,8 +128,9 @@ void symlib_default_options(symlib_options* opt) {
     opt->decomposition_threshold = -1;
     opt->simplify = 1;
     opt->share_templates = 1;
-    opt->tiled = 1;
+    opt->tiled = 0;
     opt->tile_lds_bytes = 0;
+    opt->direct_assembly = 0;
 }
 
 int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {
@@ -161,6 +184,14 @@ int symlib_unit_bw_hessian(symlib_unit_t* out, int nv, int nf, const int* faces,
         std::unique_ptr<symlib_unit> u(new symlib_unit());
         const auto t0 = Clock::now();
         Sym::DenseMatrix<double> R(nv, 3, rest);
+        if (opt && opt->direct_assembly) {
+            Sym::HessianAssembly A;
+            Sym::workloads::FaceShapes shapes(k_stretch, k_shear);
+            Sym::workloads::baraffWitkinAssembly(A, shapes, R, toFaces(nf, faces), k_bend, 0);
+            finishAssembly(u.get(), makeDevice(opt), t0, A);
+            *out = u.release();
+            return;
+        }
         Sym::DenseMatrix<double> X(nv, 3);
         auto XS = Sym::makeSymbolic(X, 0);
         const bool share = opt ? opt->share_templates != 0 : true;
@@ -168,6 +199,28 @@ int symlib_unit_bw_hessian(symlib_unit_t* out, int nv, int nf, const int* faces,
         const auto t1 = Clock::now();
         finishUnit(u.get(), makeDevice(opt), t0, t1, XS, HS);
         u->patterns.push_back(patternOf(HS));
+        *out = u.release();
+    });
+}
+
+int symlib_unit_bw_hessian_batch(symlib_unit_t* out, int count, const int* nv, const int* nf, const int* const* faces,
+                                 const double* const* rest, double k_stretch, double k_shear, double k_bend,
+                                 const symlib_options* opt) {
+    if (!out) return 1;
+    *out = nullptr;
+    return guarded([&] {
+        if (count <= 0 || !nv || !nf || !faces || !rest) throw std::invalid_argument("empty batch");
+        std::unique_ptr<symlib_unit> u(new symlib_unit());
+        const auto t0 = Clock::now();
+        Sym::HessianAssembly A;
+        Sym::workloads::FaceShapes shapes(k_stretch, k_shear);
+        for (int k = 0; k < count; ++k) {
+            checkFaces(nv[k], nf[k], faces[k]);
+            if (!rest[k]) throw std::invalid_argument("rest positions missing");
+            Sym::DenseMatrix<double> R(nv[k], 3, rest[k]);
+            Sym::workloads::baraffWitkinAssembly(A, shapes, R, toFaces(nf[k], faces[k]), k_bend, k);
+        }
+        finishAssembly(u.get(), makeDevice(opt), t0, A);
         *out = u.release();
     });
 }

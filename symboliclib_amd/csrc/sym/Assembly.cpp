@@ -1,0 +1,395 @@
+#include "Assembly.hpp"
+
+#include <algorithm>
+#include <limits>
+#include <map>
+#include <stdexcept>
+
+#include "ComputeKernel.hpp"
+#include "Parallel.hpp"
+#include "Template.hpp"
+#include "Timer.hpp"
+
+namespace Sym {
+
+int HessianAssembly::addInstance(int numVars, int group) {
+    if (numVars <= 0) throw std::invalid_argument("HessianAssembly: instance without variables");
+    for (auto& I : inst_)
+        if (I.group == group) throw std::invalid_argument("HessianAssembly: two instances share a variable group");
+    Instance I;
+    I.numVars = numVars;
+    I.group = group;
+    I.firstElem = elemTmpl_.size();
+    inst_.push_back(std::move(I));
+    return (int)inst_.size() - 1;
+}
+
+void HessianAssembly::addElement(const Symbolic& energy) {
+    if (inst_.empty()) throw std::logic_error("HessianAssembly: addElement before addInstance");
+    Instance& I = inst_.back();
+    const int group = I.group;
+    ElementShape s;
+    analyzeShape(energy, [group](const Symbolic& v) { return v.variable()[1] == group; }, s);
+    const ExprTemplate& t = hessianTemplateFor(s);
+    elemTmpl_.push_back(t.id);
+    elemArg_.push_back(args_.size());
+    for (auto& v : s.diffLeaves) {
+        const int id = v.variable()[0];
+        if (id < 0 || id >= I.numVars) throw std::invalid_argument("HessianAssembly: variable outside its instance");
+        args_.push_back({0.0, id, 1});
+    }
+    for (auto& v : s.otherLeaves) {
+        if (v.op() == CONST) args_.push_back({v.constant(), 0, 0});
+        else throw std::invalid_argument("HessianAssembly: element reads a variable of another group");
+    }
+    ++I.numElems;
+}
+
+int HessianAssembly::defineShape(const Symbolic& energy, int numLocals, int numParams) {
+    ElementShape s;
+    analyzeShape(energy, [](const Symbolic& v) { return v.variable()[1] == ELEMENT_LOCAL; }, s);
+    Shape sh;
+    sh.tmpl = hessianTemplateFor(s).id;
+    sh.numLocals = numLocals;
+    sh.numParams = numParams;
+    for (auto& v : s.diffLeaves) {
+        const int k = v.variable()[0];
+        if (k < 0 || k >= numLocals) throw std::invalid_argument("HessianAssembly: local variable out of range");
+        sh.diffLocal.push_back(k);
+    }
+    for (auto& v : s.otherLeaves) {
+        if (v.op() == VAR) {
+            const auto id = v.variable();
+            if (id[1] != ELEMENT_PARAM || id[0] < 0 || id[0] >= numParams)
+                throw std::invalid_argument("HessianAssembly: element shape reads an unknown variable");
+            sh.otherParam.push_back(id[0]);
+            sh.otherConst.push_back(0.0);
+        } else {
+            sh.otherParam.push_back(-1);
+            sh.otherConst.push_back(v.constant());
+        }
+    }
+    shapes_.push_back(std::move(sh));
+    return (int)shapes_.size() - 1;
+}
+
+void HessianAssembly::addElement(int shape, const int* vars, const double* params) {
+    if (inst_.empty()) throw std::logic_error("HessianAssembly: addElement before addInstance");
+    if (shape < 0 || (size_t)shape >= shapes_.size()) throw std::invalid_argument("HessianAssembly: unknown shape");
+    Instance& I = inst_.back();
+    const Shape& sh = shapes_[shape];
+    elemTmpl_.push_back(sh.tmpl);
+    elemArg_.push_back(args_.size());
+    for (int k : sh.diffLocal) {
+        const int id = vars[k];
+        if (id < 0 || id >= I.numVars) throw std::invalid_argument("HessianAssembly: variable outside its instance");
+        args_.push_back({0.0, id, 1});
+    }
+    for (size_t q = 0; q < sh.otherParam.size(); ++q)
+        args_.push_back({sh.otherParam[q] >= 0 ? params[sh.otherParam[q]] : sh.otherConst[q], 0, 0});
+    ++I.numElems;
+}
+
+void HessianAssembly::addConstant(int row, int col, double value) {
+    if (inst_.empty()) throw std::logic_error("HessianAssembly: addConstant before addInstance");
+    Instance& I = inst_.back();
+    if (row < 0 || col < 0 || row >= I.numVars || col >= I.numVars)
+        throw std::invalid_argument("HessianAssembly: constant outside the Hessian");
+    I.cRow.push_back(row);
+    I.cCol.push_back(col);
+    I.cVal.push_back(value);
+}
+
+namespace {
+
+// Store only base columns; a column equal to a stored one plus a constant reads that one.
+void compressColumns(std::vector<std::vector<uint32_t>>& cols, std::vector<Kernel::VariableAccessData>& access,
+                     const std::vector<char>& compressible) {
+    std::vector<std::vector<uint32_t>> kept;
+    std::vector<int> keptSrc;
+    access.clear();
+    for (size_t c = 0; c < cols.size(); ++c) {
+        int found = -1;
+        long off = 0;
+        if (compressible[c] && !cols[c].empty())
+            for (size_t k = 0; k < kept.size() && found < 0; ++k) {
+                if (!compressible[keptSrc[k]]) continue;
+                const long d = (long)cols[c][0] - (long)kept[k][0];
+                bool ok = true;
+                for (size_t i = 1; i < cols[c].size() && ok; ++i) ok = (long)cols[c][i] - (long)kept[k][i] == d;
+                if (ok && d > std::numeric_limits<int>::min() / 2 && d < std::numeric_limits<int>::max() / 2) {
+                    found = (int)k;
+                    off = d;
+                }
+            }
+        if (found >= 0) {
+            access.push_back({found, (int)off});
+        } else {
+            access.push_back({(int)kept.size(), 0});
+            kept.push_back(std::move(cols[c]));
+            keptSrc.push_back((int)c);
+        }
+    }
+    cols.swap(kept);
+}
+
+}  // namespace
+
+Program HessianAssembly::build(const BuildOptions& opt, ProgramInputs& io, std::vector<SparseMatrix<double>>& patterns) const {
+    Program prog;
+    PhaseTimer timer;
+    io = ProgramInputs();
+    for (auto& I : inst_) {
+        io.inputSizes.push_back((size_t)I.numVars);
+        io.inputOffsets[I.group] = io.inputDataSize;
+        io.inputLengths[I.group] = (size_t)I.numVars;
+        io.inputDataSize += (size_t)I.numVars;
+    }
+    prog.numInputs = io.inputDataSize;
+    if (prog.numInputs > std::numeric_limits<uint32_t>::max() / 2)
+        throw std::length_error("HessianAssembly: too many input values");
+    std::vector<size_t> inOff(inst_.size());
+    for (size_t k = 0; k < inst_.size(); ++k) inOff[k] = io.inputOffsets.at(inst_[k].group);
+
+    // ---- element kernels: one per template, elements in insertion order
+    std::map<uint32_t, std::vector<uint32_t>> byTemplate;
+    std::vector<uint32_t> elemInst(elemTmpl_.size());
+    for (size_t k = 0; k < inst_.size(); ++k)
+        for (size_t e = inst_[k].firstElem; e < inst_[k].firstElem + inst_[k].numElems; ++e) elemInst[e] = (uint32_t)k;
+    for (size_t e = 0; e < elemTmpl_.size(); ++e) byTemplate[elemTmpl_[e]].push_back((uint32_t)e);
+
+    std::vector<uint32_t> elemKernel(elemTmpl_.size()), elemLocal(elemTmpl_.size());
+    for (auto& kv : byTemplate) {
+        const ExprTemplate& t = getTemplate(kv.first);
+        const auto& elems = kv.second;
+        const size_t n = elems.size();
+        const int nv = t.numDiffVars;
+        ProgramKernel P;
+        P.n = n;
+        P.wave = 0;
+        P.userOutputs = false;
+        P.numOutSlots = (size_t)t.numOutputs;
+        P.interBase = prog.numIntermediates;
+        prog.numIntermediates += P.numOutSlots * n;
+
+        // argument slots -> variable columns, constant columns or literals
+        std::vector<std::vector<uint32_t>> vcols;
+        std::vector<char> compressible;
+        std::vector<std::vector<double>> ccols;
+        std::vector<Symbolic> argRef(t.numArgs);
+        for (int a = 0; a < t.numArgs; ++a) {
+            const Arg& first = args_[elemArg_[elems[0]] + a];
+            if (first.kind == 1) {
+                std::vector<uint32_t> col(n);
+                for (size_t i = 0; i < n; ++i) {
+                    const Arg& g = args_[elemArg_[elems[i]] + a];
+                    if (g.kind != 1) throw std::logic_error("HessianAssembly: argument kinds differ within a template");
+                    col[i] = (uint32_t)(inOff[elemInst[elems[i]]] + (size_t)g.id);
+                }
+                argRef[a] = Symbolic((int)vcols.size(), -3);
+                vcols.push_back(std::move(col));
+                compressible.push_back(1);
+            } else {
+                std::vector<double> col(n);
+                bool same = true;
+                for (size_t i = 0; i < n; ++i) {
+                    const Arg& g = args_[elemArg_[elems[i]] + a];
+                    if (g.kind != 0) throw std::logic_error("HessianAssembly: argument kinds differ within a template");
+                    col[i] = g.value;
+                    same = same && col[i] == col[0];
+                }
+                if (same) {
+                    argRef[a] = Symbolic(col[0]);
+                } else {
+                    argRef[a] = Symbolic((int)ccols.size(), -4);
+                    ccols.push_back(std::move(col));
+                }
+            }
+        }
+        (void)nv;
+        std::vector<Symbolic> assigns;
+        assigns.reserve(t.numOutputs);
+        for (int k = 0; k < t.numOutputs; ++k)
+            assigns.emplace_back(ASSIGN, Symbolic(k, -2), substituteArguments(t.outputs[k], argRef));
+        P.code = optimizeTemplate(Symbolic(BLOCK, assigns), opt.simplifyTemplates);
+
+        compressColumns(vcols, P.access, compressible);
+        P.numVarSlots = vcols.size();
+        P.indexOffset = prog.index.size();
+        for (auto& c : vcols) prog.index.insert(prog.index.end(), c.begin(), c.end());
+        P.numConstSlots = ccols.size();
+        P.constOffset = prog.consts.size();
+        for (auto& c : ccols) prog.consts.insert(prog.consts.end(), c.begin(), c.end());
+
+        const uint32_t kid = (uint32_t)prog.kernels.size();
+        for (size_t i = 0; i < n; ++i) {
+            elemKernel[elems[i]] = kid;
+            elemLocal[elems[i]] = (uint32_t)i;
+        }
+        prog.kernels.push_back(std::move(P));
+    }
+
+    timer("assembly: element kernels");
+    // ---- per instance: pattern and contribution lists
+    struct OutRec {
+        uint32_t first, count;  // into contrib
+        double cval;
+        bool hasConst;
+        uint32_t pos;           // result position
+    };
+    struct InstanceResult {
+        std::vector<OutRec> outs;  // first: into contrib of this instance; pos: within this instance
+        std::vector<uint32_t> contrib;
+        SparseMatrix<double> pattern;
+    };
+    std::vector<InstanceResult> results(inst_.size());
+
+    auto buildInstance = [&](size_t k) {
+        const Instance& I = inst_[k];
+        std::vector<OutRec>& outs = results[k].outs;
+        std::vector<uint32_t>& contrib = results[k].contrib;
+        const size_t resultBase = 0;
+        const int nvar = I.numVars;
+        // (col, row, value-source) triplets: source >= 0 intermediate position, < 0 constant -(idx+1)
+        std::vector<uint32_t> tr, tc;
+        std::vector<int64_t> src;
+        size_t est = I.cRow.size();
+        for (size_t e = I.firstElem; e < I.firstElem + I.numElems; ++e) {
+            const int nd = getTemplate(elemTmpl_[e]).numDiffVars;
+            est += (size_t)nd * nd;
+        }
+        tr.reserve(est);
+        tc.reserve(est);
+        src.reserve(est);
+        for (size_t e = I.firstElem; e < I.firstElem + I.numElems; ++e) {
+            const ExprTemplate& t = getTemplate(elemTmpl_[e]);
+            const ProgramKernel& P = prog.kernels[elemKernel[e]];
+            const int nd = t.numDiffVars;
+            const Arg* a = &args_[elemArg_[e]];
+            for (int i = 0; i < nd; ++i)
+                for (int j = 0; j < nd; ++j) {
+                    const int slot = t.entryToOutput[(size_t)i * nd + j];
+                    tr.push_back((uint32_t)a[i].id);
+                    tc.push_back((uint32_t)a[j].id);
+                    src.push_back((int64_t)(prog.numInputs + P.interBase + (size_t)slot * P.n + elemLocal[e]));
+                }
+        }
+        for (size_t c = 0; c < I.cRow.size(); ++c) {
+            tr.push_back((uint32_t)I.cRow[c]);
+            tc.push_back((uint32_t)I.cCol[c]);
+            src.push_back(-(int64_t)c - 1);
+        }
+        // counting sort by column, then rows within a column
+        const size_t m = tr.size();
+        std::vector<uint64_t> cstart((size_t)nvar + 1, 0);
+        for (size_t q = 0; q < m; ++q) ++cstart[tc[q] + 1];
+        for (int j = 0; j < nvar; ++j) cstart[j + 1] += cstart[j];
+        std::vector<uint32_t> order(m);
+        {
+            std::vector<uint64_t> fill(cstart.begin(), cstart.end() - 1);
+            for (size_t q = 0; q < m; ++q) order[fill[tc[q]]++] = (uint32_t)q;
+        }
+        SparseMatrix<double> pat(nvar, nvar);
+        std::vector<int> outer((size_t)nvar + 1, 0), inner;
+        inner.reserve(m / 2);
+        for (int j = 0; j < nvar; ++j) {
+            auto b = order.begin() + (long)cstart[j], e = order.begin() + (long)cstart[j + 1];
+            std::stable_sort(b, e, [&](uint32_t x, uint32_t y) { return tr[x] < tr[y]; });
+            outer[j] = (int)inner.size();
+            for (auto it = b; it != e;) {
+                auto it2 = it;
+                OutRec rec{(uint32_t)contrib.size(), 0, 0.0, false, (uint32_t)(resultBase + inner.size())};
+                while (it2 != e && tr[*it2] == tr[*it]) {
+                    const int64_t s = src[*it2];
+                    if (s >= 0) {
+                        contrib.push_back((uint32_t)s);
+                        ++rec.count;
+                    } else {
+                        rec.cval += I.cVal[(size_t)(-s - 1)];
+                        rec.hasConst = true;
+                    }
+                    ++it2;
+                }
+                inner.push_back((int)tr[*it]);
+                outs.push_back(rec);
+                it = it2;
+            }
+        }
+        outer[nvar] = (int)inner.size();
+        pat.setPattern(outer, inner);
+        results[k].pattern = std::move(pat);
+    };
+    parallelFor(inst_.size(), buildInstance);
+
+    // merge in instance order
+    std::vector<OutRec> outs;
+    std::vector<uint32_t> contrib;
+    size_t resultBase = 0, totalOuts = 0, totalContrib = 0;
+    for (auto& r : results) {
+        totalOuts += r.outs.size();
+        totalContrib += r.contrib.size();
+    }
+    outs.reserve(totalOuts);
+    contrib.reserve(totalContrib);
+    patterns.clear();
+    for (size_t k = 0; k < inst_.size(); ++k) {
+        InstanceResult& r = results[k];
+        const size_t nout = r.pattern.nonZeros();
+        if (resultBase + nout > std::numeric_limits<uint32_t>::max())
+            throw std::length_error("HessianAssembly: too many outputs");
+        const uint32_t cbase = (uint32_t)contrib.size();
+        for (auto rec : r.outs) {
+            rec.first += cbase;
+            rec.pos += (uint32_t)resultBase;
+            outs.push_back(rec);
+        }
+        contrib.insert(contrib.end(), r.contrib.begin(), r.contrib.end());
+        prog.outStart.push_back(resultBase);
+        prog.outLength.push_back(nout);
+        io.outputSizes.push_back(nout);
+        resultBase += nout;
+        patterns.push_back(std::move(r.pattern));
+        r = InstanceResult();
+    }
+
+    timer("assembly: patterns");
+    // ---- output kernels: one per (number of element contributions, constant term)
+    std::map<std::pair<uint32_t, bool>, std::vector<uint32_t>> classes;
+    for (size_t o = 0; o < outs.size(); ++o) classes[{outs[o].count, outs[o].hasConst}].push_back((uint32_t)o);
+    for (auto& kv : classes) {
+        const uint32_t kc = kv.first.first;
+        const bool hc = kv.first.second;
+        const auto& list = kv.second;
+        const size_t n = list.size();
+        ProgramKernel P;
+        P.n = n;
+        P.wave = 1;
+        P.userOutputs = true;
+        P.numOutSlots = 1;
+        std::vector<Symbolic> terms;
+        for (uint32_t q = 0; q < kc; ++q) terms.emplace_back((int)q, -3);
+        if (hc) terms.emplace_back(0, -4);
+        if (terms.empty()) throw std::logic_error("HessianAssembly: output without contributions");
+        Symbolic sum = terms.size() == 1 ? terms[0] : Symbolic(ADD, terms);
+        P.code = Symbolic(BLOCK, std::vector<Symbolic>{Symbolic(ASSIGN, Symbolic(0, -2), sum)});
+        P.numVarSlots = kc;
+        P.indexOffset = prog.index.size();
+        for (uint32_t q = 0; q < kc; ++q)
+            for (size_t i = 0; i < n; ++i) prog.index.push_back(contrib[outs[list[i]].first + q]);
+        for (uint32_t q = 0; q < kc; ++q) P.access.push_back({(int)q, 0});
+        P.numConstSlots = hc ? 1 : 0;
+        P.constOffset = prog.consts.size();
+        if (hc)
+            for (size_t i = 0; i < n; ++i) prog.consts.push_back(outs[list[i]].cval);
+        P.outposOffset = prog.outpos.size();
+        for (size_t i = 0; i < n; ++i) prog.outpos.push_back(outs[list[i]].pos);
+        prog.kernels.push_back(std::move(P));
+    }
+    prog.numWaves = prog.kernels.empty() ? 0 : 2;
+    timer("assembly: output kernels");
+    if (prog.numInputs + prog.numIntermediates > std::numeric_limits<uint32_t>::max())
+        throw std::length_error("HessianAssembly: value buffer exceeds 32-bit positions");
+    return prog;
+}
+
+}  // namespace Sym

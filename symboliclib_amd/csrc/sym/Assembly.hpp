@@ -1,0 +1,80 @@
+// Direct assembly of element-sum Hessians into a Program (the scalable front-end).
+//
+// The drop-in path builds one expression per output nonzero: hessianSparse -> toSparseMatrix
+// (src/SymbolicDifferentiation.hpp:234-278, src/ContainerSupport.h:78-90) -> decompose ->
+// group -> Kernel.  For a sum of element energies the result always has the same structure:
+// one kernel per element shape computing the element's Hessian block, and output kernels that
+// sum element entries (plus constants) into every nonzero.  HessianAssembly produces exactly
+// that Program directly from the element shapes and their arguments: each element energy is
+// analysed (its shape and leaves) and released, the Hessian template is differentiated once
+// per shape, and the sparse pattern and contribution lists come from a counting sort of the
+// (row, col) pairs.  No per-nonzero expression is ever built, which is what makes the batched
+// workload (hundreds of cloths, ~10^7 elements) fit in memory and time.
+//
+// Several instances (independent variable vectors) can be assembled into one program: instance
+// k reads input container k and writes output container k (the full symmetric Hessian in CSC).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "Program.hpp"
+#include "Sparse.hpp"
+#include "Symbolic.hpp"
+
+namespace Sym {
+
+// Variable groups of element shapes: local variables {k, ELEMENT_LOCAL} (differentiated) and
+// parameters {j, ELEMENT_PARAM} (per-element constants).
+constexpr int ELEMENT_LOCAL = 249;
+constexpr int ELEMENT_PARAM = 248;
+
+class HessianAssembly {
+public:
+    // Start a new instance whose variables are {0..numVars-1, group}; returns its index.
+    int addInstance(int numVars, int group);
+    // Element energy of the current instance.
+    void addElement(const Symbolic& energy);
+
+    // Element shape: an energy written over local variables {k, ELEMENT_LOCAL}, k < numLocals,
+    // and parameters {j, ELEMENT_PARAM}, j < numParams.  Differentiated once; returns its id.
+    int defineShape(const Symbolic& energy, int numLocals, int numParams);
+    // Element of the current instance with shape `shape`: local k is variable vars[k] of the
+    // instance, parameter j has value params[j].  Builds no expression.
+    void addElement(int shape, const int* vars, const double* params);
+    // Constant Hessian contribution (row, col) of the current instance (not mirrored).
+    void addConstant(int row, int col, double value);
+
+    size_t numInstances() const { return inst_.size(); }
+    size_t numElements() const { return elemTmpl_.size(); }
+
+    // Build the program (inputs: one container per instance; outputs: one CSC per instance).
+    Program build(const BuildOptions& opt, ProgramInputs& io, std::vector<SparseMatrix<double>>& patterns) const;
+
+private:
+    struct Instance {
+        int numVars = 0, group = 0;
+        size_t firstElem = 0, numElems = 0;
+        std::vector<int> cRow, cCol;
+        std::vector<double> cVal;
+    };
+    struct Arg {
+        double value;  // constant value
+        int32_t id;    // variable id (kind 1)
+        int32_t kind;  // 0 constant, 1 variable of this instance
+    };
+    struct Shape {
+        uint32_t tmpl;
+        int numLocals, numParams;
+        std::vector<int> diffLocal;   // template diff slot -> local variable
+        std::vector<int> otherParam;  // template other slot -> parameter (-1: constant below)
+        std::vector<double> otherConst;
+    };
+    std::vector<Shape> shapes_;
+    std::vector<Instance> inst_;
+    std::vector<uint32_t> elemTmpl_;
+    std::vector<uint64_t> elemArg_;  // first argument (diff args first, then others)
+    std::vector<Arg> args_;
+};
+
+}  // namespace Sym

@@ -138,6 +138,10 @@ Kernel::Kernel(const std::vector<Symbolic>& expressions, const int simdWidth_, c
 }
 
 void Kernel::optimizeTemplateExpression(bool simplifyTemplate) {
+    templateExpression = optimizeTemplate(templateExpression, simplifyTemplate);
+}
+
+Symbolic optimizeTemplate(Symbolic templateExpression, bool simplifyTemplate) {
     if (templateExpression.op() == INSTANCE) templateExpression = expandInstance(templateExpression);
 
     // fold constants; algebraically equal sub-expressions become one shared node
@@ -163,7 +167,7 @@ void Kernel::optimizeTemplateExpression(bool simplifyTemplate) {
     std::vector<Symbolic> kernelExpressions;
     kernelExpressions.reserve(order.size());
     for (int i : order) kernelExpressions.push_back(blocks[i].x);
-    templateExpression = Symbolic(BLOCK, kernelExpressions);
+    return Symbolic(BLOCK, kernelExpressions);
 }
 
 int Kernel::getPackLength() {

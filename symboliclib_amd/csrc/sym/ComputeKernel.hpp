@@ -77,4 +77,10 @@ public:
     Symbolic getTemplateExpression() const { return templateExpression; }
 };
 
+// Template optimisation shared by Kernel and the direct assembly builder
+// (src/ComputeKernel.cpp:270-323): expand an INSTANCE, fold constants and merge algebraically
+// equal sub-expressions, extract local definitions (LOCAL_VAR), simplify each block, order
+// definitions before uses.  Returns a BLOCK.
+Symbolic optimizeTemplate(Symbolic templateExpression, bool simplifyTemplate);
+
 }  // namespace Sym

@@ -68,6 +68,19 @@ void ComputeUnitBase::init() {
     opt.decompositionThreshold = (unsigned char)std::min(255u, device.decompositionThreshold);
     opt.simplifyTemplates = device.simplifyTemplates;
     program = buildProgram(outputExpressions, io, opt);
+    outputExpressions.clear();  // the program holds everything needed from here on
+    outputExpressions.shrink_to_fit();
+    finalize();
+}
+
+void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
+    BuildOptions opt;
+    opt.simplifyTemplates = device.simplifyTemplates;
+    program = assembly.build(opt, io, assemblyPatterns);
+    finalize();
+}
+
+void ComputeUnitBase::finalize() {
     EmitOptions eo;
     eo.threadsPerBlock = device.threadsPerBlock();
     tiledForm = false;
@@ -89,8 +102,6 @@ void ComputeUnitBase::init() {
         }
     }
     if (!tiledForm) code = emitHip(program, eo, launches);
-    outputExpressions.clear();  // the program holds everything needed from here on
-    outputExpressions.shrink_to_fit();
 }
 
 void ComputeUnitBase::compileProgram() {

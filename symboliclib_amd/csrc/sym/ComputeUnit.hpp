@@ -23,6 +23,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "Assembly.hpp"
 #include "ContainerSupport.hpp"
 #include "Program.hpp"
 #include "Symbolic.hpp"
@@ -46,9 +47,10 @@ struct UseCuda {};
 struct UseSinglePrecision {};
 // Skip algebraic simplification of kernel templates (faster compile of very large templates).
 struct NoSimplify {};
-// Execution form: LDS-staged work tiles (default when every output's closure fits a tile),
-// or one launch per wave of kernels with intermediates in HBM.
+// Execution form: one launch per wave of kernels with intermediates in HBM (default), or
+// LDS-staged work tiles (UseTiles; used when every output's closure fits a tile).
 struct UseWaves {};
+struct UseTiles {};
 using TileLDSBytes = NamedType<int, struct TileLDSBytesTag>;
 
 class Device {
@@ -60,7 +62,7 @@ public:
     bool cudaDevice = false;
     bool hipDevice = false;
     bool simplifyTemplates = true;
-    bool tiled = true;
+    bool tiled = false;
     unsigned int tileLdsBytes = 64 * 1024;
 
 private:
@@ -71,6 +73,7 @@ private:
     void set(const UseSinglePrecision&) { singlePrecision = true; }
     void set(const NoSimplify&) { simplifyTemplates = false; }
     void set(const UseWaves&) { tiled = false; }
+    void set(const UseTiles&) { tiled = true; }
     void set(const TileLDSBytes& b) { tileLdsBytes = (unsigned)b.x; }
     void set(const UseHIP&) {
         hipDevice = true;
@@ -117,8 +120,12 @@ protected:
     symhip_t rt = nullptr;
     int numOutputContainers = 0;
 
+    std::vector<SparseMatrix<double>> assemblyPatterns;
+
     void addExpressions(const Symbolic* expr, size_t len, int& id);
     void init();
+    void initFromAssembly(const HessianAssembly& assembly);
+    void finalize();
     void setArgImpl(int i, const double* d, size_t n);
     void getResultImpl(int k, double* d, size_t n);
     void runImpl(void* stream);
@@ -138,6 +145,8 @@ public:
     const std::string& getCode() const { return code; }
     const std::vector<LaunchDesc>& getLaunches() const { return launches; }
     bool isTiled() const { return tiledForm; }
+    // output patterns of a unit built from a HessianAssembly (one per instance)
+    const std::vector<SparseMatrix<double>>& outputPatterns() const { return assemblyPatterns; }
     const std::string& tilingInfo() const { return tilingNote; }
     size_t tilePoolSize() const { return tilePool.size(); }
     symhip_t runtime() const { return rt; }
@@ -185,6 +194,12 @@ public:
         int id = 0;
         setExpressions(id, expressions...);
         init();
+    }
+
+    // Unit of a directly assembled element-sum Hessian (one input and one output container per
+    // assembly instance, in instance order).
+    ComputeUnit(Device device_, const HessianAssembly& assembly) : ComputeUnitBase(device_) {
+        initFromAssembly(assembly);
     }
 
     ComputeUnit& compile() {

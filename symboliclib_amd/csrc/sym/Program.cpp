@@ -8,6 +8,7 @@
 #include <stdexcept>
 
 #include "Decomposition.hpp"
+#include "Timer.hpp"
 #include "Utilities.hpp"
 
 namespace Sym {
@@ -29,19 +30,6 @@ std::vector<std::vector<int>> group(const std::vector<ExpressionBlock>& blocks) 
     }
     return ret;
 }
-
-// Phase timer printed to stderr when SYMLIB_TIMING is set (the reference prints its phases
-// through Timer, src/Timer.cpp:71-98).
-struct PhaseTimer {
-    bool on = std::getenv("SYMLIB_TIMING") != nullptr;
-    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
-    void operator()(const char* what) {
-        if (!on) return;
-        auto now = std::chrono::steady_clock::now();
-        std::fprintf(stderr, "[symlib] %-28s %8.3f s\n", what, std::chrono::duration<double>(now - t).count());
-        t = now;
-    }
-};
 
 inline long long varKey(const std::array<int, 2>& v) { return ((long long)v[1] << 32) ^ (unsigned)v[0]; }
 

@@ -193,23 +193,13 @@ std::vector<Symbolic> gradientCached(const Symbolic& x, const std::vector<Symbol
         cache, true);
 }
 
-namespace {
-
-// Shape of one element term: a token string that fixes the DAG (operations, sharing,
-// integer literals) up to the identity of its leaves, plus the leaves in slot order.
-struct ElementShape {
-    std::vector<int64_t> sig;
-    std::vector<Symbolic> diffLeaves;   // differentiated variables, first-occurrence order
-    std::vector<Symbolic> otherLeaves;  // other variables and non-integer constants
-    std::vector<Symbolic> nodes;        // every DAG node, post-order
-    std::vector<std::vector<int>> childIdx;
-    std::vector<int> leafKind;  // 0 interior, 1 diff leaf, 2 other leaf, 3 literal
-    std::vector<int> leafSlot;
-};
-
 bool isLiteralConstant(double c) { return c == (double)(long long)c && c > -1e9 && c < 1e9; }
 
 void analyzeShape(const Symbolic& root, const std::unordered_map<hash_t, size_t>& varId, ElementShape& s) {
+    analyzeShape(root, [&](const Symbolic& v) { return varId.count(v.ahash()) > 0; }, s);
+}
+
+void analyzeShape(const Symbolic& root, const std::function<bool(const Symbolic&)>& isDiff, ElementShape& s) {
     std::unordered_map<long long, int> num;
     std::unordered_map<hash_t, int> diffSlot;
     std::vector<std::pair<const Symbolic*, unsigned>> stack{{&root, 0u}};
@@ -230,8 +220,7 @@ void analyzeShape(const Symbolic& root, const std::unordered_map<hash_t, size_t>
         s.childIdx.emplace_back();
         const OpType op = y->op();
         if (op == VAR) {
-            auto it = varId.find(y->ahash());
-            if (it != varId.end()) {
+            if (isDiff(*y)) {
                 auto ds = diffSlot.find(y->ahash());
                 int slot;
                 if (ds == diffSlot.end()) {
@@ -277,6 +266,8 @@ void analyzeShape(const Symbolic& root, const std::unordered_map<hash_t, size_t>
     }
     s.sig.push_back((int64_t)s.diffLeaves.size());
 }
+
+namespace {
 
 // Differentiate a representative element once, over placeholder arguments.
 const ExprTemplate& buildHessianTemplate(const ElementShape& s, hash_t sigHash) {
@@ -333,6 +324,12 @@ const ExprTemplate& buildHessianTemplate(const ElementShape& s, hash_t sigHash) 
 
 }  // namespace
 
+const ExprTemplate& hessianTemplateFor(const ElementShape& s) {
+    const hash_t sh = hash(s.sig);
+    const ExprTemplate* t = findTemplate(sh, s.sig);
+    return t ? *t : buildHessianTemplate(s, sh);
+}
+
 std::vector<std::tuple<int, int, Symbolic>> hessianSparseVars(const Symbolic& expr, const std::vector<Symbolic>& vars,
                                                               bool shareTemplates) {
     std::unordered_map<hash_t, size_t> varId;
@@ -363,9 +360,7 @@ std::vector<std::tuple<int, int, Symbolic>> hessianSparseVars(const Symbolic& ex
     for (auto& c : expr) {
         ElementShape s;
         analyzeShape(c, varId, s);
-        const hash_t sh = hash(s.sig);
-        const ExprTemplate* t = findTemplate(sh, s.sig);
-        if (!t) t = &buildHessianTemplate(s, sh);
+        const ExprTemplate* t = &hessianTemplateFor(s);
 
         const int nv = (int)s.diffLeaves.size();
         std::vector<Symbolic> children;

@@ -12,7 +12,9 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
+#include <unordered_map>
 #include <vector>
 
 #include "Symbolic.hpp"
@@ -42,5 +44,25 @@ Symbolic expandInstance(const Symbolic& inst);
 
 // Replace VAR(slot, TEMPLATE_ARG) leaves of `x` by args[slot].
 Symbolic substituteArguments(const Symbolic& x, const std::vector<Symbolic>& args);
+
+// Shape of one element term: a token string that fixes the DAG (operations, sharing, integer
+// literals) up to the identity of its leaves, plus the leaves in slot order.  Variables found
+// in `varId` are differentiated arguments (slots 0..nd-1, first-occurrence order); other
+// variables and non-integer constants are further arguments (slots nd..).
+struct ElementShape {
+    std::vector<int64_t> sig;
+    std::vector<Symbolic> diffLeaves;
+    std::vector<Symbolic> otherLeaves;
+    std::vector<Symbolic> nodes;  // every DAG node, post-order
+    std::vector<std::vector<int>> childIdx;
+    std::vector<int> leafKind;  // 0 interior, 1 differentiated leaf, 2 other argument, 3 literal
+    std::vector<int> leafSlot;
+};
+
+bool isLiteralConstant(double c);
+void analyzeShape(const Symbolic& root, const std::unordered_map<hash_t, size_t>& varId, ElementShape& s);
+void analyzeShape(const Symbolic& root, const std::function<bool(const Symbolic&)>& isDiff, ElementShape& s);
+// Hessian template of the element shape over its differentiated arguments (built once per shape).
+const ExprTemplate& hessianTemplateFor(const ElementShape& s);
 
 }  // namespace Sym

@@ -7,6 +7,7 @@
 #include <sstream>
 #include <stdexcept>
 
+#include "Timer.hpp"
 #include "Traverse.hpp"
 
 namespace Sym {
@@ -22,6 +23,7 @@ struct Inst {
 
 TiledProgram buildTiles(const Program& prog, const TileOptions& opt) {
     TiledProgram tp;
+    PhaseTimer timer;
     const size_t K = prog.kernels.size();
     const size_t budget = opt.ldsBytes / sizeof(double);
 
@@ -104,6 +106,7 @@ TiledProgram buildTiles(const Program& prog, const TileOptions& opt) {
     std::vector<uint32_t> slotsOf(numInter);
     for (size_t g = 0; g < numInter; ++g) slotsOf[g] = (uint32_t)prog.kernels[interInst[g].kernel].numOutSlots;
 
+    timer("tiles: producers");
     // ---- seed order: outputs by their first output position (follows the caller's order)
     std::vector<uint32_t> seedOrder(numOut);
     {
@@ -258,6 +261,7 @@ TiledProgram buildTiles(const Program& prog, const TileOptions& opt) {
     }
     tp.numOutputInstances = numOut;
 
+    timer("tiles: growth");
     // ---- tables
     const size_t T = tileOuts.size();
     tp.numTiles = T;
@@ -326,7 +330,9 @@ TiledProgram buildTiles(const Program& prog, const TileOptions& opt) {
     pool[T] = (uint32_t)pool.size();
     tp.colKind.assign(tp.colOffset[K], 0);
     for (size_t k = 0; k < tp.colKind.size(); ++k) tp.colKind[k] = seenLds[k] ? (seenInput[k] ? 2 : 1) : 0;
+    timer("tiles: tables");
     validateTiles(prog, tp);
+    timer("tiles: validate");
     tp.valid = true;
     return tp;
 }

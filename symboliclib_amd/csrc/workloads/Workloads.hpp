@@ -11,10 +11,12 @@
 
 #include <array>
 #include <cmath>
+#include <limits>
 #include <map>
 #include <stdexcept>
 #include <vector>
 
+#include "../sym/Assembly.hpp"
 #include "../sym/ContainerSupport.hpp"
 #include "../sym/Sparse.hpp"
 #include "../sym/Symbolic.hpp"
@@ -117,26 +119,30 @@ inline RestShape precomputeRestShape(const DenseMatrix<double>& V, const Faces& 
     return r;
 }
 
-// Per-face energy -k_stretch |C_stretch|^2 - k_shear C_shear^2 (stretch_shear_forces_impl.hpp:51-73).
+// Energy -k_stretch |C_stretch|^2 - k_shear C_shear^2 of face f (stretch_shear_forces_impl.hpp:51-73).
+template <class T>
+T faceEnergy(const DenseMatrix<T>& X, const std::array<int, 3>& fv, const RestShape& r, size_t f, double kStretch,
+             double kShear) {
+    const Vec3<T> D0 = row(X, fv[1]) - row(X, fv[0]);
+    const Vec3<T> D1 = row(X, fv[2]) - row(X, fv[0]);
+    const auto& Vi = r.Vinv[f];
+    const T v00 = T(Vi[0]), v01 = T(Vi[1]), v10 = T(Vi[2]), v11 = T(Vi[3]);
+    const Vec3<T> J0 = scaled(D0, v00) + scaled(D1, v10);
+    const Vec3<T> J1 = scaled(D0, v01) + scaled(D1, v11);
+    const T a = T(r.area[f]);
+    const T one = T(1.0);
+    const T c0 = a * (norm(J0) - one);
+    const T c1 = a * (norm(J1) - one);
+    const T cs = a * J0.dot(J1);
+    return T(-kStretch) * (c0 * c0 + c1 * c1) - T(kShear) * cs * cs;
+}
+
 template <class T>
 std::vector<T> stretchShearEnergy(const DenseMatrix<T>& X, const Faces& F, const RestShape& r, double kStretch,
                                   double kShear) {
     std::vector<T> ret;
     ret.reserve(F.size());
-    for (size_t f = 0; f < F.size(); ++f) {
-        const Vec3<T> D0 = row(X, F[f][1]) - row(X, F[f][0]);
-        const Vec3<T> D1 = row(X, F[f][2]) - row(X, F[f][0]);
-        const auto& Vi = r.Vinv[f];
-        const T v00 = T(Vi[0]), v01 = T(Vi[1]), v10 = T(Vi[2]), v11 = T(Vi[3]);
-        const Vec3<T> J0 = scaled(D0, v00) + scaled(D1, v10);
-        const Vec3<T> J1 = scaled(D0, v01) + scaled(D1, v11);
-        const T a = T(r.area[f]);
-        const T one = T(1.0);
-        const T c0 = a * (norm(J0) - one);
-        const T c1 = a * (norm(J1) - one);
-        const T cs = a * J0.dot(J1);
-        ret.push_back(T(-kStretch) * (c0 * c0 + c1 * c1) - T(kShear) * cs * cs);
-    }
+    for (size_t f = 0; f < F.size(); ++f) ret.push_back(faceEnergy(X, F[f], r, f, kStretch, kShear));
     return ret;
 }
 
@@ -228,6 +234,73 @@ inline SparseMatrix<Symbolic> baraffWitkinHessian(const DenseMatrix<Symbolic>& X
     SparseMatrix<Symbolic> H(3 * n, 3 * n);
     H.setFromTriplets(triplets.begin(), triplets.end());
     return H;
+}
+
+// Element shapes of the face energy, one per pattern of integer-valued rest constants (an
+// integer constant stays a literal of the expression, as in the per-face trees; the others
+// become parameters).  Shared by all instances of an assembly.
+class FaceShapes {
+    std::map<std::array<int64_t, 5>, int> cache_;
+    double kStretch_, kShear_;
+
+public:
+    FaceShapes(double kStretch, double kShear) : kStretch_(kStretch), kShear_(kShear) {}
+
+    // shape of face f and its parameters (the non-integer constants among area, Vinv)
+    int get(HessianAssembly& A, const RestShape& r, size_t f, double params[5]) {
+        const double c[5] = {r.area[f], r.Vinv[f][0], r.Vinv[f][1], r.Vinv[f][2], r.Vinv[f][3]};
+        std::array<int64_t, 5> key;
+        for (int k = 0; k < 5; ++k) {
+            // literal: the integer value; parameter: a code no integer value can take
+            const bool lit = c[k] == (double)(long long)c[k] && c[k] > -1e9 && c[k] < 1e9;
+            key[k] = lit ? (int64_t)c[k] : std::numeric_limits<int64_t>::min();
+            params[k] = c[k];
+        }
+        auto it = cache_.find(key);
+        if (it != cache_.end()) return it->second;
+        // build the energy once over local variables and parameters
+        DenseMatrix<Symbolic> XL(3, 3);
+        for (int i = 0; i < 3; ++i)
+            for (int d = 0; d < 3; ++d) XL(i, d) = Symbolic(3 * i + d, ELEMENT_LOCAL);
+        RestShape rl;
+        Symbolic sym[5];
+        for (int k = 0; k < 5; ++k)
+            sym[k] = key[k] == std::numeric_limits<int64_t>::min() ? Symbolic(k, ELEMENT_PARAM) : Symbolic(c[k]);
+        const Vec3<Symbolic> D0 = row(XL, 1) - row(XL, 0);
+        const Vec3<Symbolic> D1 = row(XL, 2) - row(XL, 0);
+        const Vec3<Symbolic> J0 = scaled(D0, sym[1]) + scaled(D1, sym[3]);
+        const Vec3<Symbolic> J1 = scaled(D0, sym[2]) + scaled(D1, sym[4]);
+        const Symbolic one(1.0);
+        const Symbolic c0 = sym[0] * (norm(J0) - one);
+        const Symbolic c1 = sym[0] * (norm(J1) - one);
+        const Symbolic cs = sym[0] * J0.dot(J1);
+        const Symbolic e = Symbolic(-kStretch_) * (c0 * c0 + c1 * c1) - Symbolic(kShear_) * cs * cs;
+        const int id = A.defineShape(e, 9, 5);
+        cache_[key] = id;
+        return id;
+    }
+};
+
+// The same Hessian as one instance of a direct assembly; variables are {v + c n, group}.
+// The face energy is the expression of faceEnergy() with the rest constants as parameters.
+inline void baraffWitkinAssembly(HessianAssembly& A, FaceShapes& shapes, const DenseMatrix<double>& rest,
+                                 const Faces& F, double kBend, int group) {
+    const int n = rest.rows();
+    A.addInstance(3 * n, group);
+    const RestShape r = precomputeRestShape(rest, F);
+    double params[5];
+    int vars[9];
+    for (size_t f = 0; f < F.size(); ++f) {
+        const int s = shapes.get(A, r, f, params);
+        for (int i = 0; i < 3; ++i)
+            for (int d = 0; d < 3; ++d) vars[3 * i + d] = F[f][i] + d * n;
+        A.addElement(s, vars, params);
+    }
+    if (kBend != 0.0) {
+        std::vector<Triplet<double>> bend;
+        appendBendTriplets(rest, F, kBend, n, bend);
+        for (auto& t : bend) A.addConstant(t.row(), t.col(), t.value());
+    }
 }
 
 }  // namespace workloads

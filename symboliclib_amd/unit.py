@@ -16,8 +16,8 @@ from . import _native
 from .workloads import to_colmajor
 
 
-def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True, tiled=True,
-          tile_lds_bytes=0):
+def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True, tiled=False,
+          tile_lds_bytes=0, direct_assembly=False):
     o = _native.Options()
     _native.lib().symlib_default_options(ctypes.byref(o))
     o.threads_per_block = int(threads_per_block)
@@ -26,6 +26,7 @@ def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, shar
     o.share_templates = int(bool(share_templates))
     o.tiled = int(bool(tiled))
     o.tile_lds_bytes = int(tile_lds_bytes)
+    o.direct_assembly = int(bool(direct_assembly))
     return o
 
 
@@ -67,6 +68,26 @@ class Unit:
         _native.check(L.symlib_unit_bw_hessian(ctypes.byref(h), len(rest), len(F) // 3, F.ctypes.data, R.ctypes.data,
                                                k_stretch, k_shear, k_bend, ctypes.byref(o)), "symlib_unit_bw_hessian")
         return cls(h, [3 * len(rest)])
+
+    @classmethod
+    def bw_hessian_batch(cls, cloths, k_stretch, k_shear, k_bend, **opt):
+        """cloths: list of (rest, faces); one program, cloth k = input k and output k."""
+        L = _native.lib()
+        Fs = [np.ascontiguousarray(F, np.int32).reshape(-1) for _, F in cloths]
+        Rs = [to_colmajor(r) for r, _ in cloths]
+        nv = np.array([len(r) for r, _ in cloths], np.int32)
+        nf = np.array([len(F) // 3 for F in Fs], np.int32)
+        fp = (ctypes.c_void_p * len(Fs))(*[F.ctypes.data for F in Fs])
+        rp = (ctypes.c_void_p * len(Rs))(*[R.ctypes.data for R in Rs])
+        h = ctypes.c_void_p()
+        o = _opts(**opt)
+        _native.check(L.symlib_unit_bw_hessian_batch(ctypes.byref(h), len(cloths), nv.ctypes.data, nf.ctypes.data,
+                                                     ctypes.cast(fp, ctypes.c_void_p), ctypes.cast(rp, ctypes.c_void_p),
+                                                     k_stretch, k_shear, k_bend, ctypes.byref(o)),
+                      "symlib_unit_bw_hessian_batch")
+        u = cls(h, [3 * int(v) for v in nv])
+        u._keep = [Fs, Rs]
+        return u
 
     # ---- ComputeUnit surface
     def compile(self):

@@ -133,10 +133,10 @@ def test_template_sharing_matches_direct_tables():
 
 
 @pytest.mark.parametrize("side", [8, 24])
-def test_tiled_form_is_default_and_compiles(tmp_path, side):
+def test_tiled_form_compiles(tmp_path, side):
     L = _native.lib()
     rest, F, X = W.c3_cloth(side=side)
-    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND) as u:
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tiled=True) as u:
         info = u.info()
         assert info["tiled"] == 1, info["tiling"]
         fields = dict(kv.split("=") for kv in info["tiling"].split())
@@ -159,6 +159,6 @@ def test_waves_form_on_request():
 def test_tiling_with_tiny_budget_falls_back():
     # an output that needs more intermediates than one tile holds cannot be tiled
     rest, F, X = W.c3_cloth(side=4)
-    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tile_lds_bytes=256) as u:
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tiled=True, tile_lds_bytes=256) as u:
         info = u.info()
         assert info["tiled"] == 0 and info["tiling"].startswith("untiled")

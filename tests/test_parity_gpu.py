@@ -23,9 +23,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TOL = 1e-12
 
 
-def _check(name, u, ref, values):
+def _check(name, u, ref, values, k=0):
     cp, ri, rv, sc = ref
-    outer, inner = u.output_pattern()
+    outer, inner = u.output_pattern(k)
     assert np.array_equal(outer, cp), f"{name}: outer index array differs"
     assert np.array_equal(inner, ri), f"{name}: inner index array differs"
     err = O.rel_errors(values, rv, sc)
@@ -107,7 +107,7 @@ def test_single_entry_matrix(require_device):
 @pytest.mark.parametrize("lds", [16 * 1024, 160 * 1024])
 def test_bw_hessian_tile_budgets(require_device, lds):
     rest, F, X = W.c3_cloth(side=24)
-    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tile_lds_bytes=lds) as u:
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tiled=True, tile_lds_bytes=lds) as u:
         assert u.info()["tiled"] == 1, u.info()["tiling"]
         u.compile().execute(W.to_colmajor(X))
         ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
@@ -122,3 +122,23 @@ def test_matrix_with_empty_columns(require_device):
     with Unit.ata_plus_a(cp, ri) as u:
         u.compile().execute(a)
         _check("empty columns", u, O.ata_plus_a(cp, ri, a), u.get_results())
+
+
+@pytest.mark.parametrize("side,tiled", [(16, False), (16, True), (256, False)])
+def test_bw_hessian_direct_assembly(require_device, side, tiled):
+    rest, F, X = W.c3_cloth(side=side)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, direct_assembly=True, tiled=tiled) as u:
+        u.compile().execute(W.to_colmajor(X))
+        ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+        _check(f"Baraff-Witkin {side}x{side} direct assembly tiled={tiled}", u, ref, u.get_results())
+
+
+@pytest.mark.parametrize("tiled", FORMS)
+def test_bw_hessian_batch(require_device, tiled):
+    sides = [5, 17, 32, 9, 40]
+    cloths = [W.cloth(s, seed=100 + s) for s in sides]
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, tiled=tiled) as u:
+        u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
+        for k, (rest, F, X) in enumerate(cloths):
+            ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+            _check(f"batch cloth {k} ({sides[k]}x{sides[k]}) tiled={tiled}", u, ref, u.get_results(k), k)
