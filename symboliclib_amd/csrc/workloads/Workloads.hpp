@@ -152,29 +152,34 @@ std::vector<T> stretchShearEnergy(const DenseMatrix<T>& X, const Faces& F, const
 // (examples/BaraffWitkin/adjacency.cpp:131-203).  Edges are enumerated in increasing
 // (min, max) vertex order.
 inline std::vector<std::array<int, 4>> interiorEdgeQuads(const Faces& F) {
-    std::map<std::pair<int, int>, std::vector<int>> edgeFaces;
+    // half-edges sorted by (min vertex, max vertex, face, corner)
+    struct HalfEdge {
+        int lo, hi, f, k;
+    };
+    std::vector<HalfEdge> he;
+    he.reserve(3 * F.size());
     for (int f = 0; f < (int)F.size(); ++f)
         for (int k = 0; k < 3; ++k) {
-            int a = F[f][k], b = F[f][(k + 1) % 3];
-            edgeFaces[{std::min(a, b), std::max(a, b)}].push_back(f);
+            const int a = F[f][k], b = F[f][(k + 1) % 3];
+            he.push_back({std::min(a, b), std::max(a, b), f, k});
         }
+    std::sort(he.begin(), he.end(), [](const HalfEdge& x, const HalfEdge& y) {
+        return x.lo != y.lo ? x.lo < y.lo : x.hi != y.hi ? x.hi < y.hi : x.f != y.f ? x.f < y.f : x.k < y.k;
+    });
     std::vector<std::array<int, 4>> E4;
-    for (auto& e : edgeFaces) {
-        if (e.second.size() != 2) continue;
-        const int fa = e.second[0], fb = e.second[1];
-        // orient (v0, v1) as in face fa, v2 = third vertex of fa, v3 = third vertex of fb
-        int v0 = -1, v1 = -1, v2 = -1, v3 = -1;
-        for (int k = 0; k < 3; ++k) {
-            const int a = F[fa][k], b = F[fa][(k + 1) % 3];
-            if (std::min(a, b) == e.first.first && std::max(a, b) == e.first.second) {
-                v0 = a;
-                v1 = b;
-                v2 = F[fa][(k + 2) % 3];
-            }
+    for (size_t s = 0; s < he.size();) {
+        size_t e = s + 1;
+        while (e < he.size() && he[e].lo == he[s].lo && he[e].hi == he[s].hi) ++e;
+        if (e - s == 2) {
+            // orient (v0, v1) as in the first face, v2 = its third vertex, v3 = third vertex of the second
+            const int fa = he[s].f, ka = he[s].k, fb = he[s + 1].f;
+            const int v0 = F[fa][ka], v1 = F[fa][(ka + 1) % 3], v2 = F[fa][(ka + 2) % 3];
+            int v3 = -1;
+            for (int k = 0; k < 3; ++k)
+                if (F[fb][k] != v0 && F[fb][k] != v1) v3 = F[fb][k];
+            E4.push_back({v0, v1, v2, v3});
         }
-        for (int k = 0; k < 3; ++k)
-            if (F[fb][k] != v0 && F[fb][k] != v1) v3 = F[fb][k];
-        E4.push_back({v0, v1, v2, v3});
+        s = e;
     }
     return E4;
 }
@@ -236,9 +241,10 @@ inline SparseMatrix<Symbolic> baraffWitkinHessian(const DenseMatrix<Symbolic>& X
     return H;
 }
 
-// Element shapes of the face energy, one per pattern of integer-valued rest constants (an
-// integer constant stays a literal of the expression, as in the per-face trees; the others
-// become parameters).  Shared by all instances of an assembly.
+// Element shapes of the face energy, one per pattern of zero rest constants (a zero stays a
+// literal, so the terms it multiplies vanish from the derivatives as in the per-face trees; all
+// other constants become parameters, so cloths of every size share the shapes).  Shared by all
+// instances of an assembly.
 class FaceShapes {
     std::map<std::array<int64_t, 5>, int> cache_;
     double kStretch_, kShear_;
@@ -251,9 +257,8 @@ public:
         const double c[5] = {r.area[f], r.Vinv[f][0], r.Vinv[f][1], r.Vinv[f][2], r.Vinv[f][3]};
         std::array<int64_t, 5> key;
         for (int k = 0; k < 5; ++k) {
-            // literal: the integer value; parameter: a code no integer value can take
-            const bool lit = c[k] == (double)(long long)c[k] && c[k] > -1e9 && c[k] < 1e9;
-            key[k] = lit ? (int64_t)c[k] : std::numeric_limits<int64_t>::min();
+            // literal zero, or parameter (a code no literal takes)
+            key[k] = c[k] == 0.0 ? 0 : std::numeric_limits<int64_t>::min();
             params[k] = c[k];
         }
         auto it = cache_.find(key);

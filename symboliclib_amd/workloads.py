@@ -143,3 +143,17 @@ def c2_mesh(side=C2_SIDE, seed=C2_SEED):
 
 def c3_cloth(side=C3_SIDE, seed=C3_SEED):
     return cloth(side, seed)
+
+
+C4_COUNT, C4_MIN_SIDE, C4_MAX_SIDE, C4_SEED = 256, 32, 256, 4
+
+
+def c4_sides(count=C4_COUNT, lo=C4_MIN_SIDE, hi=C4_MAX_SIDE, seed=C4_SEED):
+    """Side lengths drawn log-uniformly from [lo, hi] (rounded)."""
+    rng = np.random.default_rng(seed)
+    return [int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi)))))) for _ in range(count)]
+
+
+def c4_cloths(count=C4_COUNT, lo=C4_MIN_SIDE, hi=C4_MAX_SIDE, seed=C4_SEED):
+    """Batched cloths: one (rest, F, X) per instance, each with its own deformation seed."""
+    return [cloth(s, seed * 100003 + k) for k, s in enumerate(c4_sides(count, lo, hi, seed))]
