@@ -48,10 +48,14 @@ enum {
 
 typedef struct symhip_program* symhip_t;
 
+/* Grid of a persistent launch: as many workgroups as the device keeps resident at once
+ * (occupancy of the kernel at its block and LDS size times the compute units). */
+#define SYMHIP_GRID_FILL 0xffffffffu
+
 /* One kernel launch of a program; launches run in order on one stream. */
 typedef struct {
     const char* name; /* extern "C" __global__ function in the compiled source */
-    uint32_t grid;    /* workgroups */
+    uint32_t grid;    /* workgroups, or SYMHIP_GRID_FILL */
     uint32_t block;   /* work-items per workgroup */
     uint32_t shared;  /* dynamic LDS bytes per workgroup */
 } symhip_launch;
@@ -65,8 +69,10 @@ typedef struct {
  *   index     : u32 gather table (positions inside x)
  *   consts    : FP64 per-instance constants
  *   outpos    : u32 placement table (positions inside the result buffer)
- *   tiles     : u32 tile pool of the tile-fused form (may be empty)
- * Generated kernels take (x, x, consts, index, outpos, result, tiles) in that order.
+ *   tiles     : u32 tile pool of the tiled and fused forms (may be empty)
+ *   work      : u32 words of device scratch, zero at creation (work-queue counters of
+ *               persistent launches, which leave them zero when they finish)
+ * Generated kernels take (x, x, consts, index, outpos, result, tiles, work) in that order.
  */
 typedef struct {
     uint64_t num_values;
@@ -86,6 +92,7 @@ typedef struct {
     const uint32_t* tiles_data;
     uint32_t num_launches;
     const symhip_launch* launches;
+    uint64_t num_work_words;
 } symhip_program_desc;
 
 /* 1 when a HIP device is visible, 0 otherwise (never initialises more than the runtime). */
@@ -105,6 +112,11 @@ int symhip_compile(symhip_t h, const char* source, const char* options);
  * (used to fill the cache ahead of time).  `arch` e.g. "gfx950". */
 int symhip_compile_to_file(const char* source, const char* options, const char* arch, const char* out_path);
 
+/* Compile `source` into the on-disk code-object cache for `arch` (NULL = the current device's,
+ * gfx950 without one) unless it is there already, so that a later symhip_compile of the same
+ * source and options only loads it.  Thread-safe: several sources may be compiled at once. */
+int symhip_cache_compile(const char* source, const char* options, const char* arch);
+
 /* Upload input container i from host memory (reference: setArg).  n must equal its length. */
 int symhip_set_arg(symhip_t h, int i, const double* host, size_t n);
 /* Same from device memory, asynchronously on `stream` (hipStream_t, NULL = default). */
@@ -112,9 +124,15 @@ int symhip_set_arg_device(symhip_t h, int i, const double* dev, size_t n, void* 
 
 /* Enqueue every launch on `stream` (reference: run()).  Does not synchronise. */
 int symhip_run(symhip_t h, void* stream);
+/* Same launches replayed from a HIP graph, captured on the first call (the program's
+ * launches are the graph's only nodes: no memset or copy nodes).  Later calls only launch
+ * the graph on `stream`. */
+int symhip_run_graph(symhip_t h, void* stream);
 /* Enqueue launch `idx` only (for per-kernel timing). */
 int symhip_launch_one(symhip_t h, uint32_t idx, void* stream);
 uint32_t symhip_num_launches(symhip_t h);
+/* Workgroups of launch `idx` as launched (SYMHIP_GRID_FILL resolved after compile). */
+uint32_t symhip_launch_grid(symhip_t h, uint32_t idx);
 
 /* Copy output container k to host memory after synchronising (reference: getResult). */
 int symhip_get_result(symhip_t h, int k, double* host, size_t n);

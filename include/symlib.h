@@ -40,6 +40,13 @@ typedef struct {
     int tile_lds_bytes;          /* LDS budget per tile, 0 = default (64 KiB) */
     int direct_assembly;         /* cloth Hessian: 1 = HessianAssembly (no per-nonzero expressions),
                                     0 = symbolic SparseMatrix through ComputeUnit (default) */
+    int fused;                   /* cloth Hessian: 1 = fused assembly form (implies direct_assembly):
+                                    tiles of owner vertices, element Hessians in LDS, one launch */
+    int schedule;                /* fused form: 0 one workgroup per tile, 1 persistent work queue
+                                    (default), 2 queue with largest tiles first, 3 queue with
+                                    instances interleaved */
+    int output_groups;           /* fused form: work items per element, 0 = default (5) */
+    int waves_per_eu;            /* fused form: minimum waves per SIMD to compile for, 0 = none */
 } symlib_options;
 
 typedef struct {
@@ -55,7 +62,7 @@ typedef struct {
     double frontend_seconds;    /* symbolic front-end (building the expressions) */
     double build_seconds;       /* ComputeUnit construction (decompose .. emit) */
     double compile_seconds;     /* hipRTC + table upload */
-    uint32_t tiled;             /* 1 = tile-fused form */
+    uint32_t tiled;             /* 0 = wave launches, 1 = tile-fused form, 2 = fused assembly form */
     uint64_t tile_pool_entries; /* u32 entries of the tile tables */
     char tiling[128];           /* tile count, LDS bytes and recomputation factor, or why untiled */
 } symlib_unit_info;
@@ -79,6 +86,8 @@ int symlib_unit_compile(symlib_unit_t u);
 int symlib_unit_execute(symlib_unit_t u, int num_inputs, const double* const* inputs, const size_t* lengths);
 /* enqueue the program on a stream without copying inputs (they are already on the device) */
 int symlib_unit_run(symlib_unit_t u, void* stream);
+/* same, replayed from a HIP graph captured on the first call */
+int symlib_unit_run_graph(symlib_unit_t u, void* stream);
 int symlib_unit_get_results(symlib_unit_t u, int k, double* out, size_t n);
 
 int symlib_unit_num_outputs(symlib_unit_t u);

@@ -17,7 +17,8 @@ class SymlibError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("threads_per_block", ctypes.c_int), ("decomposition_threshold", ctypes.c_int),
                 ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int), ("tiled", ctypes.c_int),
-                ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int)]
+                ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int), ("fused", ctypes.c_int),
+                ("schedule", ctypes.c_int), ("output_groups", ctypes.c_int), ("waves_per_eu", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):
@@ -43,7 +44,7 @@ class ProgramDesc(ctypes.Structure):
                 ("const_data", ctypes.POINTER(ctypes.c_double)), ("num_outpos", ctypes.c_uint64),
                 ("outpos_data", ctypes.POINTER(ctypes.c_uint32)), ("num_tiles_data", ctypes.c_uint64),
                 ("tiles_data", ctypes.POINTER(ctypes.c_uint32)), ("num_launches", ctypes.c_uint32),
-                ("launches", ctypes.POINTER(Launch))]
+                ("launches", ctypes.POINTER(Launch)), ("num_work_words", ctypes.c_uint64)]
 
 
 # symbol -> (restype, argtypes); every exported C function of include/symhip.h and include/symlib.h
@@ -56,11 +57,14 @@ SIGNATURES = {
     "symhip_create": (_I, [ctypes.POINTER(_VP), ctypes.POINTER(ProgramDesc)]),
     "symhip_compile": (_I, [_VP, ctypes.c_char_p, ctypes.c_char_p]),
     "symhip_compile_to_file": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "symhip_cache_compile": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
     "symhip_set_arg": (_I, [_VP, _I, _DP, _SZ]),
     "symhip_set_arg_device": (_I, [_VP, _I, _VP, _SZ, _VP]),
     "symhip_run": (_I, [_VP, _VP]),
+    "symhip_run_graph": (_I, [_VP, _VP]),
     "symhip_launch_one": (_I, [_VP, _U32, _VP]),
     "symhip_num_launches": (_U32, [_VP]),
+    "symhip_launch_grid": (_U32, [_VP, _U32]),
     "symhip_get_result": (_I, [_VP, _I, _DP, _SZ]),
     "symhip_get_result_device": (_I, [_VP, _I, _VP, _SZ, _VP]),
     "symhip_output_device_ptr": (_VP, [_VP, _I]),
@@ -77,6 +81,7 @@ SIGNATURES = {
     "symlib_unit_compile": (_I, [_VP]),
     "symlib_unit_execute": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_SZ)]),
     "symlib_unit_run": (_I, [_VP, _VP]),
+    "symlib_unit_run_graph": (_I, [_VP, _VP]),
     "symlib_unit_get_results": (_I, [_VP, _I, _VP, _SZ]),
     "symlib_unit_num_outputs": (_I, [_VP]),
     "symlib_unit_output_shape": (_I, [_VP, _I, _IP, _IP, ctypes.POINTER(_U64)]),

@@ -40,7 +40,14 @@ Sym::Device makeDevice(const symlib_options* opt) {
     if (o.decomposition_threshold >= 0) d.decompositionThreshold = (unsigned)o.decomposition_threshold;
     d.simplifyTemplates = o.simplify != 0;
     d.tiled = o.tiled != 0;
-    if (o.tile_lds_bytes > 0) d.tileLdsBytes = (unsigned)o.tile_lds_bytes;
+    if (o.tile_lds_bytes > 0) {
+        d.tileLdsBytes = (unsigned)o.tile_lds_bytes;
+        d.fusedLdsBytes = (unsigned)o.tile_lds_bytes;
+    }
+    d.fused = o.fused != 0;
+    d.schedule = o.schedule;
+    if (o.output_groups > 0) d.outputGroups = o.output_groups;
+    d.wavesPerEU = o.waves_per_eu;
     return d;
 }
 
@@ -92,6 +99,15 @@ void finishAssembly(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0
     u->info.source_bytes = u->unit->getCode().size();
     u->info.tiled = u->unit->isTiled() ? 1 : 0;
     u->info.tile_pool_entries = u->unit->tilePoolSize();
+    if (u->unit->isFused()) {
+        const auto& F = u->unit->getFusedProgram();
+        u->info.num_const = F.consts.size();
+        u->info.num_results = F.numResults;
+        u->info.num_kernels = (uint32_t)F.templates.size();
+        u->info.num_waves = 1;
+        u->info.tiled = 2;
+        u->info.tile_pool_entries = F.pool.size();
+    }
     std::snprintf(u->info.tiling, sizeof(u->info.tiling), "%s", u->unit->tilingInfo().c_str());
     u->info.frontend_seconds = seconds(t0, t1);
     u->info.build_seconds = seconds(t1, t2);
@@ -131,6 +147,10 @@ void symlib_default_options(symlib_options* opt) {
     opt->tiled = 0;
     opt->tile_lds_bytes = 0;
     opt->direct_assembly = 0;
+    opt->fused = 0;
+    opt->schedule = 1;
+    opt->output_groups = 0;
+    opt->waves_per_eu = 0;
 }
 
 int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {
@@ -184,7 +204,7 @@ int symlib_unit_bw_hessian(symlib_unit_t* out, int nv, int nf, const int* faces,
         std::unique_ptr<symlib_unit> u(new symlib_unit());
         const auto t0 = Clock::now();
         Sym::DenseMatrix<double> R(nv, 3, rest);
-        if (opt && opt->direct_assembly) {
+        if (opt && (opt->direct_assembly || opt->fused)) {
             Sym::HessianAssembly A;
             Sym::workloads::FaceShapes shapes(k_stretch, k_shear);
             Sym::workloads::baraffWitkinAssembly(A, shapes, R, toFaces(nf, faces), k_bend, 0);
@@ -246,6 +266,11 @@ int symlib_unit_execute(symlib_unit_t u, int num_inputs, const double* const* in
 int symlib_unit_run(symlib_unit_t u, void* stream) {
     if (!u) return 1;
     return guarded([&] { u->unit->run(stream); });
+}
+
+int symlib_unit_run_graph(symlib_unit_t u, void* stream) {
+    if (!u) return 1;
+    return guarded([&] { u->unit->runGraph(stream); });
 }
 
 int symlib_unit_get_results(symlib_unit_t u, int k, double* out, size_t n) {

@@ -8,6 +8,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -124,7 +125,8 @@ bool readFile(const std::string& path, std::vector<char>& data) {
 }
 
 bool writeFileAtomic(const std::string& path, const std::vector<char>& data) {
-    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    static std::atomic<unsigned> serial{0};
+    std::string tmp = path + ".tmp" + std::to_string(getpid()) + "_" + std::to_string(serial++);
     {
         std::ofstream f(tmp, std::ios::binary);
         if (!f) return false;
@@ -145,7 +147,11 @@ struct symhip_program {
     uint32_t* p = nullptr;
     uint32_t* o = nullptr;
     uint32_t* tiles = nullptr;
+    uint32_t* work = nullptr;
     std::vector<symhip_launch> launches;
+    std::vector<uint32_t> grids;  // resolved workgroup counts
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
     std::vector<std::string> names;
     hipModule_t module = nullptr;
     std::vector<hipFunction_t> functions;
@@ -166,13 +172,16 @@ const char* symhip_last_error(void) { return g_lastError.c_str(); }
 
 void symhip_destroy(symhip_t h) {
     if (!h) return;
-    if (h->module) hipModuleUnload(h->module);
+    if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    if (h->module) (void)hipModuleUnload(h->module);
     hipFree(h->x);
     hipFree(h->result);
     hipFree(h->c);
     hipFree(h->p);
     hipFree(h->o);
     hipFree(h->tiles);
+    hipFree(h->work);
     delete h;
 }
 
@@ -217,7 +226,8 @@ int symhip_create(symhip_t* out, const symhip_program_desc* d) {
         (rc = upload((void**)&h->p, d->index_data, sizeof(uint32_t) * d->num_index, "upload(index)")) ||
         (rc = upload((void**)&h->c, d->const_data, sizeof(double) * d->num_const, "upload(consts)")) ||
         (rc = upload((void**)&h->o, d->outpos_data, sizeof(uint32_t) * d->num_outpos, "upload(outpos)")) ||
-        (rc = upload((void**)&h->tiles, d->tiles_data, sizeof(uint32_t) * d->num_tiles_data, "upload(tiles)"))) {
+        (rc = upload((void**)&h->tiles, d->tiles_data, sizeof(uint32_t) * d->num_tiles_data, "upload(tiles)")) ||
+        (rc = upload((void**)&h->work, nullptr, sizeof(uint32_t) * d->num_work_words, "hipMalloc(work)"))) {
         symhip_destroy(h);
         return rc;
     }
@@ -231,6 +241,21 @@ int symhip_compile_to_file(const char* source, const char* options, const char* 
     int rc = compileSource(source, splitOptions(options), arch ? arch : "gfx950", code);
     if (rc) return rc;
     if (!writeFileAtomic(out_path, code)) return fail(SYMHIP_E_IO, std::string("cannot write ") + out_path);
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_cache_compile(const char* source, const char* options, const char* arch) {
+    if (!source) return fail(SYMHIP_E_INVALID, "symhip_cache_compile: null source");
+    const std::string a = arch ? std::string(arch) : defaultArch();
+    const std::string dir = cacheDir();
+    const std::string path = dir + "/" + cacheKey(source, options, a) + "." + a + ".co";
+    struct stat st;
+    if (stat(path.c_str(), &st) == 0 && st.st_size > 0) return SYMHIP_SUCCESS;
+    std::vector<char> code;
+    int rc = compileSource(source, splitOptions(options), a, code);
+    if (rc) return rc;
+    mkdir(dir.c_str(), 0755);
+    if (!writeFileAtomic(path, code)) return fail(SYMHIP_E_IO, "cannot write " + path);
     return SYMHIP_SUCCESS;
 }
 
@@ -264,11 +289,32 @@ int symhip_compile(symhip_t h, const char* source, const char* options) {
         }
         h->functions.push_back(f);
     }
-    for (size_t i = 0; i < h->launches.size(); ++i)
-        if (h->launches[i].shared > 65536)
+    h->grids.assign(h->launches.size(), 0);
+    for (size_t i = 0; i < h->launches.size(); ++i) {
+        const symhip_launch& L = h->launches[i];
+        if (L.shared > 65536)
             HIP_TRY(hipFuncSetAttribute((const void*)h->functions[i], hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)h->launches[i].shared),
+                                        (int)L.shared),
                     "hipFuncSetAttribute(max dynamic LDS)");
+        h->grids[i] = L.grid;
+        if (L.grid == SYMHIP_GRID_FILL) {
+            int perCU = 0, dev = 0, cus = 0;
+            HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, h->functions[i], (int)L.block, L.shared),
+                    "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+            HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+            HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+            if (perCU <= 0) return fail(SYMHIP_E_INVALID, "symhip_compile: " + h->names[i] + " cannot be resident (block or LDS too large)");
+            h->grids[i] = (uint32_t)perCU * (uint32_t)cus;
+        }
+    }
+    if (h->graphExec) {
+        (void)hipGraphExecDestroy(h->graphExec);
+        h->graphExec = nullptr;
+    }
+    if (h->graph) {
+        (void)hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
     return SYMHIP_SUCCESS;
 }
 
@@ -292,15 +338,53 @@ int symhip_set_arg_device(symhip_t h, int i, const double* dev, size_t n, void* 
 
 uint32_t symhip_num_launches(symhip_t h) { return h ? (uint32_t)h->launches.size() : 0; }
 
+uint32_t symhip_launch_grid(symhip_t h, uint32_t idx) {
+    if (!h || idx >= h->launches.size()) return 0;
+    return idx < h->grids.size() ? h->grids[idx] : h->launches[idx].grid;
+}
+
 int symhip_launch_one(symhip_t h, uint32_t idx, void* stream) {
     if (!h || idx >= h->launches.size()) return fail(SYMHIP_E_INVALID, "symhip_launch_one: bad launch index");
     if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
     const symhip_launch& L = h->launches[idx];
-    if (L.grid == 0) return SYMHIP_SUCCESS;
-    void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result, &h->tiles};
-    HIP_TRY(hipModuleLaunchKernel(h->functions[idx], L.grid, 1, 1, L.block, 1, 1, L.shared, (hipStream_t)stream, args,
+    const uint32_t grid = h->grids[idx];
+    if (grid == 0) return SYMHIP_SUCCESS;
+    void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result, &h->tiles, &h->work};
+    HIP_TRY(hipModuleLaunchKernel(h->functions[idx], grid, 1, 1, L.block, 1, 1, L.shared, (hipStream_t)stream, args,
                                   nullptr),
             "hipModuleLaunchKernel");
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_run_graph(symhip_t h, void* stream) {
+    if (!h) return fail(SYMHIP_E_INVALID, "symhip_run_graph: null handle");
+    if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
+    if (!h->graphExec) {
+        hipStream_t cs = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        int rc = SYMHIP_SUCCESS;
+        if (e == hipSuccess) {
+            for (uint32_t i = 0; i < h->launches.size() && !rc; ++i) rc = symhip_launch_one(h, i, cs);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(cs, &g);
+            if (!rc && e2 != hipSuccess) rc = hipFail(e2, "hipStreamEndCapture");
+            if (!rc) {
+                e2 = hipGraphInstantiate(&h->graphExec, g, nullptr, nullptr, 0);
+                if (e2 != hipSuccess) rc = hipFail(e2, "hipGraphInstantiate");
+            }
+            if (!rc) h->graph = g;
+            else if (g) (void)hipGraphDestroy(g);
+        } else {
+            rc = hipFail(e, "hipStreamBeginCapture");
+        }
+        (void)hipStreamDestroy(cs);
+        if (rc) {
+            h->graphExec = nullptr;
+            return rc;
+        }
+    }
+    HIP_TRY(hipGraphLaunch(h->graphExec, (hipStream_t)stream), "hipGraphLaunch");
     return SYMHIP_SUCCESS;
 }
 

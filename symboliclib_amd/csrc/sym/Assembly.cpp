@@ -12,13 +12,15 @@
 
 namespace Sym {
 
-int HessianAssembly::addInstance(int numVars, int group) {
+int HessianAssembly::addInstance(int numVars, int group, int dofs) {
     if (numVars <= 0) throw std::invalid_argument("HessianAssembly: instance without variables");
+    if (dofs <= 0 || numVars % dofs) throw std::invalid_argument("HessianAssembly: dofs must divide the variable count");
     for (auto& I : inst_)
         if (I.group == group) throw std::invalid_argument("HessianAssembly: two instances share a variable group");
     Instance I;
     I.numVars = numVars;
     I.group = group;
+    I.dofs = dofs;
     I.firstElem = elemTmpl_.size();
     inst_.push_back(std::move(I));
     return (int)inst_.size() - 1;
@@ -100,6 +102,19 @@ void HessianAssembly::addConstant(int row, int col, double value) {
     I.cVal.push_back(value);
 }
 
+void HessianAssembly::addNodalConstant(int numNodes, const int* nodes, const double* M) {
+    if (inst_.empty()) throw std::logic_error("HessianAssembly: addNodalConstant before addInstance");
+    Instance& I = inst_.back();
+    const int N = I.numVars / I.dofs;
+    if (numNodes <= 0) throw std::invalid_argument("HessianAssembly: nodal constant without nodes");
+    for (int i = 0; i < numNodes; ++i)
+        if (nodes[i] < 0 || nodes[i] >= N) throw std::invalid_argument("HessianAssembly: nodal constant outside the nodes");
+    I.ncNodes.insert(I.ncNodes.end(), nodes, nodes + numNodes);
+    I.ncStart.push_back(I.ncNodes.size());
+    I.ncVal.insert(I.ncVal.end(), M, M + (size_t)numNodes * numNodes);
+    I.ncValStart.push_back(I.ncVal.size());
+}
+
 namespace {
 
 // Store only base columns; a column equal to a stored one plus a constant reads that one.
@@ -135,9 +150,7 @@ void compressColumns(std::vector<std::vector<uint32_t>>& cols, std::vector<Kerne
 
 }  // namespace
 
-Program HessianAssembly::build(const BuildOptions& opt, ProgramInputs& io, std::vector<SparseMatrix<double>>& patterns) const {
-    Program prog;
-    PhaseTimer timer;
+void HessianAssembly::inputLayout(ProgramInputs& io, std::vector<size_t>& inOff) const {
     io = ProgramInputs();
     for (auto& I : inst_) {
         io.inputSizes.push_back((size_t)I.numVars);
@@ -145,11 +158,112 @@ Program HessianAssembly::build(const BuildOptions& opt, ProgramInputs& io, std::
         io.inputLengths[I.group] = (size_t)I.numVars;
         io.inputDataSize += (size_t)I.numVars;
     }
-    prog.numInputs = io.inputDataSize;
-    if (prog.numInputs > std::numeric_limits<uint32_t>::max() / 2)
+    if (io.inputDataSize > std::numeric_limits<uint32_t>::max() / 2)
         throw std::length_error("HessianAssembly: too many input values");
-    std::vector<size_t> inOff(inst_.size());
+    inOff.resize(inst_.size());
     for (size_t k = 0; k < inst_.size(); ++k) inOff[k] = io.inputOffsets.at(inst_[k].group);
+}
+
+// Pattern of instance k: every element contributes all numDiffVars^2 (row, col) pairs of its
+// variables; constants add (row, col) pairs; duplicates are summed in CSC order (counting sort
+// by column, stable sort by row, as setFromTriplets does).
+void HessianAssembly::instancePattern(size_t k, InstancePattern& out) const {
+    const Instance& I = inst_[k];
+    const int nvar = I.numVars;
+    std::vector<uint32_t> tr, tc;
+    std::vector<uint64_t> src;  // element contribution (e << 16 | slot), or CONST_FLAG | constant index
+    const int D = I.dofs, Nn = I.numVars / I.dofs;
+    size_t est = I.cRow.size() + (size_t)D * I.ncVal.size();
+    for (size_t e = I.firstElem; e < I.firstElem + I.numElems; ++e) {
+        const int nd = getTemplate(elemTmpl_[e]).numDiffVars;
+        est += (size_t)nd * nd;
+    }
+    tr.reserve(est);
+    tc.reserve(est);
+    src.reserve(est);
+    const uint64_t CONST_FLAG = uint64_t(1) << 63;
+    for (size_t e = I.firstElem; e < I.firstElem + I.numElems; ++e) {
+        const ExprTemplate& t = getTemplate(elemTmpl_[e]);
+        const int nd = t.numDiffVars;
+        const Arg* a = &args_[elemArg_[e]];
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) {
+                const int slot = t.entryToOutput[(size_t)i * nd + j];
+                tr.push_back((uint32_t)a[i].id);
+                tc.push_back((uint32_t)a[j].id);
+                src.push_back((uint64_t)e << 16 | (uint64_t)slot);
+            }
+    }
+    for (size_t c = 0; c < I.cRow.size(); ++c) {
+        tr.push_back((uint32_t)I.cRow[c]);
+        tc.push_back((uint32_t)I.cCol[c]);
+        src.push_back(CONST_FLAG | c);
+    }
+    const uint64_t NODAL_FLAG = uint64_t(1) << 62;
+    for (size_t b = 0; b + 1 < I.ncStart.size(); ++b) {
+        const int* nodes = &I.ncNodes[I.ncStart[b]];
+        const size_t nn = I.ncStart[b + 1] - I.ncStart[b];
+        for (int d = 0; d < D; ++d)
+            for (size_t i = 0; i < nn; ++i)
+                for (size_t j = 0; j < nn; ++j) {
+                    tr.push_back((uint32_t)(nodes[i] + d * Nn));
+                    tc.push_back((uint32_t)(nodes[j] + d * Nn));
+                    src.push_back(CONST_FLAG | NODAL_FLAG | (uint64_t)(I.ncValStart[b] + i * nn + j));
+                }
+    }
+    const size_t m = tr.size();
+    std::vector<uint64_t> cstart((size_t)nvar + 1, 0);
+    for (size_t q = 0; q < m; ++q) ++cstart[tc[q] + 1];
+    for (int j = 0; j < nvar; ++j) cstart[j + 1] += cstart[j];
+    std::vector<uint32_t> order(m);
+    {
+        std::vector<uint64_t> fill(cstart.begin(), cstart.end() - 1);
+        for (size_t q = 0; q < m; ++q) order[fill[tc[q]]++] = (uint32_t)q;
+    }
+    out = InstancePattern();
+    std::vector<int> outer((size_t)nvar + 1, 0), inner;
+    inner.reserve(m / 2);
+    out.contrib.reserve(m);
+    for (int j = 0; j < nvar; ++j) {
+        auto b = order.begin() + (long)cstart[j], e = order.begin() + (long)cstart[j + 1];
+        std::stable_sort(b, e, [&](uint32_t x, uint32_t y) { return tr[x] < tr[y]; });
+        outer[j] = (int)inner.size();
+        for (auto it = b; it != e;) {
+            auto it2 = it;
+            uint32_t first = (uint32_t)out.contrib.size(), count = 0;
+            double cval = 0.0;
+            bool hasConst = false;
+            while (it2 != e && tr[*it2] == tr[*it]) {
+                const uint64_t s = src[*it2];
+                if (s & CONST_FLAG) {
+                    if (s & NODAL_FLAG) cval += I.ncVal[(size_t)(s & ~(CONST_FLAG | NODAL_FLAG))];
+                    else cval += I.cVal[(size_t)(s & ~CONST_FLAG)];
+                    hasConst = true;
+                } else {
+                    out.contrib.push_back(s);
+                    ++count;
+                }
+                ++it2;
+            }
+            inner.push_back((int)tr[*it]);
+            out.first.push_back(first);
+            out.count.push_back(count);
+            out.cval.push_back(cval);
+            out.hasConst.push_back(hasConst);
+            it = it2;
+        }
+    }
+    outer[nvar] = (int)inner.size();
+    out.pattern = SparseMatrix<double>(nvar, nvar);
+    out.pattern.setPattern(outer, inner);
+}
+
+Program HessianAssembly::build(const BuildOptions& opt, ProgramInputs& io, std::vector<SparseMatrix<double>>& patterns) const {
+    Program prog;
+    PhaseTimer timer;
+    std::vector<size_t> inOff;
+    inputLayout(io, inOff);
+    prog.numInputs = io.inputDataSize;
 
     // ---- element kernels: one per template, elements in insertion order
     std::map<uint32_t, std::vector<uint32_t>> byTemplate;
@@ -237,119 +351,39 @@ Program HessianAssembly::build(const BuildOptions& opt, ProgramInputs& io, std::
         bool hasConst;
         uint32_t pos;           // result position
     };
-    struct InstanceResult {
-        std::vector<OutRec> outs;  // first: into contrib of this instance; pos: within this instance
-        std::vector<uint32_t> contrib;
-        SparseMatrix<double> pattern;
-    };
-    std::vector<InstanceResult> results(inst_.size());
+    std::vector<InstancePattern> results(inst_.size());
+    parallelFor(inst_.size(), [&](size_t k) { instancePattern(k, results[k]); });
 
-    auto buildInstance = [&](size_t k) {
-        const Instance& I = inst_[k];
-        std::vector<OutRec>& outs = results[k].outs;
-        std::vector<uint32_t>& contrib = results[k].contrib;
-        const size_t resultBase = 0;
-        const int nvar = I.numVars;
-        // (col, row, value-source) triplets: source >= 0 intermediate position, < 0 constant -(idx+1)
-        std::vector<uint32_t> tr, tc;
-        std::vector<int64_t> src;
-        size_t est = I.cRow.size();
-        for (size_t e = I.firstElem; e < I.firstElem + I.numElems; ++e) {
-            const int nd = getTemplate(elemTmpl_[e]).numDiffVars;
-            est += (size_t)nd * nd;
-        }
-        tr.reserve(est);
-        tc.reserve(est);
-        src.reserve(est);
-        for (size_t e = I.firstElem; e < I.firstElem + I.numElems; ++e) {
-            const ExprTemplate& t = getTemplate(elemTmpl_[e]);
-            const ProgramKernel& P = prog.kernels[elemKernel[e]];
-            const int nd = t.numDiffVars;
-            const Arg* a = &args_[elemArg_[e]];
-            for (int i = 0; i < nd; ++i)
-                for (int j = 0; j < nd; ++j) {
-                    const int slot = t.entryToOutput[(size_t)i * nd + j];
-                    tr.push_back((uint32_t)a[i].id);
-                    tc.push_back((uint32_t)a[j].id);
-                    src.push_back((int64_t)(prog.numInputs + P.interBase + (size_t)slot * P.n + elemLocal[e]));
-                }
-        }
-        for (size_t c = 0; c < I.cRow.size(); ++c) {
-            tr.push_back((uint32_t)I.cRow[c]);
-            tc.push_back((uint32_t)I.cCol[c]);
-            src.push_back(-(int64_t)c - 1);
-        }
-        // counting sort by column, then rows within a column
-        const size_t m = tr.size();
-        std::vector<uint64_t> cstart((size_t)nvar + 1, 0);
-        for (size_t q = 0; q < m; ++q) ++cstart[tc[q] + 1];
-        for (int j = 0; j < nvar; ++j) cstart[j + 1] += cstart[j];
-        std::vector<uint32_t> order(m);
-        {
-            std::vector<uint64_t> fill(cstart.begin(), cstart.end() - 1);
-            for (size_t q = 0; q < m; ++q) order[fill[tc[q]]++] = (uint32_t)q;
-        }
-        SparseMatrix<double> pat(nvar, nvar);
-        std::vector<int> outer((size_t)nvar + 1, 0), inner;
-        inner.reserve(m / 2);
-        for (int j = 0; j < nvar; ++j) {
-            auto b = order.begin() + (long)cstart[j], e = order.begin() + (long)cstart[j + 1];
-            std::stable_sort(b, e, [&](uint32_t x, uint32_t y) { return tr[x] < tr[y]; });
-            outer[j] = (int)inner.size();
-            for (auto it = b; it != e;) {
-                auto it2 = it;
-                OutRec rec{(uint32_t)contrib.size(), 0, 0.0, false, (uint32_t)(resultBase + inner.size())};
-                while (it2 != e && tr[*it2] == tr[*it]) {
-                    const int64_t s = src[*it2];
-                    if (s >= 0) {
-                        contrib.push_back((uint32_t)s);
-                        ++rec.count;
-                    } else {
-                        rec.cval += I.cVal[(size_t)(-s - 1)];
-                        rec.hasConst = true;
-                    }
-                    ++it2;
-                }
-                inner.push_back((int)tr[*it]);
-                outs.push_back(rec);
-                it = it2;
-            }
-        }
-        outer[nvar] = (int)inner.size();
-        pat.setPattern(outer, inner);
-        results[k].pattern = std::move(pat);
-    };
-    parallelFor(inst_.size(), buildInstance);
-
-    // merge in instance order
+    // merge in instance order; contributions become intermediate positions
     std::vector<OutRec> outs;
     std::vector<uint32_t> contrib;
     size_t resultBase = 0, totalOuts = 0, totalContrib = 0;
     for (auto& r : results) {
-        totalOuts += r.outs.size();
+        totalOuts += r.first.size();
         totalContrib += r.contrib.size();
     }
     outs.reserve(totalOuts);
     contrib.reserve(totalContrib);
     patterns.clear();
     for (size_t k = 0; k < inst_.size(); ++k) {
-        InstanceResult& r = results[k];
+        InstancePattern& r = results[k];
         const size_t nout = r.pattern.nonZeros();
         if (resultBase + nout > std::numeric_limits<uint32_t>::max())
             throw std::length_error("HessianAssembly: too many outputs");
         const uint32_t cbase = (uint32_t)contrib.size();
-        for (auto rec : r.outs) {
-            rec.first += cbase;
-            rec.pos += (uint32_t)resultBase;
-            outs.push_back(rec);
+        for (size_t q = 0; q < nout; ++q)
+            outs.push_back({r.first[q] + cbase, r.count[q], r.cval[q], (bool)r.hasConst[q], (uint32_t)(resultBase + q)});
+        for (uint64_t s : r.contrib) {
+            const size_t e = (size_t)(s >> 16), slot = (size_t)(s & 0xffff);
+            const ProgramKernel& P = prog.kernels[elemKernel[e]];
+            contrib.push_back((uint32_t)(prog.numInputs + P.interBase + slot * P.n + elemLocal[e]));
         }
-        contrib.insert(contrib.end(), r.contrib.begin(), r.contrib.end());
         prog.outStart.push_back(resultBase);
         prog.outLength.push_back(nout);
         io.outputSizes.push_back(nout);
         resultBase += nout;
         patterns.push_back(std::move(r.pattern));
-        r = InstanceResult();
+        r = InstancePattern();
     }
 
     timer("assembly: patterns");

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "Fused.hpp"
 #include "Program.hpp"
 #include "Sparse.hpp"
 #include "Symbolic.hpp"
@@ -32,7 +33,9 @@ constexpr int ELEMENT_PARAM = 248;
 class HessianAssembly {
 public:
     // Start a new instance whose variables are {0..numVars-1, group}; returns its index.
-    int addInstance(int numVars, int group);
+    // `dofs` > 1 declares the node layout variable = node + dof * (numVars / dofs) used by the
+    // fused form (an n x dofs column-major position matrix).
+    int addInstance(int numVars, int group, int dofs = 1);
     // Element energy of the current instance.
     void addElement(const Symbolic& energy);
 
@@ -44,19 +47,32 @@ public:
     void addElement(int shape, const int* vars, const double* params);
     // Constant Hessian contribution (row, col) of the current instance (not mirrored).
     void addConstant(int row, int col, double value);
+    // Constant block on the nodes of the current instance (dofs > 1): for every dof d,
+    // H(nodes[i] + d n, nodes[j] + d n) += M[i * numNodes + j], n = numVars / dofs.
+    // The isotropic form of quadratic energies such as discrete bending; stored compactly.
+    void addNodalConstant(int numNodes, const int* nodes, const double* M);
 
     size_t numInstances() const { return inst_.size(); }
     size_t numElements() const { return elemTmpl_.size(); }
 
     // Build the program (inputs: one container per instance; outputs: one CSC per instance).
     Program build(const BuildOptions& opt, ProgramInputs& io, std::vector<SparseMatrix<double>>& patterns) const;
+    // Build the fused form (same inputs, outputs and patterns).  Throws std::invalid_argument
+    // when an element template has no common node layout or an output has more than 255
+    // element contributions.
+    FusedProgram buildFused(const BuildOptions& opt, const FusedOptions& fopt, ProgramInputs& io,
+                            std::vector<SparseMatrix<double>>& patterns) const;
 
 private:
     struct Instance {
-        int numVars = 0, group = 0;
+        int numVars = 0, group = 0, dofs = 1;
         size_t firstElem = 0, numElems = 0;
         std::vector<int> cRow, cCol;
         std::vector<double> cVal;
+        std::vector<int> ncNodes;          // nodal constant blocks: nodes
+        std::vector<size_t> ncStart{0};    // block b: nodes [ncStart[b], ncStart[b+1])
+        std::vector<double> ncVal;         // block b: values from ncValStart[b], numNodes^2
+        std::vector<size_t> ncValStart{0};
     };
     struct Arg {
         double value;  // constant value
@@ -70,6 +86,17 @@ private:
         std::vector<int> otherParam;  // template other slot -> parameter (-1: constant below)
         std::vector<double> otherConst;
     };
+    // Output pattern of one instance in CSC order; contributions are (element << 16 | slot).
+    struct InstancePattern {
+        std::vector<uint32_t> first, count;  // per output, into contrib
+        std::vector<double> cval;            // per output, sum of constants
+        std::vector<char> hasConst;
+        std::vector<uint64_t> contrib;
+        SparseMatrix<double> pattern;
+    };
+    void instancePattern(size_t k, InstancePattern& out) const;
+    void inputLayout(ProgramInputs& io, std::vector<size_t>& inOff) const;
+
     std::vector<Shape> shapes_;
     std::vector<Instance> inst_;
     std::vector<uint32_t> elemTmpl_;

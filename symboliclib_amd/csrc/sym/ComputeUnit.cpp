@@ -64,6 +64,7 @@ void ComputeUnitBase::addExpressions(const Symbolic* expr, size_t len, int& id) 
 
 void ComputeUnitBase::init() {
     if (io.outputSizes.empty()) throw std::invalid_argument("ComputeUnit: no output expressions");
+    if (device.fused) throw std::invalid_argument("ComputeUnit: UseFused applies to units built from a HessianAssembly");
     BuildOptions opt;
     opt.decompositionThreshold = (unsigned char)std::min(255u, device.decompositionThreshold);
     opt.simplifyTemplates = device.simplifyTemplates;
@@ -76,6 +77,29 @@ void ComputeUnitBase::init() {
 void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
     BuildOptions opt;
     opt.simplifyTemplates = device.simplifyTemplates;
+    if (device.fused) {
+        FusedOptions fo;
+        fo.ldsBytes = device.fusedLdsBytes;
+        fo.threadsPerBlock = device.threadsPerBlock();
+        if (device.schedule < 0 || device.schedule > 3) throw std::invalid_argument("ComputeUnit: unknown schedule");
+        fo.schedule = (FusedSchedule)device.schedule;
+        if (device.outputGroups < 1 || device.outputGroups > 64) throw std::invalid_argument("ComputeUnit: output groups must be 1..64");
+        if (device.wavesPerEU < 0 || device.wavesPerEU > 8) throw std::invalid_argument("ComputeUnit: waves per EU must be 0..8");
+        fo.outputGroups = device.outputGroups;
+        fo.wavesPerEU = device.wavesPerEU;
+        fusedProgram = assembly.buildFused(opt, fo, io, assemblyPatterns);
+        code = emitHipFused(fusedProgram, fo, launches);
+        fusedForm = true;
+        tiledForm = false;
+        tilingNote = fusedProgram.summary();
+        program = Program();
+        program.numInputs = fusedProgram.numInputs;
+        for (size_t k = 0; k < fusedProgram.outStart.size(); ++k) {
+            program.outStart.push_back(fusedProgram.outStart[k]);
+            program.outLength.push_back(fusedProgram.outLength[k]);
+        }
+        return;
+    }
     program = assembly.build(opt, io, assemblyPatterns);
     finalize();
 }
@@ -139,6 +163,16 @@ void ComputeUnitBase::compileProgram() {
         d.num_values = program.numInputs;
         d.num_index = 0;
     }
+    if (fusedForm) {
+        d.num_values = fusedProgram.numInputs;
+        d.num_index = 0;
+        d.num_outpos = 0;
+        d.num_const = fusedProgram.consts.size();
+        d.const_data = fusedProgram.consts.data();
+        d.num_tiles_data = fusedProgram.pool.size();
+        d.tiles_data = fusedProgram.pool.data();
+    }
+    d.num_work_words = 2;
     d.num_launches = (uint32_t)ls.size();
     d.launches = ls.data();
 
@@ -163,6 +197,11 @@ void ComputeUnitBase::setArgImpl(int i, const double* d, size_t n) {
 void ComputeUnitBase::runImpl(void* stream) {
     if (!rt) throw std::runtime_error("ComputeUnit: run before compile()");
     check(symhip_run(rt, stream), "symhip_run");
+}
+
+void ComputeUnitBase::runGraph(void* stream) {
+    if (!rt) throw std::runtime_error("ComputeUnit: run before compile()");
+    check(symhip_run_graph(rt, stream), "symhip_run_graph");
 }
 
 void ComputeUnitBase::getResultImpl(int k, double* d, size_t n) {

@@ -52,6 +52,15 @@ struct NoSimplify {};
 struct UseWaves {};
 struct UseTiles {};
 using TileLDSBytes = NamedType<int, struct TileLDSBytesTag>;
+// Fused assembly form of a unit built from a HessianAssembly: tiles of owner nodes evaluate
+// their elements into LDS and write their outputs directly (one launch).  Schedule picks the
+// tile order and whether workgroups are persistent (FusedSchedule values).
+struct UseFused {};
+using Schedule = NamedType<int, struct ScheduleTag>;
+// Fused form: work items per element (its outputs split between them) and the minimum
+// waves per SIMD the kernel is compiled for (0 = compiler's choice).
+using OutputGroups = NamedType<int, struct OutputGroupsTag>;
+using WavesPerEU = NamedType<int, struct WavesPerEUTag>;
 
 class Device {
 public:
@@ -64,6 +73,11 @@ public:
     bool simplifyTemplates = true;
     bool tiled = false;
     unsigned int tileLdsBytes = 64 * 1024;
+    bool fused = false;
+    unsigned int fusedLdsBytes = 32 * 1024;
+    int schedule = (int)FusedSchedule::Queue;
+    int outputGroups = 5;
+    int wavesPerEU = 0;
 
 private:
     void set(const DecompositionThreshold& nt) { decompositionThreshold = (unsigned)nt.x; }
@@ -74,7 +88,14 @@ private:
     void set(const NoSimplify&) { simplifyTemplates = false; }
     void set(const UseWaves&) { tiled = false; }
     void set(const UseTiles&) { tiled = true; }
-    void set(const TileLDSBytes& b) { tileLdsBytes = (unsigned)b.x; }
+    void set(const TileLDSBytes& b) {
+        tileLdsBytes = (unsigned)b.x;
+        fusedLdsBytes = (unsigned)b.x;
+    }
+    void set(const UseFused&) { fused = true; }
+    void set(const Schedule& s) { schedule = s.x; }
+    void set(const OutputGroups& g) { outputGroups = g.x; }
+    void set(const WavesPerEU& w) { wavesPerEU = w.x; }
     void set(const UseHIP&) {
         hipDevice = true;
         cudaDevice = false;
@@ -116,6 +137,8 @@ protected:
     std::vector<LaunchDesc> launches;
     std::vector<uint32_t> tilePool;
     bool tiledForm = false;
+    FusedProgram fusedProgram;
+    bool fusedForm = false;
     std::string tilingNote;
     symhip_t rt = nullptr;
     int numOutputContainers = 0;
@@ -145,6 +168,8 @@ public:
     const std::string& getCode() const { return code; }
     const std::vector<LaunchDesc>& getLaunches() const { return launches; }
     bool isTiled() const { return tiledForm; }
+    bool isFused() const { return fusedForm; }
+    const FusedProgram& getFusedProgram() const { return fusedProgram; }
     // output patterns of a unit built from a HessianAssembly (one per instance)
     const std::vector<SparseMatrix<double>>& outputPatterns() const { return assemblyPatterns; }
     const std::string& tilingInfo() const { return tilingNote; }
@@ -159,6 +184,8 @@ public:
     void compileProgram();
     // run on a stream without synchronising (inputs already on the device)
     void run(void* stream = nullptr) { runImpl(stream); }
+    // same, replayed from a HIP graph captured on the first call
+    void runGraph(void* stream = nullptr);
 };
 
 template <class RealT>

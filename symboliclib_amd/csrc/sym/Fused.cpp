@@ -1,0 +1,541 @@
+// Fused assembly form: tiling of owner nodes, tile pool layout and validation.
+#include "Fused.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+
+#include "Assembly.hpp"
+#include "ComputeKernel.hpp"
+#include "Parallel.hpp"
+#include "Template.hpp"
+#include "Timer.hpp"
+
+namespace Sym {
+
+namespace {
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr int HEADER = 10;  // fixed header words before the per-template triples
+
+uint32_t u32(size_t v, const char* what) {
+    if (v > std::numeric_limits<uint32_t>::max()) throw std::length_error(std::string("fused form: ") + what + " exceeds 32 bits");
+    return (uint32_t)v;
+}
+
+// Per-template layout shared by every element of the template.
+struct TemplateLayout {
+    uint32_t tmplId = 0;
+    std::vector<int> argGather;   // template argument -> gathered index, or -1
+    std::vector<int> argParam;    // template argument -> parameter column, or -1
+    std::vector<double> literal;  // uniform constant of a constant argument
+    std::vector<char> uniform;
+    std::vector<int> gatherNode, gatherDof;
+    int numNodes = 0, numParams = 0, numOutputs = 0;
+};
+
+// Tiles of one instance, pool offsets relative to the instance's chunk.
+struct InstanceTiles {
+    std::vector<uint32_t> pool;
+    std::vector<double> consts;
+    std::vector<size_t> header;  // tile header offsets in pool
+    std::vector<double> cost;    // per tile: outputs + element values
+    size_t evaluations = 0, contributions = 0, lds = 0, outputs = 0;
+};
+
+}  // namespace
+
+std::string FusedProgram::summary() const {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "fused tiles=%zu lds=%zu recompute=%.3f templates=%zu contributions=%zu pool=%zu",
+                  numTiles, ldsBytes, recompute(), templates.size(), numContributions, pool.size());
+    return buf;
+}
+
+FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOptions& fo, ProgramInputs& io,
+                                         std::vector<SparseMatrix<double>>& patterns) const {
+    PhaseTimer timer;
+    const int T = fo.threadsPerBlock;
+    if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("fused form: threads per block must be a multiple of 64 up to 1024");
+    FusedProgram fp;
+    std::vector<size_t> inOff;
+    inputLayout(io, inOff);
+    fp.numInputs = io.inputDataSize;
+    fp.numElements = elemTmpl_.size();
+
+    // ---- templates in first-use order, argument kinds, node layouts
+    std::map<uint32_t, int> tIndex;
+    std::vector<TemplateLayout> L;
+    std::vector<int> elemT(elemTmpl_.size());
+    std::vector<uint32_t> elemInst(elemTmpl_.size());
+    for (size_t k = 0; k < inst_.size(); ++k)
+        for (size_t e = inst_[k].firstElem; e < inst_[k].firstElem + inst_[k].numElems; ++e) elemInst[e] = (uint32_t)k;
+    std::vector<int> nodeOf;  // scratch: local node index -> node id of the current element
+    for (size_t e = 0; e < elemTmpl_.size(); ++e) {
+        auto it = tIndex.find(elemTmpl_[e]);
+        const ExprTemplate& t = getTemplate(elemTmpl_[e]);
+        const Instance& I = inst_[elemInst[e]];
+        const int N = I.numVars / I.dofs;
+        const Arg* a = &args_[elemArg_[e]];
+        if (it == tIndex.end()) {
+            it = tIndex.emplace(elemTmpl_[e], (int)L.size()).first;
+            TemplateLayout tl;
+            tl.tmplId = elemTmpl_[e];
+            tl.numOutputs = t.numOutputs;
+            std::vector<int> nodes;
+            for (int q = 0; q < t.numArgs; ++q) {
+                if (a[q].kind == 1) {
+                    const int node = a[q].id % N, dof = a[q].id / N;
+                    int j = (int)(std::find(nodes.begin(), nodes.end(), node) - nodes.begin());
+                    if (j == (int)nodes.size()) nodes.push_back(node);
+                    tl.argGather.push_back((int)tl.gatherNode.size());
+                    tl.argParam.push_back(-1);
+                    tl.gatherNode.push_back(j);
+                    tl.gatherDof.push_back(dof);
+                    tl.literal.push_back(0.0);
+                    tl.uniform.push_back(0);
+                } else {
+                    tl.argGather.push_back(-1);
+                    tl.argParam.push_back(-1);
+                    tl.literal.push_back(a[q].value);
+                    tl.uniform.push_back(1);
+                }
+            }
+            tl.numNodes = (int)nodes.size();
+            L.push_back(std::move(tl));
+        }
+        TemplateLayout& tl = L[it->second];
+        elemT[e] = it->second;
+        nodeOf.assign((size_t)tl.numNodes, -1);
+        for (int q = 0; q < t.numArgs; ++q) {
+            const bool isVar = a[q].kind == 1;
+            if (isVar != (tl.argGather[q] >= 0))
+                throw std::invalid_argument("fused form: argument kinds differ within an element template");
+            if (isVar) {
+                const int g = tl.argGather[q];
+                const int node = a[q].id % N, dof = a[q].id / N;
+                int& slot = nodeOf[(size_t)tl.gatherNode[g]];
+                if (slot < 0) slot = node;
+                if (slot != node || dof != tl.gatherDof[g])
+                    throw std::invalid_argument("fused form: elements of one template have different node layouts");
+            } else if (tl.uniform[q] && a[q].value != tl.literal[q]) {
+                tl.uniform[q] = 0;
+            }
+        }
+    }
+    for (auto& tl : L) {
+        const ExprTemplate& t = getTemplate(tl.tmplId);
+        std::vector<Symbolic> argRef((size_t)t.numArgs);
+        for (int q = 0; q < t.numArgs; ++q) {
+            if (tl.argGather[q] >= 0) argRef[q] = Symbolic(tl.argGather[q], -3);
+            else if (tl.uniform[q]) argRef[q] = Symbolic(tl.literal[q]);
+            else {
+                tl.argParam[q] = tl.numParams++;
+                argRef[q] = Symbolic(tl.argParam[q], -4);
+            }
+        }
+        std::vector<Symbolic> assigns;
+        for (int s = 0; s < t.numOutputs; ++s)
+            assigns.emplace_back(ASSIGN, Symbolic(s, -2), substituteArguments(t.outputs[s], argRef));
+        FusedTemplate ft;
+        ft.code = optimizeTemplate(Symbolic(BLOCK, assigns), opt.simplifyTemplates);
+        ft.numNodes = tl.numNodes;
+        ft.argNode = tl.gatherNode;
+        ft.argDof = tl.gatherDof;
+        ft.numParams = tl.numParams;
+        ft.numOutputs = tl.numOutputs;
+        if ((size_t)tl.numOutputs * 8 > fo.ldsBytes)
+            throw std::invalid_argument("fused form: one element needs more LDS than a tile holds");
+        fp.templates.push_back(std::move(ft));
+    }
+    const size_t NT = L.size();
+    timer("fused: templates");
+
+    // ---- per instance: pattern and tiles
+    std::vector<InstanceTiles> tiles(inst_.size());
+    std::vector<SparseMatrix<double>> pats(inst_.size());
+    const size_t ldsValues = fo.ldsBytes / 8;
+    parallelFor(inst_.size(), [&](size_t k) {
+        const Instance& I = inst_[k];
+        InstancePattern P;
+        instancePattern(k, P);
+        const int D = I.dofs, N = I.numVars / I.dofs;
+        const std::vector<int>& colptr = P.pattern.outer();
+        const size_t E = I.numElems, e0 = I.firstElem;
+
+        // element nodes (by local node index) and node -> incident elements
+        std::vector<uint32_t> enStart(E + 1, 0), en;
+        for (size_t i = 0; i < E; ++i) enStart[i + 1] = enStart[i] + (uint32_t)L[elemT[e0 + i]].numNodes;
+        en.assign(enStart[E], 0);
+        for (size_t i = 0; i < E; ++i) {
+            const TemplateLayout& tl = L[elemT[e0 + i]];
+            const Arg* a = &args_[elemArg_[e0 + i]];
+            for (size_t q = 0; q < tl.argGather.size(); ++q)
+                if (tl.argGather[q] >= 0) en[enStart[i] + tl.gatherNode[tl.argGather[q]]] = (uint32_t)(a[q].id % N);
+        }
+        std::vector<uint32_t> incStart((size_t)N + 1, 0), inc;
+        for (size_t i = 0; i < E; ++i)
+            for (uint32_t j = enStart[i]; j < enStart[i + 1]; ++j) {
+                bool dup = false;
+                for (uint32_t j2 = enStart[i]; j2 < j; ++j2) dup = dup || en[j2] == en[j];
+                if (!dup) ++incStart[en[j] + 1];
+            }
+        for (int v = 0; v < N; ++v) incStart[v + 1] += incStart[v];
+        inc.assign(incStart[N], 0);
+        {
+            std::vector<uint32_t> fill(incStart.begin(), incStart.end() - 1);
+            for (size_t i = 0; i < E; ++i)
+                for (uint32_t j = enStart[i]; j < enStart[i + 1]; ++j) {
+                    bool dup = false;
+                    for (uint32_t j2 = enStart[i]; j2 < j; ++j2) dup = dup || en[j2] == en[j];
+                    if (!dup) inc[fill[en[j]]++] = (uint32_t)i;
+                }
+        }
+        std::vector<uint32_t> outCount((size_t)N, 0);
+        for (int b = 0; b < N; ++b)
+            for (int d = 0; d < D; ++d) outCount[b] += (uint32_t)(colptr[b + d * N + 1] - colptr[b + d * N]);
+
+        std::vector<char> assigned((size_t)N, 0);
+        for (int b = 0; b < N; ++b) assigned[b] = outCount[b] == 0;
+        std::vector<int64_t> stamp(E, -1);
+        std::vector<uint32_t> local(E, 0);
+        std::vector<std::vector<uint32_t>> buckets;
+        std::vector<size_t> heads;
+        InstanceTiles& out = tiles[k];
+        int64_t tileId = 0;
+        size_t cursor = 0;
+        std::vector<uint32_t> tNodes;
+        std::vector<std::vector<uint32_t>> tElems(NT);
+
+        auto cost = [&](uint32_t v, int64_t id) {
+            uint32_t c = 0;
+            for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q) c += stamp[inc[q]] != id;
+            return c;
+        };
+        auto push = [&](uint32_t v, uint32_t c) {
+            if (buckets.size() <= c) {
+                buckets.resize(c + 1);
+                heads.resize(c + 1, 0);
+            }
+            buckets[c].push_back(v);
+        };
+
+        for (;;) {
+            while (cursor < (size_t)N && assigned[cursor]) ++cursor;
+            if (cursor == (size_t)N) break;
+            for (auto& b : buckets) b.clear();
+            std::fill(heads.begin(), heads.end(), 0);
+            tNodes.clear();
+            for (auto& te : tElems) te.clear();
+            size_t slots = 0, outs = 0;
+            push((uint32_t)cursor, cost((uint32_t)cursor, tileId));
+            for (;;) {
+                // pop the cheapest frontier node (FIFO within a cost)
+                uint32_t v = 0;
+                bool found = false;
+                uint32_t popped = 0;
+                for (size_t c = 0; c < buckets.size() && !found; ++c)
+                    while (heads[c] < buckets[c].size()) {
+                        v = buckets[c][heads[c]++];
+                        if (assigned[v]) continue;
+                        popped = (uint32_t)c;
+                        found = true;
+                        break;
+                    }
+                if (!found) break;
+                const uint32_t cur = cost(v, tileId);
+                if (cur != popped) {
+                    push(v, cur);
+                    continue;
+                }
+                size_t newSlots = 0;
+                for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q)
+                    if (stamp[inc[q]] != tileId) newSlots += (size_t)L[elemT[e0 + inc[q]]].numOutputs;
+                if (!tNodes.empty() && (slots + newSlots > ldsValues || outs + outCount[v] > fo.maxOutputs)) break;
+                if (slots + newSlots > ldsValues)
+                    throw std::invalid_argument("fused form: the elements of one node need more LDS than a tile holds");
+                assigned[v] = 1;
+                tNodes.push_back(v);
+                outs += outCount[v];
+                slots += newSlots;
+                for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q) {
+                    const uint32_t i = inc[q];
+                    if (stamp[i] == tileId) continue;
+                    stamp[i] = tileId;
+                    tElems[(size_t)elemT[e0 + i]].push_back(i);
+                    for (uint32_t j = enStart[i]; j < enStart[i + 1]; ++j)
+                        if (!assigned[en[j]]) push(en[j], cost(en[j], tileId));
+                }
+            }
+
+            // ---- emit the tile
+            std::sort(tNodes.begin(), tNodes.end());
+            std::vector<size_t> ldsBase(NT + 1, 0);
+            for (size_t t = 0; t < NT; ++t) {
+                std::sort(tElems[t].begin(), tElems[t].end());
+                for (size_t q = 0; q < tElems[t].size(); ++q) local[tElems[t][q]] = (uint32_t)q;
+                ldsBase[t + 1] = ldsBase[t] + tElems[t].size() * (size_t)L[t].numOutputs;
+            }
+            if (ldsBase[NT] > 65536) throw std::logic_error("fused form: tile LDS index exceeds 16 bits");
+            std::vector<uint32_t> runs;  // (position, first output)
+            std::vector<uint8_t> counts;
+            std::vector<uint16_t> slotList;
+            std::vector<double> cvals;
+            bool anyConst = false;
+            size_t m = 0;
+            uint32_t prevEnd = NONE;
+            for (int d = 0; d < D; ++d)
+                for (uint32_t b : tNodes) {
+                    const int c = (int)b + d * N;
+                    for (int q = colptr[c]; q < colptr[c + 1]; ++q, ++m) {
+                        if ((uint32_t)q != prevEnd) {
+                            runs.push_back((uint32_t)q);
+                            runs.push_back((uint32_t)m);
+                        }
+                        prevEnd = (uint32_t)q + 1;
+                        const uint32_t cnt = P.count[q];
+                        if (cnt > 127) throw std::invalid_argument("fused form: an output has more than 127 element contributions");
+                        counts.push_back((uint8_t)(cnt | (P.hasConst[q] ? 0x80u : 0u)));
+                        for (uint32_t r = 0; r < cnt; ++r) {
+                            const uint64_t s = P.contrib[P.first[q] + r];
+                            const size_t i = (size_t)(s >> 16) - e0;
+                            const size_t slot = (size_t)(s & 0xffff);
+                            if (stamp[i] != tileId) throw std::logic_error("fused form: contribution from an element outside the tile");
+                            const size_t t = (size_t)elemT[e0 + i];
+                            slotList.push_back((uint16_t)(ldsBase[t] + slot * tElems[t].size() + local[i]));
+                        }
+                        if (P.hasConst[q]) {
+                            cvals.push_back(P.cval[q]);
+                            anyConst = true;
+                        }
+                    }
+                }
+            runs.push_back(0);
+            runs.push_back((uint32_t)m);
+            const size_t nch = (m + 63) / 64;
+            std::vector<uint32_t> chunks(3 * nch);
+            {
+                size_t sb = 0, cb = 0, r = 0;
+                for (size_t ch = 0; ch < nch; ++ch) {
+                    const size_t o0 = ch * 64;
+                    while (runs[2 * (r + 1) + 1] <= o0) ++r;
+                    chunks[3 * ch] = (uint32_t)sb;
+                    chunks[3 * ch + 1] = (uint32_t)r;
+                    chunks[3 * ch + 2] = (uint32_t)cb;
+                    for (size_t o = o0; o < std::min(m, o0 + 64); ++o) {
+                        sb += counts[o] & 0x7fu;
+                        cb += counts[o] >> 7;
+                    }
+                }
+            }
+
+            std::vector<uint32_t>& pool = out.pool;
+            const size_t H = pool.size();
+            out.header.push_back(H);
+            pool.resize(H + HEADER + 3 * NT, 0);
+            auto appendWords = [&](const void* data, size_t bytes) {
+                const size_t off = pool.size() - H;
+                const size_t words = (bytes + 3) / 4;
+                pool.resize(pool.size() + words, 0);
+                if (bytes) std::memcpy(&pool[H + off], data, bytes);
+                return u32(off, "tile size");
+            };
+            const uint32_t runsOff = appendWords(runs.data(), runs.size() * 4);
+            const uint32_t countsOff = appendWords(counts.data(), counts.size());
+            const uint32_t slotsOff = appendWords(slotList.data(), slotList.size() * 2);
+            const uint32_t chunksOff = appendWords(chunks.data(), chunks.size() * 4);
+            pool[H + 0] = u32(inOff[k], "input base");
+            pool[H + 1] = (uint32_t)N;
+            pool[H + 2] = u32(m, "outputs per tile");
+            pool[H + 3] = (uint32_t)nch;
+            pool[H + 4] = runsOff;
+            pool[H + 5] = countsOff;
+            pool[H + 6] = slotsOff;
+            pool[H + 7] = chunksOff;
+            if (anyConst) {
+                pool[H + 8] = u32(out.consts.size(), "constants");
+                out.consts.insert(out.consts.end(), cvals.begin(), cvals.end());
+            } else {
+                pool[H + 8] = NONE;
+            }
+            pool[H + 9] = 0;  // result base, set at merge
+            size_t evals = 0;
+            for (size_t t = 0; t < NT; ++t) {
+                const auto& te = tElems[t];
+                const TemplateLayout& tl = L[t];
+                const size_t cnt = te.size();
+                evals += cnt;
+                std::vector<uint32_t> nodes((size_t)tl.numNodes * cnt);
+                for (size_t q = 0; q < cnt; ++q)
+                    for (int j = 0; j < tl.numNodes; ++j) nodes[(size_t)j * cnt + q] = en[enStart[te[q]] + (uint32_t)j];
+                const uint32_t nodesOff = appendWords(nodes.data(), nodes.size() * 4);
+                pool[H + HEADER + 3 * t] = (uint32_t)cnt;
+                pool[H + HEADER + 3 * t + 1] = nodesOff;
+                pool[H + HEADER + 3 * t + 2] = u32(out.consts.size(), "constants");
+                const size_t pb = out.consts.size();
+                out.consts.resize(pb + (size_t)tl.numParams * cnt);
+                for (size_t q = 0; q < cnt; ++q) {
+                    const Arg* a = &args_[elemArg_[e0 + te[q]]];
+                    for (size_t g = 0; g < tl.argParam.size(); ++g)
+                        if (tl.argParam[g] >= 0) out.consts[pb + (size_t)tl.argParam[g] * cnt + q] = a[g].value;
+                }
+            }
+            out.cost.push_back((double)m + (double)ldsBase[NT]);
+            out.evaluations += evals;
+            out.contributions += slotList.size();
+            out.lds = std::max(out.lds, ldsBase[NT] * 8);
+            out.outputs += m;
+            ++tileId;
+        }
+        pats[k] = std::move(P.pattern);
+    });
+    timer("fused: patterns and tiles");
+
+    // ---- merge instances (offsets first, then copies in parallel); order the tile list
+    const size_t K = inst_.size();
+    std::vector<size_t> poolBase(K), constBase(K), resBase(K), tileBase(K);
+    size_t nTiles = 0, pb = 0, cb = 0, rb = 0;
+    for (size_t k = 0; k < K; ++k) {
+        tileBase[k] = nTiles;
+        nTiles += tiles[k].header.size();
+    }
+    pb = 1 + nTiles;
+    for (size_t k = 0; k < K; ++k) {
+        if (tiles[k].outputs != pats[k].nonZeros())
+            throw std::logic_error("fused form: tiles do not cover every output of an instance");
+        poolBase[k] = pb;
+        constBase[k] = cb;
+        resBase[k] = rb;
+        pb += tiles[k].pool.size();
+        cb += tiles[k].consts.size();
+        rb += pats[k].nonZeros();
+        fp.elementEvaluations += tiles[k].evaluations;
+        fp.numContributions += tiles[k].contributions;
+        fp.ldsBytes = std::max(fp.ldsBytes, tiles[k].lds);
+    }
+    u32(pb, "tile pool");
+    u32(cb, "constants");
+    u32(rb, "results");
+    fp.pool.assign(pb, 0);
+    fp.consts.assign(cb, 0.0);
+    fp.pool[0] = (uint32_t)nTiles;
+    struct TileRef {
+        uint32_t header;
+        uint32_t inst, index;
+        double cost;
+    };
+    std::vector<TileRef> refs(nTiles);
+    parallelFor(K, [&](size_t k) {
+        InstanceTiles& it = tiles[k];
+        const uint32_t cbase = (uint32_t)constBase[k];
+        for (size_t q = 0; q < it.header.size(); ++q) {
+            uint32_t* H = &it.pool[it.header[q]];
+            if (H[8] != NONE) H[8] += cbase;
+            H[9] = (uint32_t)resBase[k];
+            for (size_t t = 0; t < NT; ++t) H[HEADER + 3 * t + 2] += cbase;
+            refs[tileBase[k] + q] = {(uint32_t)(poolBase[k] + it.header[q]), (uint32_t)k, (uint32_t)q, it.cost[q]};
+        }
+        std::copy(it.pool.begin(), it.pool.end(), fp.pool.begin() + (long)poolBase[k]);
+        std::copy(it.consts.begin(), it.consts.end(), fp.consts.begin() + (long)constBase[k]);
+        it = InstanceTiles();
+    });
+    patterns.clear();
+    for (size_t k = 0; k < K; ++k) {
+        const size_t nout = pats[k].nonZeros();
+        fp.outStart.push_back(resBase[k]);
+        fp.outLength.push_back(nout);
+        io.outputSizes.push_back(nout);
+        patterns.push_back(std::move(pats[k]));
+    }
+    fp.numResults = rb;
+    switch (fo.schedule) {
+        case FusedSchedule::QueueLargest:
+            std::stable_sort(refs.begin(), refs.end(), [](const TileRef& a, const TileRef& b) { return a.cost > b.cost; });
+            break;
+        case FusedSchedule::Interleaved:
+            std::stable_sort(refs.begin(), refs.end(), [](const TileRef& a, const TileRef& b) {
+                return a.index != b.index ? a.index < b.index : a.inst < b.inst;
+            });
+            break;
+        default:
+            break;
+    }
+    for (size_t t = 0; t < nTiles; ++t) fp.pool[1 + t] = refs[t].header;
+    fp.numTiles = nTiles;
+    if (fp.ldsBytes == 0) fp.ldsBytes = 8;
+    timer("fused: merge");
+    validateFused(fp);
+    timer("fused: validate");
+    return fp;
+}
+
+void validateFused(const FusedProgram& fp) {
+    const size_t NT = fp.templates.size();
+    const auto& P = fp.pool;
+    auto need = [](bool ok, const char* what) {
+        if (!ok) throw std::logic_error(std::string("fused form: invalid tile pool: ") + what);
+    };
+    need(!P.empty() && P[0] == fp.numTiles && P.size() >= 1 + fp.numTiles, "tile table");
+    size_t totalContrib = 0;
+    for (size_t t = 0; t < fp.numTiles; ++t) {
+        const size_t H = P[1 + t];
+        need(H + HEADER + 3 * NT <= P.size(), "header outside the pool");
+        const uint32_t B = P[H], N = P[H + 1], m = P[H + 2], nch = P[H + 3];
+        need(nch == (m + 63) / 64, "chunk count");
+        const size_t runs = H + P[H + 4], counts = H + P[H + 5], slots = H + P[H + 6], chunks = H + P[H + 7];
+        const uint32_t cOff = P[H + 8], rbase = P[H + 9];
+        need(chunks + 3 * (size_t)nch <= P.size() && counts + ((size_t)m + 3) / 4 <= P.size(), "tables outside the pool");
+        // LDS of the tile and element data
+        size_t lds = 0;
+        for (size_t q = 0; q < NT; ++q) {
+            const FusedTemplate& ft = fp.templates[q];
+            const size_t cnt = P[H + HEADER + 3 * q], nodes = H + P[H + HEADER + 3 * q + 1], prm = P[H + HEADER + 3 * q + 2];
+            need(nodes + (size_t)ft.numNodes * cnt <= P.size(), "node list outside the pool");
+            need(prm + (size_t)ft.numParams * cnt <= fp.consts.size(), "parameters outside the constants");
+            for (size_t i = 0; i < (size_t)ft.numNodes * cnt; ++i) need(P[nodes + i] < N, "node outside its instance");
+            for (size_t g = 0; g < ft.argDof.size(); ++g)
+                need((size_t)B + (size_t)ft.argDof[g] * N + N <= fp.numInputs, "gather outside the inputs");
+            lds += cnt * (size_t)ft.numOutputs;
+        }
+        need(lds * 8 <= fp.ldsBytes && lds <= 65536, "tile LDS exceeds the launch's LDS");
+        const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[counts]);
+        size_t S = 0, numConst = 0;
+        for (size_t o = 0; o < m; ++o) {
+            S += cnts[o] & 0x7fu;
+            numConst += cnts[o] >> 7;
+        }
+        if (numConst) need(cOff != 0xffffffffu && (size_t)cOff + numConst <= fp.consts.size(), "output constants outside the constants");
+        need(slots + (S + 1) / 2 <= P.size(), "slot list outside the pool");
+        const uint16_t* sl = reinterpret_cast<const uint16_t*>(&P[slots]);
+        for (size_t s = 0; s < S; ++s) need(sl[s] < lds, "contribution outside the tile's LDS");
+        totalContrib += S;
+        // runs: first outputs increasing from 0 to the sentinel m; positions inside the results
+        size_t r = 0;
+        need(m == 0 || P[runs + 1] == 0, "first run");
+        while (P[runs + 2 * r + 1] != m) {
+            need(runs + 2 * r + 3 < P.size(), "run list outside the pool");
+            const size_t first = P[runs + 2 * r + 1], next = P[runs + 2 * r + 3];
+            need(next > first && next <= m, "runs not increasing");
+            need((size_t)rbase + P[runs + 2 * r] + (next - first) <= fp.numResults, "run outside the results");
+            ++r;
+        }
+        const size_t numRuns = r;
+        size_t sb = 0, cb = 0;
+        for (size_t ch = 0; ch < nch; ++ch) {
+            need(P[chunks + 3 * ch] == sb && P[chunks + 3 * ch + 2] == cb, "chunk slot or constant base");
+            const size_t rr = P[chunks + 3 * ch + 1];
+            need(rr < numRuns && P[runs + 2 * rr + 1] <= ch * 64 && P[runs + 2 * rr + 3] > ch * 64, "chunk run");
+            for (size_t o = ch * 64; o < std::min<size_t>(m, ch * 64 + 64); ++o) {
+                sb += cnts[o] & 0x7fu;
+                cb += cnts[o] >> 7;
+            }
+        }
+    }
+    need(totalContrib == fp.numContributions, "contribution count");
+}
+
+}  // namespace Sym

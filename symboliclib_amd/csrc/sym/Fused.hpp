@@ -1,0 +1,86 @@
+// Fused assembly form of a HessianAssembly: one work tile per compact patch of owner nodes.
+//
+// The waves form of an element-sum Hessian writes every element's Hessian block to HBM and
+// reads it back in a second launch through a per-contribution gather index; on a cloth that
+// is five times the algorithmic traffic.  The fused form removes both: a tile owns a set of
+// nodes (every output column of those nodes), evaluates every element touching them into LDS,
+// and sums each owned output from LDS into its final CSC position.  Elements on the boundary
+// of two tiles are evaluated by both (the recompute factor is reported).
+//
+// Node layout: instance variables are numNodes x dofs, column-major (variable = node + dof *
+// numNodes), the layout of an n x 3 position matrix; an element's variables are dofs of its
+// nodes, so a tile gathers them as x[base + node + dof * numNodes] from a per-element node
+// list.
+//
+// Tile pool (u32 words, read by the generated kernel and checked by validateFused):
+//   pool[0]              number of tiles
+//   pool[1 + t]          offset of tile t's header, tiles listed in launch order
+//   header H (offsets marked rel are relative to H):
+//     H[0] input base of the instance     H[1] numNodes of the instance
+//     H[2] m, owned outputs               H[3] 64-output chunks
+//     H[4] rel runs: (result position, first output) pairs, one sentinel (first = m)
+//     H[5] rel counts: u8 per output, element contributions (bits 0-6) and whether the
+//          output adds a constant (bit 7)
+//     H[6] rel slots: u16 LDS indices, contributions of every output in order
+//     H[7] rel chunks: (first slot, run, first constant) per 64-output chunk
+//     H[8] output constants in c[], one per output with bit 7 set, in output order, or
+//          0xffffffff
+//     H[9 + 3 t], H[10 + 3 t], H[11 + 3 t]: template t's element count, rel node list
+//       (numNodes_t columns of count), parameter block in c[] (numParams_t columns of count)
+//   LDS of a tile: template t's outputs at L_t + slot * count_t + i, L_t = sum over t' < t
+//   of count_t' * numOutputs_t'.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "Program.hpp"
+#include "Symbolic.hpp"
+
+namespace Sym {
+
+enum class FusedSchedule : int {
+    Static = 0,       // one workgroup per tile, tiles in build order
+    Queue = 1,        // persistent workgroups pull tiles from an atomic counter, build order
+    QueueLargest = 2, // persistent, largest tiles first
+    Interleaved = 3   // persistent, tiles of different instances interleaved round-robin
+};
+
+struct FusedOptions {
+    size_t ldsBytes = 32 * 1024;      // element values per tile
+    size_t maxOutputs = 1u << 16;     // owned outputs per tile
+    int threadsPerBlock = 256;
+    FusedSchedule schedule = FusedSchedule::Queue;
+    int outputGroups = 5;             // work items per element (outputs split between them)
+    int wavesPerEU = 0;               // minimum waves per SIMD asked of the compiler (0 = none)
+};
+
+struct FusedTemplate {
+    Symbolic code;                 // ASSIGN(VAR(s,-2), ...) over VAR(a,-3) arguments, VAR(q,-4) parameters
+    int numNodes = 0;              // nodes per element
+    std::vector<int> argNode, argDof;  // per gathered argument
+    int numParams = 0;             // parameter columns (non-uniform constants)
+    int numOutputs = 0;            // LDS values per element
+};
+
+struct FusedProgram {
+    std::vector<FusedTemplate> templates;
+    std::vector<uint32_t> pool;
+    std::vector<double> consts;
+    size_t numInputs = 0, numResults = 0;
+    std::vector<size_t> outStart, outLength;
+    size_t numTiles = 0, ldsBytes = 0;
+    size_t elementEvaluations = 0, numElements = 0, numContributions = 0;
+    double recompute() const { return numElements ? (double)elementEvaluations / (double)numElements : 0.0; }
+    std::string summary() const;
+};
+
+// Walks the pool as the kernel does and checks every index against its buffer; throws
+// std::logic_error on the first violation.
+void validateFused(const FusedProgram& fp);
+
+// Generated source: one __global__ symlib_fused and one device function per template.
+std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches);
+
+}  // namespace Sym

@@ -13,9 +13,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
+#include <limits>
+#include <map>
+#include <set>
 #include <sstream>
 #include <stdexcept>
 
+#include "Fused.hpp"
 #include "Program.hpp"
 #include "Tiling.hpp"
 #include "Traverse.hpp"
@@ -47,6 +52,23 @@ struct KernelEmitter {
     // tiled form: per-tile table D (operands | ids | outpos), cnt instances, LDS base L
     const TiledProgram* tiled = nullptr;
     size_t kernelIndex = 0;
+    // fused form: element template evaluated into LDS (L[slot * cnt + i])
+    const FusedTemplate* fusedT = nullptr;
+
+    std::string fusedLeaf(const std::array<int, 2>& v) const {
+        switch (v[1]) {
+            case -2:
+                return "L[" + std::to_string(v[0]) + "u * cnt + i]";
+            case -3:
+                return "a" + std::to_string(v[0]);
+            case -4:
+                return "q" + std::to_string(v[0]);
+            case LOCAL_VAR:
+                return "r" + std::to_string(v[0]);
+            default:
+                throw std::logic_error("emitHipFused: unexpected variable group " + std::to_string(v[1]));
+        }
+    }
 
     std::string tiledLeaf(const std::array<int, 2>& v) const {
         const int slot = v[0];
@@ -79,6 +101,7 @@ struct KernelEmitter {
     }
 
     std::string leaf(const std::array<int, 2>& v) const {
+        if (fusedT) return fusedLeaf(v);
         if (tiled) return tiledLeaf(v);
         const int slot = v[0];
         switch (v[1]) {
@@ -188,6 +211,109 @@ struct KernelEmitter {
     }
 };
 
+// Straight-line template code as a statement list, for scheduling.  Leaves are keyed
+// LOCAL k -> k, argument a -> ARG + a, parameter q -> PARAM + q.
+struct Stmt {
+    Symbolic s;              // DEFINE or ASSIGN
+    int def = -1;            // LOCAL defined, or -1
+    int out = -1;            // output slot assigned, or -1
+    std::vector<int> uses;   // distinct leaf keys
+};
+constexpr int ARG = 1 << 24, PARAM = 1 << 25;
+
+std::vector<Stmt> flatten(const Symbolic& block) {
+    std::vector<Stmt> out;
+    std::vector<const Symbolic*> st{&block};
+    std::vector<Symbolic> seq;
+    // BLOCKs nest only at the top; keep statement order
+    std::function<void(const Symbolic&)> walk = [&](const Symbolic& b) {
+        if (b.op() == BLOCK) {
+            for (auto& c : b) walk(c);
+            return;
+        }
+        seq.push_back(b);
+    };
+    walk(block);
+    for (auto& b : seq) {
+        Stmt s;
+        s.s = b;
+        if (b.op() == DEFINE) s.def = b[0].variable()[0];
+        else if (b.op() == ASSIGN) s.out = b[0].variable()[0];
+        else throw std::logic_error("emitHipFused: unexpected statement " + OpInfos[b.op()].code);
+        std::vector<int> u;
+        preOrderTraverse(b[1], [&](const Symbolic& y) {
+            if (y.op() == VAR) {
+                const auto v = y.variable();
+                if (v[1] == LOCAL_VAR) u.push_back(v[0]);
+                else if (v[1] == -3) u.push_back(ARG + v[0]);
+                else if (v[1] == -4) u.push_back(PARAM + v[0]);
+            }
+            return true;
+        });
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        s.uses = std::move(u);
+        out.push_back(std::move(s));
+    }
+    return out;
+}
+
+// Greedy list schedule of the statements in `keep` (a dependency-closed subset) that keeps
+// few values live: among ready statements take the one that frees the most values and
+// creates the fewest (outputs create none); ties keep the original order.
+std::vector<size_t> lowPressureOrder(const std::vector<Stmt>& S, const std::vector<char>& keep) {
+    const size_t n = S.size();
+    std::map<int, size_t> defAt;
+    for (size_t i = 0; i < n; ++i)
+        if (keep[i] && S[i].def >= 0) defAt[S[i].def] = i;
+    std::map<int, int> remaining;
+    std::vector<int> indeg(n, 0);
+    std::vector<std::vector<size_t>> users(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (!keep[i]) continue;
+        for (int u : S[i].uses) {
+            ++remaining[u];
+            auto it = defAt.find(u);
+            if (u < ARG && it != defAt.end()) {
+                ++indeg[i];
+                users[it->second].push_back(i);
+            }
+        }
+    }
+    std::vector<size_t> ready, order;
+    for (size_t i = 0; i < n; ++i)
+        if (keep[i] && indeg[i] == 0) ready.push_back(i);
+    std::set<int> live;
+    while (!ready.empty()) {
+        size_t bestPos = 0;
+        long bestScore = std::numeric_limits<long>::min();
+        for (size_t r = 0; r < ready.size(); ++r) {
+            const Stmt& s = S[ready[r]];
+            long score = 0;
+            for (int u : s.uses) {
+                if (remaining[u] == 1 && (u >= ARG ? true : live.count(u) > 0)) ++score;
+                if (u >= ARG && !live.count(u) && remaining[u] > 1) --score;
+            }
+            if (s.def >= 0) --score;
+            if (score > bestScore || (score == bestScore && ready[r] < ready[bestPos])) {
+                bestScore = score;
+                bestPos = r;
+            }
+        }
+        const size_t i = ready[bestPos];
+        ready.erase(ready.begin() + (long)bestPos);
+        order.push_back(i);
+        for (int u : S[i].uses) {
+            if (--remaining[u] == 0) live.erase(u);
+            else if (u >= ARG) live.insert(u);
+        }
+        if (S[i].def >= 0) live.insert(S[i].def);
+        for (size_t j : users[i])
+            if (--indeg[j] == 0) ready.push_back(j);
+    }
+    return order;
+}
+
 }  // namespace
 
 std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<LaunchDesc>& launches) {
@@ -206,7 +332,7 @@ std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<Lau
         os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name
            << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
            << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
-           << "    const INDEX* __restrict__ tiles) {\n";
+           << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
         os << "    const unsigned b = blockIdx.x;\n";
         size_t blocks = 0;
         for (const auto& K : prog.kernels) {
@@ -240,7 +366,7 @@ std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const Emit
     os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") symlib_tiled"
        << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
        << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
-       << "    const INDEX* __restrict__ tiles) {\n";
+       << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
     os << "    extern __shared__ double lds[];\n";
     os << "    const INDEX* __restrict__ H = tiles + tiles[blockIdx.x];\n";
     for (int w = 0; w < prog.numWaves; ++w) {
@@ -274,6 +400,167 @@ std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const Emit
     os << "}\n";
     launches.clear();
     launches.push_back({"symlib_tiled", (uint32_t)tp.numTiles, (uint32_t)T, (uint32_t)tp.ldsBytes});
+    return os.str();
+}
+
+// Fused assembly form (tile pool layout in Fused.hpp).  Phase A: the workgroup's threads
+// evaluate the tile's elements, one element per thread, into LDS.  Phase B: each wave takes
+// 64 consecutive owned outputs, finds its lanes' contribution lists by a wave prefix sum of
+// their counts, sums the LDS values and writes the result to its CSC position.  Persistent
+// schedules loop over tiles taken from an atomic counter in `work`; the last workgroup to
+// finish clears the counter, so consecutive launches (and graph replays) start from zero.
+std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches) {
+    const int T = opt.threadsPerBlock;
+    if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("emitHipFused: threads per block must be a multiple of 64 up to 1024");
+    const bool persistent = opt.schedule != FusedSchedule::Static;
+    static const Program noProgram;
+    static const ProgramKernel noKernel;
+    std::ostringstream os;
+    os << "// generated by symboliclib_amd: fused assembly form (LDS-staged element tiles), FP64\n";
+    os << "typedef unsigned int INDEX;\n\n";
+    // per template: outputs split into groups evaluated by separate work items (fewer live
+    // values per work item), statements of each group in a low-pressure order
+    std::vector<int> groupsOf(fp.templates.size());
+    for (size_t t = 0; t < fp.templates.size(); ++t) {
+        const FusedTemplate& ft = fp.templates[t];
+        const std::vector<Stmt> S = flatten(ft.code);
+        std::vector<size_t> outs;
+        for (size_t i = 0; i < S.size(); ++i)
+            if (S[i].out >= 0) outs.push_back(i);
+        std::sort(outs.begin(), outs.end(), [&](size_t a, size_t b) { return S[a].out < S[b].out; });
+        const int G = std::max(1, std::min<int>(opt.outputGroups, (int)outs.size()));
+        groupsOf[t] = G;
+        std::map<int, size_t> defAt;
+        for (size_t i = 0; i < S.size(); ++i)
+            if (S[i].def >= 0) defAt[S[i].def] = i;
+        for (int g = 0; g < G; ++g) {
+            const size_t o0 = outs.size() * (size_t)g / (size_t)G, o1 = outs.size() * (size_t)(g + 1) / (size_t)G;
+            std::vector<char> keep(S.size(), 0);
+            std::vector<size_t> st(outs.begin() + (long)o0, outs.begin() + (long)o1);
+            while (!st.empty()) {
+                const size_t i = st.back();
+                st.pop_back();
+                if (keep[i]) continue;
+                keep[i] = 1;
+                for (int u : S[i].uses)
+                    if (u < ARG) {
+                        auto it = defAt.find(u);
+                        if (it != defAt.end()) st.push_back(it->second);
+                    }
+            }
+            std::set<int> leaves;
+            for (size_t i = 0; i < S.size(); ++i)
+                if (keep[i])
+                    for (int u : S[i].uses)
+                        if (u >= ARG) leaves.insert(u);
+            os << "__device__ __forceinline__ void symlib_elem" << t << "_" << g
+               << "(const double* __restrict__ x, unsigned B, unsigned N, const INDEX* __restrict__ nd,\n"
+               << "    const double* __restrict__ pr, unsigned cnt, unsigned i, double* __restrict__ L) {\n";
+            std::set<int> nodes;
+            for (int u : leaves)
+                if (u < PARAM) nodes.insert(ft.argNode[(size_t)(u - ARG)]);
+            for (int j : nodes) os << "    const unsigned n" << j << " = nd[" << j << "u * cnt + i];\n";
+            for (int u : leaves) {
+                if (u < PARAM) {
+                    const size_t a = (size_t)(u - ARG);
+                    os << "    const double a" << a << " = x[B + n" << ft.argNode[a];
+                    if (ft.argDof[a]) os << " + " << ft.argDof[a] << "u * N";
+                    os << "];\n";
+                } else {
+                    const int q = u - PARAM;
+                    os << "    const double q" << q << " = pr[" << q << "u * cnt + i];\n";
+                }
+            }
+            KernelEmitter ke{noProgram, noKernel, false};
+            ke.fusedT = &ft;
+            for (size_t i : lowPressureOrder(S, keep)) ke.statements(S[i].s, os, "    ");
+            os << "}\n\n";
+        }
+    }
+    os << "__device__ __forceinline__ unsigned symlib_excl_scan64(unsigned v, unsigned lane) {\n"
+       << "    unsigned s = v;\n"
+       << "    for (unsigned d = 1; d < 64u; d <<= 1) {\n"
+       << "        const unsigned u = __shfl_up(s, d, 64);\n"
+       << "        if (lane >= d) s += u;\n"
+       << "    }\n"
+       << "    return s - v;\n}\n\n";
+    os << "extern \"C\" __global__ void __launch_bounds__(" << T;
+    if (opt.wavesPerEU > 0) os << ", " << opt.wavesPerEU;
+    os << ") symlib_fused"
+       << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+       << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
+       << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
+    os << "    extern __shared__ double lds[];\n";
+    if (persistent) os << "    __shared__ unsigned s_next;\n";
+    os << "    const unsigned numTiles = tiles[0];\n";
+    os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
+    os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
+    os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
+    os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3];\n";
+    os << "        {\n            unsigned Lb = 0;\n";
+    for (size_t t = 0; t < fp.templates.size(); ++t) {
+        const size_t h = 10 + 3 * t;
+        os << "            {\n"
+           << "                const unsigned cnt = H[" << h << "];\n"
+           << "                const INDEX* __restrict__ nd = H + H[" << h + 1 << "];\n"
+           << "                const double* __restrict__ pr = c + H[" << h + 2 << "];\n"
+           << "                for (unsigned w = threadIdx.x; w < " << groupsOf[t] << "u * cnt; w += " << T << "u) {\n"
+           << "                    const unsigned g = w / cnt, i = w - g * cnt;\n";
+        for (int g = 0; g < groupsOf[t]; ++g)
+            os << "                    " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
+               << "symlib_elem" << t << "_" << g << "(x, B, N, nd, pr, cnt, i, lds + Lb);\n";
+        os << "                }\n"
+           << "                Lb += cnt * " << fp.templates[t].numOutputs << "u;\n"
+           << "            }\n";
+    }
+    os << "        }\n";
+    os << "        __syncthreads();\n";
+    os << "        {\n"
+       << "            const INDEX* __restrict__ R = H + H[4];\n"
+       << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(H + H[5]);\n"
+       << "            const unsigned short* __restrict__ S = (const unsigned short*)(H + H[6]);\n"
+       << "            const INDEX* __restrict__ CH = H + H[7];\n"
+       << "            const unsigned cOff = H[8];\n"
+       << "            double* __restrict__ res = out + H[9];\n"
+       << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
+       << "                const unsigned oi = ch * 64u + lane;\n"
+       << "                const bool act = oi < m;\n"
+       << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
+       << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
+       << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
+       << "                const unsigned base = CH[3u * ch] + (ex & 0xffffu);\n"
+       << "                double s = 0.0;\n"
+       << "                for (unsigned q = 0; q < k; ++q) s += lds[S[base + q]];\n"
+       << "                if (act) {\n"
+       << "                    if (hc) s += c[cOff + CH[3u * ch + 2u] + (ex >> 16)];\n"
+       << "                    unsigned r = CH[3u * ch + 1u];\n"
+       << "                    while (R[2u * r + 3u] <= oi) ++r;\n"
+       << "                    res[R[2u * r] + (oi - R[2u * r + 1u])] = s;\n"
+       << "                }\n"
+       << "            }\n"
+       << "        }\n";
+    if (persistent) {
+        os << "        __syncthreads();\n"
+           << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work, 1u);\n"
+           << "        __syncthreads();\n"
+           << "        it = s_next;\n";
+    } else {
+        os << "        break;\n";
+    }
+    os << "    }\n";
+    if (persistent)
+        os << "    if (threadIdx.x == 0) {\n"
+           << "        __threadfence();\n"
+           << "        if (atomicAdd(work + 1, 1u) == gridDim.x - 1u) {\n"
+           << "            atomicExch(work, 0u);\n"
+           << "            atomicExch(work + 1, 0u);\n"
+           << "        }\n"
+           << "    }\n";
+    os << "}\n";
+    launches.clear();
+    const uint32_t grid = persistent ? LaunchDesc::GRID_FILL : (uint32_t)fp.numTiles;
+    if (!persistent && fp.numTiles > 0xffffffffull) throw std::length_error("emitHipFused: grid too large");
+    launches.push_back({"symlib_fused", grid, (uint32_t)T, (uint32_t)fp.ldsBytes});
     return os.str();
 }
 

@@ -69,6 +69,8 @@ struct BuildOptions {
 Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& io, const BuildOptions& opt);
 
 struct LaunchDesc {
+    // grid value asking the runtime for as many workgroups as the device runs at once
+    static constexpr uint32_t GRID_FILL = 0xffffffffu;
     std::string name;
     uint32_t grid = 0, block = 0;
     uint32_t shared = 0;  // dynamic LDS bytes

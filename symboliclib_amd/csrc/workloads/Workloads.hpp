@@ -291,7 +291,7 @@ public:
 inline void baraffWitkinAssembly(HessianAssembly& A, FaceShapes& shapes, const DenseMatrix<double>& rest,
                                  const Faces& F, double kBend, int group) {
     const int n = rest.rows();
-    A.addInstance(3 * n, group);
+    A.addInstance(3 * n, group, 3);
     const RestShape r = precomputeRestShape(rest, F);
     double params[5];
     int vars[9];
@@ -301,11 +301,13 @@ inline void baraffWitkinAssembly(HessianAssembly& A, FaceShapes& shapes, const D
             for (int d = 0; d < 3; ++d) vars[3 * i + d] = F[f][i] + d * n;
         A.addElement(s, vars, params);
     }
-    if (kBend != 0.0) {
-        std::vector<Triplet<double>> bend;
-        appendBendTriplets(rest, F, kBend, n, bend);
-        for (auto& t : bend) A.addConstant(t.row(), t.col(), t.value());
-    }
+    if (kBend != 0.0)
+        for (const auto& q : interiorEdgeQuads(F)) {
+            // k_bend Q(e) on every coordinate block, as appendBendTriplets
+            auto Q = bendStiffness(rest, q);
+            for (double& v : Q) v *= kBend;
+            A.addNodalConstant(4, q.data(), Q.data());
+        }
 }
 
 }  // namespace workloads

@@ -16,8 +16,11 @@ from . import _native
 from .workloads import to_colmajor
 
 
+SCHEDULES = {"static": 0, "queue": 1, "largest": 2, "interleaved": 3}
+
+
 def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True, tiled=False,
-          tile_lds_bytes=0, direct_assembly=False):
+          tile_lds_bytes=0, direct_assembly=False, fused=False, schedule="queue", output_groups=0, waves_per_eu=0):
     o = _native.Options()
     _native.lib().symlib_default_options(ctypes.byref(o))
     o.threads_per_block = int(threads_per_block)
@@ -27,6 +30,10 @@ def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, shar
     o.tiled = int(bool(tiled))
     o.tile_lds_bytes = int(tile_lds_bytes)
     o.direct_assembly = int(bool(direct_assembly))
+    o.fused = int(bool(fused))
+    o.schedule = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
+    o.output_groups = int(output_groups)
+    o.waves_per_eu = int(waves_per_eu)
     return o
 
 
@@ -105,6 +112,11 @@ class Unit:
         _native.check(_native.lib().symlib_unit_run(self._h, stream), "symlib_unit_run")
         return self
 
+    def run_graph(self, stream=None):
+        """run() replayed from a HIP graph captured on the first call."""
+        _native.check(_native.lib().symlib_unit_run_graph(self._h, stream), "symlib_unit_run_graph")
+        return self
+
     def get_results(self, k=0):
         n = self.output_shape(k)[2]
         out = np.empty(n, np.float64)
@@ -142,6 +154,10 @@ class Unit:
     def num_launches(self):
         rt = self.runtime()
         return _native.lib().symhip_num_launches(rt) if rt else 0
+
+    def launch_grid(self, idx=0):
+        rt = self.runtime()
+        return _native.lib().symhip_launch_grid(rt, idx) if rt else 0
 
     def launch_one(self, idx, stream=None):
         _native.check(_native.lib().symhip_launch_one(self.runtime(), idx, stream), "symhip_launch_one")

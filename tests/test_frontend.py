@@ -162,3 +162,56 @@ def test_tiling_with_tiny_budget_falls_back():
     with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, tiled=True, tile_lds_bytes=256) as u:
         info = u.info()
         assert info["tiled"] == 0 and info["tiling"].startswith("untiled")
+
+
+def _fields(info):
+    return dict(kv.split("=") for kv in info["tiling"].split()[1:])
+
+
+@pytest.mark.parametrize("side,lds", [(3, 0), (16, 0), (40, 16384), (40, 65536)])
+def test_fused_pattern_and_tiles(side, lds):
+    rest, F, X = W.c3_cloth(side=side)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, tile_lds_bytes=lds) as u:
+        info = u.info()
+        assert info["tiled"] == 2 and info["tiling"].startswith("fused"), info["tiling"]
+        outer, inner = u.output_pattern()
+        cp, ri = O.bw_pattern(len(rest), F)
+        assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
+        f = _fields(info)
+        assert int(f["tiles"]) >= 1
+        assert int(f["lds"]) <= (lds or 32768)
+        # every face evaluated at least once; compact patches keep re-evaluation bounded
+        assert 1.0 <= float(f["recompute"]) < 2.5
+        assert info["num_results"] == len(ri)
+
+
+def test_fused_batch_patterns():
+    sides = [3, 11, 6]
+    cloths = [W.cloth(s, seed=s) for s in sides]
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], 1e3, 1e2, 1e-2, fused=True, schedule="interleaved") as u:
+        for k, (rest, F, _) in enumerate(cloths):
+            outer, inner = u.output_pattern(k)
+            cp, ri = O.bw_pattern(len(rest), F)
+            assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
+
+
+@pytest.mark.parametrize("groups,wpe,schedule", [(1, 0, "static"), (9, 5, "queue")])
+def test_fused_source_compiles_for_gfx950(tmp_path, groups, wpe, schedule):
+    L = _native.lib()
+    rest, F, X = W.c3_cloth(side=6)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, output_groups=groups,
+                         waves_per_eu=wpe, schedule=schedule) as u:
+        src = u.source()
+        assert "symlib_fused" in src and "symlib_elem0_0" in src
+        assert ("atomicAdd" in src) == (schedule != "static")
+        out = str(tmp_path / "fused.co")
+        assert L.symhip_compile_to_file(src.encode(), None, b"gfx950", out.encode()) == 0, L.symhip_last_error()
+
+
+def test_fused_form_errors():
+    n, cp, ri, a = W.c1_matrix(subdiv=0)
+    with pytest.raises(_native.SymlibError, match="HessianAssembly"):
+        Unit.ata_plus_a(cp, ri, fused=True)
+    rest, F, X = W.c3_cloth(side=4)
+    with pytest.raises(_native.SymlibError, match="more LDS"):
+        Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=256)

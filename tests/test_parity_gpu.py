@@ -9,6 +9,7 @@ absolute value of every factor and of every additive term (products A(k,i) A(k,j
 per-face Hessian entry from the edge vectors on, plus |k_bend Q| per bending constant).  The
 maximum and median errors are printed for every case.
 """
+import ctypes
 import os
 
 import numpy as np
@@ -142,3 +143,63 @@ def test_bw_hessian_batch(require_device, tiled):
         for k, (rest, F, X) in enumerate(cloths):
             ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
             _check(f"batch cloth {k} ({sides[k]}x{sides[k]}) tiled={tiled}", u, ref, u.get_results(k), k)
+
+
+FUSED_CASES = [
+    # (side, schedule, lds bytes, output groups, waves per EU)
+    (4, "queue", 0, 0, 0),
+    (16, "static", 0, 0, 0),
+    (16, "queue", 8192, 1, 0),
+    (33, "largest", 16384, 9, 5),
+    (64, "interleaved", 49152, 3, 4),
+    (256, "queue", 0, 9, 5),
+]
+
+
+@pytest.mark.parametrize("side,schedule,lds,groups,wpe", FUSED_CASES)
+def test_bw_hessian_fused(require_device, side, schedule, lds, groups, wpe):
+    rest, F, X = W.c3_cloth(side=side)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, schedule=schedule, tile_lds_bytes=lds,
+                         output_groups=groups, waves_per_eu=wpe) as u:
+        assert u.info()["tiled"] == 2, u.info()["tiling"]
+        u.compile().execute(W.to_colmajor(X))
+        ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+        _check(f"Baraff-Witkin {side}x{side} fused {schedule} lds={lds} groups={groups} wpe={wpe}", u, ref,
+               u.get_results())
+        # a second evaluation of the same launch (persistent work-queue counters start from zero again)
+        X2 = W.cloth(side, seed=7)[2]
+        u.execute(W.to_colmajor(X2))
+        _check(f"Baraff-Witkin {side}x{side} fused second positions", u,
+               O.bw_hessian(rest, F, X2, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results())
+
+
+@pytest.mark.parametrize("schedule", ["static", "queue", "largest", "interleaved"])
+def test_bw_hessian_batch_fused(require_device, schedule):
+    sides = [5, 17, 32, 9, 40, 3]
+    cloths = [W.cloth(s, seed=200 + s) for s in sides]
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True,
+                               schedule=schedule) as u:
+        u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
+        for k, (rest, F, X) in enumerate(cloths):
+            ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+            _check(f"fused batch {schedule} cloth {k} ({sides[k]}x{sides[k]})", u, ref, u.get_results(k), k)
+
+
+def test_fused_graph_replays_match_eager(require_device):
+    import torch
+
+    sides = [12, 30, 7]
+    cloths = [W.cloth(s, seed=300 + s) for s in sides]
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True) as u:
+        u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
+        eager = [u.get_results(k) for k in range(len(sides))]
+        stream = torch.cuda.current_stream()
+        for rep in range(4):
+            u.run_graph(ctypes.c_void_p(stream.cuda_stream))
+            torch.cuda.synchronize()
+            for k in range(len(sides)):
+                assert np.array_equal(u.get_results(k), eager[k]), f"graph replay {rep} differs on cloth {k}"
+            u.run()  # eager launch between replays
+            torch.cuda.synchronize()
+            for k in range(len(sides)):
+                assert np.array_equal(u.get_results(k), eager[k]), f"eager launch after replay {rep} differs"
