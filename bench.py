@@ -1,15 +1,23 @@
 """Benchmark: FP64 sparse-expression evaluation throughput (output nonzeros per second).
 
-One step = one evaluation of the compiled program (all launches) over one value set whose
-inputs are already resident in HBM.  Default workload: the Baraff-Witkin cloth Hessian
-(stretch + shear by symbolic differentiation, plus bending) of a 256 x 256 sheet, 4,890,648
-output nonzeros, through the plain single-instance HIP path.
+Default workload (c4): 256 Baraff-Witkin cloth Hessians (stretch + shear by symbolic
+differentiation, plus bending) of log-uniform sides 32..256, 345.8 M output nonzeros, batched
+into one program in the fused assembly form (owner-node tiles, element Hessians staged in LDS,
+one persistent work-queue launch replayed from a HIP graph) with the tuned tile shape and
+schedule, against the plain single-instance form launched once per instance (one waves-form
+program per cloth, two launches each) on the same stream.
 
-Prints one JSON line with the throughput, the roofline figure of the evaluation (algorithmic
-bytes / kernel time against the 8 TB/s HBM peak), and the CPU baseline: the FP64 oracle
-(OpenMP over the host cores) timed on a bounded sample of the same workload.
+Single-instance workloads: c3 (one 256 x 256 cloth, 4,890,648 nonzeros), c2 (cotangent
+Laplacian of a 512 x 512 grid), c1 (A^T A + A of an icosphere); --form selects waves, tiled
+or fused (cloth only).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1] [--no-cpu]
+One step = one evaluation of the compiled program (every launch) over one value set whose
+inputs are already resident in HBM.  Prints one JSON line with the throughput, the roofline
+of the dominant kernel (algorithmic bytes per launch / its average launch time, against the
+8 TB/s HBM peak) and the CPU baseline: the FP64 oracle (OpenMP over the host cores) timed on
+a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c3|c2|c1] [--no-cpu]
 """
 import argparse
 import ctypes
@@ -17,6 +25,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -24,123 +33,126 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TUNED = os.path.join(ROOT, "symboliclib_amd", "tuning", "c4_selected.json")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_workload(name, **opt):
+def b_alg(counts, n_out):
+    """Algorithmic bytes of one evaluation: 8 B per input and per-element constant, 8 B per
+    output value plus its 4 B placement index, 4 B per connectivity index."""
+    return 8 * counts["n_in"] + 8 * counts["n_const"] + 12 * n_out + 4 * counts["n_conn"]
+
+
+def fused_options(args, tuned=None):
+    o = {"fused": True}
+    if tuned:
+        o.update(tile_lds_bytes=tuned["lds"], output_groups=tuned["groups"], waves_per_eu=tuned["wpe"],
+                 threads_per_block=tuned["tpb"], schedule=tuned["schedule"])
+    for k, v in (("tile_lds_bytes", args.tile_lds), ("output_groups", args.groups), ("waves_per_eu", args.wpe),
+                 ("threads_per_block", args.tpb), ("schedule", args.schedule)):
+        if v is not None:
+            o[k] = v
+    return o
+
+
+def time_events(fn, stream, reps):
+    import torch
+
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    for _ in range(reps):
+        fn()
+    e.record(stream)
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def copy_bandwidth(dev, stream):
+    import torch
+
+    cbuf = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+    dbuf = torch.empty_like(cbuf)
+    dbuf.copy_(cbuf)
+    ms = time_events(lambda: dbuf.copy_(cbuf), stream, 10)
+    return 2 * cbuf.numel() * 8 / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(evals, seconds, O):
+    """evals: list of (callable, nnz); runs them round-robin for about `seconds`."""
+    evals[0][0]()  # warm
+    n, done = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        f, nnz = evals[n % len(evals)]
+        f()
+        done += nnz
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    tc = time.perf_counter() - t0
+    return done / tc, n, tc
+
+
+# ---------------------------------------------------------------------------------------------
+# single-instance workloads
+
+
+def build_single(name, opt):
     from symboliclib_amd import workloads as W
     from symboliclib_amd.unit import Unit
+
+    import oracle as O
 
     if name == "c1":
         n, cp, ri, a = W.c1_matrix()
         unit = Unit.ata_plus_a(cp, ri, **opt)
-        inputs = a
         counts = {"n_in": len(ri), "n_const": 0, "n_conn": len(ri) + n + 1}
         desc = {"workload": "A^T A + A, icosphere subdivision 5", "rows": n, "nnz_A": int(len(ri))}
-
-        def oracle_fn():
-            import oracle as O
-
-            return O.ata_plus_a(cp, ri, a)
-
-        def cpu_eval():
-            import oracle as O
-
-            O.ata_plus_a(cp, ri, a)
-
-    elif name == "c2":
+        return unit, [a], counts, desc, lambda: O.ata_plus_a(cp, ri, a), lambda: O.ata_plus_a(cp, ri, a)
+    if name == "c2":
         V, F = W.c2_mesh()
         unit = Unit.cotan(len(V), F, **opt)
-        inputs = W.to_colmajor(V)
         counts = {"n_in": 3 * len(V), "n_const": 0, "n_conn": 3 * len(F)}
         desc = {"workload": "cotangent Laplacian, 512x512 jittered grid", "vertices": len(V), "faces": len(F)}
-
-        def oracle_fn():
-            import oracle as O
-
-            return O.cotan(V, F)
-
-        plan = {}
-
-        def cpu_eval():
-            import oracle as O
-
-            if "p" not in plan:
-                plan["p"] = O.CotanPlan(len(V), F)
-                plan["nnz"] = O.cotan_pattern(len(V), F)[0][-1]
-            plan["p"].values(V, plan["nnz"])
-
-    else:
-        rest, F, X = W.c3_cloth()
-        unit = Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, **opt)
-        inputs = W.to_colmajor(X)
-        import oracle as O
-
-        ne = len(O.interior_edges(F))
-        counts = {"n_in": 3 * len(rest), "n_const": 5 * len(F) + 10 * ne, "n_conn": 3 * len(F) + 4 * ne}
-        desc = {"workload": "Baraff-Witkin cloth Hessian (stretch+shear AD + bend), 256x256 sheet",
-                "vertices": len(rest), "faces": len(F), "interior_edges": ne}
-
-        def oracle_fn():
-            return O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
-
-        plan = {}
-
-        def cpu_eval():
-            if "p" not in plan:
-                plan["p"] = O.BWPlan(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
-            plan["p"].values(X, with_scale=False)
-
-    return unit, inputs, counts, desc, oracle_fn, cpu_eval
+        plan = O.CotanPlan(len(V), F)
+        nnz = O.cotan_pattern(len(V), F)[0][-1]
+        return unit, [W.to_colmajor(V)], counts, desc, lambda: O.cotan(V, F), lambda: plan.values(V, nnz)
+    rest, F, X = W.c3_cloth()
+    unit = Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, **opt)
+    counts = W.bw_algorithmic_counts(len(rest), F)
+    desc = {"workload": "Baraff-Witkin cloth Hessian (stretch+shear AD + bend), 256x256 sheet",
+            "vertices": len(rest), "faces": len(F), "interior_edges": counts["interior_edges"]}
+    plan = O.BWPlan(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+    return (unit, [W.to_colmajor(X)], counts, desc, lambda: O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND),
+            lambda: plan.values(X, with_scale=False))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--form", default="waves", choices=["tiled", "waves"],
-                    help="waves: one launch per wave, intermediates in HBM (default); tiled: LDS-staged work tiles")
-    ap.add_argument("--tile-lds", type=int, default=0, help="LDS bytes per tile (0 = default)")
-    ap.add_argument("--tpb", type=int, default=256, help="threads per block")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
-              "the path shards without communication, see DESIGN.md", file=sys.stderr)
-        sys.exit(2)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-
+def run_single(args, dev, stream, sp):
     import torch
 
-    from symboliclib_amd.unit import device_available
+    import oracle as O
 
-    if not torch.cuda.is_available() or not device_available():
-        raise RuntimeError("bench.py needs a HIP device")
-    dev = torch.device("cuda:0")
-
-    log(f"[bench] building workload {args.workload}")
+    if args.form == "fused":
+        if args.workload != "c3":
+            raise SystemExit("bench.py: the fused form applies to the cloth Hessian (c3, c4)")
+        opt = fused_options(args)
+    else:
+        opt = {"tiled": args.form == "tiled", "tile_lds_bytes": args.tile_lds or 0,
+               "threads_per_block": args.tpb or 256}
+    log(f"[bench] building workload {args.workload} ({args.form})")
     t0 = time.perf_counter()
-    unit, inputs, counts, desc, oracle_fn, cpu_eval = build_workload(
-        args.workload, tiled=args.form == "tiled", tile_lds_bytes=args.tile_lds, threads_per_block=args.tpb)
+    unit, inputs, counts, desc, oracle_fn, cpu_eval = build_single(args.workload, opt)
     t1 = time.perf_counter()
     unit.compile()
     t2 = time.perf_counter()
     info = unit.info()
-    log(f"[bench] front-end+build {t1 - t0:.1f}s, compile {t2 - t1:.1f}s, info {info}")
-
-    unit.execute(inputs)  # H2D of the inputs + one evaluation
+    log(f"[bench] front-end+build {t1 - t0:.1f}s, compile {t2 - t1:.1f}s, {info['tiling']}")
+    unit.execute(*inputs)
     torch.cuda.synchronize()
     vals = unit.get_results()
-
-    import oracle as O
-
     cp, ri, rv, sc = oracle_fn()
     outer, inner = unit.output_pattern()
     pattern_ok = bool(np.array_equal(outer, cp) and np.array_equal(inner, ri))
@@ -151,122 +163,254 @@ def main():
         raise RuntimeError("bench.py: result does not match the oracle")
 
     n_out = int(info["num_results"])
-    b_alg = 8 * counts["n_in"] + 8 * counts["n_const"] + 12 * n_out + 4 * counts["n_conn"]
-
-    stream = torch.cuda.current_stream(dev)
-    sp = ctypes.c_void_p(stream.cuda_stream)
+    step = (lambda: unit.run_graph(sp)) if args.graph else (lambda: unit.run(sp))
     for _ in range(args.warmup):
-        unit.run(sp)
+        step()
     torch.cuda.synchronize()
-    log(f"[bench] warmup done ({args.warmup} steps)")
-
-    # timed region: K evaluations back to back
     torch.cuda.synchronize()
     ts = time.perf_counter()
     for _ in range(args.steps):
-        unit.run(sp)
+        step()
     torch.cuda.synchronize()
-    te = time.perf_counter()
-    ms_per_step = (te - ts) * 1e3 / args.steps
+    ms_per_step = (time.perf_counter() - ts) * 1e3 / args.steps
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step")
 
-    # kernel time per launch (HIP events on the launch stream)
     nl = unit.num_launches()
     reps = max(10, min(args.steps, 100))
-    launch_ms = []
-    for li in range(nl):
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record(stream)
-        for _ in range(reps):
-            unit.launch_one(li, sp)
-        e.record(stream)
-        torch.cuda.synchronize()
-        launch_ms.append(s.elapsed_time(e) / reps)
-    kernel_ms = sum(launch_ms)
-
-    # cold evaluations: a 1 GiB write evicts caches before each timed evaluation
+    launch_ms = [time_events(lambda li=li: unit.launch_one(li, sp), stream, reps) for li in range(nl)]
+    # cold evaluations: a 1 GiB write evicts the caches before each one
     flush = torch.empty(1 << 27, dtype=torch.float64, device=dev)
     cold = []
     for _ in range(20):
         flush.zero_()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record(stream)
-        unit.run(sp)
-        e.record(stream)
-        torch.cuda.synchronize()
-        cold.append(s.elapsed_time(e))
-    cold_ms = float(np.median(cold))
+        cold.append(time_events(step, stream, 1))
     del flush
+    cold_ms = float(np.median(cold))
+    copy_gbs = copy_bandwidth(dev, stream)
 
-    # device copy bandwidth for reference
-    cbuf = torch.empty(1 << 27, dtype=torch.float64, device=dev)
-    dbuf = torch.empty_like(cbuf)
-    dbuf.copy_(cbuf)
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record(stream)
-    for _ in range(10):
-        dbuf.copy_(cbuf)
-    e.record(stream)
-    torch.cuda.synchronize()
-    copy_gbs = 2 * cbuf.numel() * 8 * 10 / (s.elapsed_time(e) * 1e-3) / 1e9
-    del cbuf, dbuf
-
-    value = n_out / (ms_per_step * 1e-3)
-    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    dominant = int(np.argmax(launch_ms))
+    names = {"fused": "symlib_fused", "tiled": "symlib_tiled"}
+    kernel = names.get(args.form, f"symlib_wave{dominant}")
+    balg = b_alg(counts, n_out)
+    # the roofline covers the dominant kernel; with one launch that is the whole evaluation
+    achieved = balg / (sum(launch_ms) * 1e-3) / 1e9
     line = {
-        "metric": "output nnz/s FP64 (plain single-instance HIP)",
-        "value": value,
+        "metric": "output nnz/s FP64 (single instance)",
+        "value": n_out / (ms_per_step * 1e-3),
         "unit": "nnz/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": dict(desc, output_nnz=n_out, form=f"plain single-instance, {args.form}", tiling=info["tiling"],
-                       threads_per_block=args.tpb, launches=nl, parallelism="single GPU"),
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "symlib_wave* (all launches of one evaluation)",
-            "algorithmic_bytes": b_alg,
-            "kernel_ms": kernel_ms,
-            "launch_ms": launch_ms,
-        },
-        "cold_ms_per_step": cold_ms,
-        "cold_nnz_per_s": n_out / (cold_ms * 1e-3),
-        "copy_bandwidth_gbs": copy_gbs,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(desc, output_nnz=n_out, form=args.form, tiling=info["tiling"], options=opt, launches=nl,
+                       graph=bool(args.graph), parallelism="single GPU"),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                     "algorithmic_bytes": balg, "kernel_ms": sum(launch_ms), "launch_ms": launch_ms,
+                     "note": "achieved = algorithmic bytes of the evaluation / summed launch times"},
+        "cold_ms_per_step": cold_ms, "cold_nnz_per_s": n_out / (cold_ms * 1e-3), "copy_bandwidth_gbs": copy_gbs,
         "parity": {"pattern_bit_exact": pattern_ok, "max_rel_err": float(err.max()),
                    "median_rel_err": float(np.median(err))},
         "build_seconds": {"frontend": info["frontend_seconds"], "build": info["build_seconds"],
                           "compile": info["compile_seconds"]},
     }
-
     if not args.no_cpu:
-        cpu_eval()  # warm (builds the plan)
-        n = 0
-        tc0 = time.perf_counter()
-        while True:
-            cpu_eval()
-            n += 1
-            if time.perf_counter() - tc0 >= args.cpu_seconds:
-                break
-        tc = time.perf_counter() - tc0
-        line["cpu_baseline"] = {
-            "value": n_out * n / tc,
-            "unit": "nnz/s",
-            "cores": O.num_threads(),
-            "kind": "port",
-            "sample": f"{n} full evaluations of the same workload by the FP64 oracle (OpenMP, "
-                      f"{O.num_threads()} threads) in {tc:.1f} s",
-        }
+        v, n, tc = cpu_baseline([(cpu_eval, n_out)], args.cpu_seconds, O)
+        line["cpu_baseline"] = {"value": v, "unit": "nnz/s", "cores": O.num_threads(), "kind": "port",
+                                "sample": f"{n} full evaluations of the same workload by the FP64 oracle "
+                                          f"(OpenMP, {O.num_threads()} threads) in {tc:.1f} s"}
+    return line
+
+
+# ---------------------------------------------------------------------------------------------
+# batched cloths (c4)
+
+
+def run_batched(args, dev, stream, sp):
+    import torch
+
+    from symboliclib_amd import _native
+    from symboliclib_amd import workloads as W
+    from symboliclib_amd.unit import Unit
+
+    import oracle as O
+
+    tuned = None
+    if os.path.exists(TUNED) and not args.untuned:
+        with open(TUNED) as f:
+            tuned = json.load(f)["selected"]
+    opt = fused_options(args, tuned)
+    sides = W.c4_sides(count=args.count)
+    log(f"[bench] c4: {len(sides)} cloths, sides {min(sides)}..{max(sides)}, fused options {opt}")
+    t0 = time.perf_counter()
+    cloths = W.c4_cloths(count=args.count)
+    counts = [W.bw_algorithmic_counts(len(r), F) for r, F, _ in cloths]
+    batched = Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, **opt)
+    t1 = time.perf_counter()
+    batched.compile()
+    t2 = time.perf_counter()
+    info = batched.info()
+    log(f"[bench] batched build {t1 - t0:.1f}s compile {t2 - t1:.1f}s, {info['tiling']}")
+    inputs = [W.to_colmajor(X) for _, _, X in cloths]
+    batched.execute(*inputs)
+    torch.cuda.synchronize()
+
+    # parity on a spread of instances (smallest, largest, four more)
+    order = np.argsort(sides, kind="stable")
+    check = sorted({int(order[0]), int(order[-1]), *[int(k) for k in np.linspace(0, len(sides) - 1, 4).astype(int)]})
+    worst, med = 0.0, []
+    for k in check:
+        rest, F, X = cloths[k]
+        cp, ri, rv, sc = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+        outer, inner = batched.output_pattern(k)
+        if not (np.array_equal(outer, cp) and np.array_equal(inner, ri)):
+            raise RuntimeError(f"bench.py: pattern of cloth {k} differs from the oracle")
+        err = O.rel_errors(batched.get_results(k), rv, sc)
+        worst = max(worst, float(err.max()))
+        med.append(float(np.median(err)))
+    log(f"[bench] parity on cloths {check}: patterns bit-exact, max rel err {worst:.3e}")
+    if worst > 1e-12:
+        raise RuntimeError("bench.py: batched result does not match the oracle")
+
+    n_out = int(info["num_results"])
+    step = lambda: batched.run_graph(sp)  # noqa: E731
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - ts) * 1e3 / args.steps
+    log(f"[bench] batched: {args.steps} steps, {ms_per_step:.4f} ms/step")
+    kernel_ms = time_events(lambda: batched.launch_one(0, sp), stream, max(5, min(args.steps, 50)))
+    balg = sum(b_alg(c, 0) for c in counts) + 12 * n_out
+
+    # plain single-instance form, one program per distinct side, launched once per instance
+    plain_line = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        by_side = {}
+        for k, s in enumerate(sides):
+            by_side.setdefault(s, k)
+        plain = {}
+        for s, k in by_side.items():
+            rest, F, X = cloths[k]
+            plain[s] = Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, direct_assembly=True)
+        L = _native.lib()
+        with ThreadPoolExecutor(16) as ex:
+            rcs = list(ex.map(lambda u: L.symhip_cache_compile(u.source().encode(), None, None), plain.values()))
+        if any(rcs):
+            raise RuntimeError("bench.py: compiling the plain programs failed")
+        for s, u in plain.items():
+            u.compile().execute(inputs[by_side[s]])
+        torch.cuda.synchronize()
+        log(f"[bench] plain: {len(plain)} programs for {len(sides)} cloths built in {time.perf_counter() - t0:.1f}s")
+        for s, k in list(by_side.items())[:: max(1, len(by_side) // 4)]:
+            rest, F, X = cloths[k]
+            cp, ri, rv, sc = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+            if O.rel_errors(plain[s].get_results(), rv, sc).max() > 1e-12:
+                raise RuntimeError(f"bench.py: plain program of side {s} does not match the oracle")
+        seq = [plain[s] for s in sides]
+
+        def plain_step():
+            for u in seq:
+                u.run(sp)
+
+        plain_step()
+        torch.cuda.synchronize()
+        reps = max(3, min(args.steps, 10))
+        plain_ms = float(np.median([time_events(plain_step, stream, 1) for _ in range(reps)]))
+        plain_line = {"value": n_out / (plain_ms * 1e-3), "unit": "nnz/s", "ms_per_step": plain_ms,
+                      "programs": len(plain), "launches": 2 * len(sides),
+                      "note": "one waves-form program per cloth side; cloths of equal side share a program "
+                              "(same kernels and sizes, the first such cloth's positions)"}
+        for u in plain.values():
+            u.close()
+        log(f"[bench] plain per-instance: {plain_ms:.3f} ms/step")
+
+    copy_gbs = copy_bandwidth(dev, stream)
+    achieved = balg / (kernel_ms * 1e-3) / 1e9
+    value = n_out / (ms_per_step * 1e-3)
+    line = {
+        "metric": "output nnz/s FP64, batched+learned-tuned vs plain HIP; HBM GB/s vs peak",
+        "value": value,
+        "unit": "nnz/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched Baraff-Witkin cloth Hessians: {len(sides)} cloths, sides log-uniform "
+                               f"32..256 (seed 4), stretch+shear AD + bend",
+                   "instances": len(sides), "vertices": int(sum(len(r) for r, _, _ in cloths)),
+                   "faces": int(sum(len(F) for _, F, _ in cloths)), "output_nnz": n_out,
+                   "form": "fused assembly, one persistent launch replayed from a HIP graph",
+                   "options": opt, "tuned_from": os.path.relpath(TUNED, ROOT) if tuned else None,
+                   "tiling": info["tiling"], "grid": batched.launch_grid(0), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "symlib_fused",
+                     "algorithmic_bytes": balg, "kernel_ms": kernel_ms},
+        "plain_per_instance": plain_line,
+        "speedup_vs_plain": (value / plain_line["value"]) if plain_line else None,
+        "copy_bandwidth_gbs": copy_gbs,
+        "parity": {"pattern_bit_exact": True, "checked_instances": check, "max_rel_err": worst,
+                   "median_rel_err": float(np.median(med))},
+        "build_seconds": {"frontend": info["frontend_seconds"], "build": info["build_seconds"],
+                          "compile": info["compile_seconds"]},
+    }
+    if not args.no_cpu:
+        plans = {}
+        evals = []
+        for k in order[: max(1, len(order) // 2)][:: 8]:
+            rest, F, X = cloths[int(k)]
+            s = sides[int(k)]
+            if s not in plans:
+                plans[s] = O.BWPlan(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+            evals.append((lambda p=plans[s], X=X: p.values(X, with_scale=False), int(batched.output_shape(int(k))[2])))
+        v, n, tc = cpu_baseline(evals, args.cpu_seconds, O)
+        line["cpu_baseline"] = {"value": v, "unit": "nnz/s", "cores": O.num_threads(), "kind": "port",
+                                "sample": f"{n} cloth evaluations by the FP64 oracle (OpenMP, {O.num_threads()} "
+                                          f"threads) round-robin over {len(evals)} cloths of the batch "
+                                          f"(sides {min(sides[int(k)] for k in order[:len(order) // 2:8])}.."
+                                          f"{max(sides[int(k)] for k in order[:len(order) // 2:8])}) in {tc:.1f} s"}
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--count", type=int, default=256, help="c4: number of cloths")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="c4: skip the plain per-instance baseline")
+    ap.add_argument("--untuned", action="store_true", help="c4: ignore the tuned configuration")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--form", default="waves", choices=["tiled", "waves", "fused"], help="c1-c3 execution form")
+    ap.add_argument("--graph", action="store_true", help="c1-c3: replay the launches from a HIP graph")
+    ap.add_argument("--tile-lds", type=int, default=None, help="LDS bytes per tile")
+    ap.add_argument("--tpb", type=int, default=None, help="threads per block")
+    ap.add_argument("--groups", type=int, default=None, help="fused: work items per element")
+    ap.add_argument("--wpe", type=int, default=None, help="fused: minimum waves per SIMD")
+    ap.add_argument("--schedule", default=None, choices=["static", "queue", "largest", "interleaved"])
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the batched workload "
+              "shards by instance without communication (see the sharding section of the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs a HIP device")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    from symboliclib_amd.unit import device_available
+
+    if not device_available():
+        raise RuntimeError("bench.py: the native runtime sees no HIP device")
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    line = run_batched(args, dev, stream, sp) if args.workload == "c4" else run_single(args, dev, stream, sp)
     print(json.dumps(line), flush=True)
 
 

@@ -134,6 +134,19 @@ uint32_t symhip_num_launches(symhip_t h);
 /* Workgroups of launch `idx` as launched (SYMHIP_GRID_FILL resolved after compile). */
 uint32_t symhip_launch_grid(symhip_t h, uint32_t idx);
 
+/* Re-tuning a compiled program without rebuilding it: overwrite `n` words of the tile pool
+ * from word `offset`, replace launch `idx` (takes effect at the next symhip_compile), resize
+ * the work words (zeroed), read work words back (after synchronising). */
+int symhip_update_tiles(symhip_t h, uint64_t offset, uint64_t n, const uint32_t* data);
+int symhip_set_launch(symhip_t h, uint32_t idx, const symhip_launch* launch);
+int symhip_set_work_words(symhip_t h, uint64_t n);
+int symhip_read_work(symhip_t h, uint64_t offset, uint64_t n, uint32_t* out);
+/* Registers and residency of launch `idx` after compile: VGPRs per work-item, workgroups per
+ * compute unit at its block and LDS size, compute units of the device. */
+int symhip_launch_info(symhip_t h, uint32_t idx, int* vgprs, int* blocks_per_cu, int* cus);
+/* Rate of the device wall clock read by profiled kernels, in kHz. */
+int symhip_wall_clock_khz(int* khz);
+
 /* Copy output container k to host memory after synchronising (reference: getResult). */
 int symhip_get_result(symhip_t h, int k, double* host, size_t n);
 /* Copy output container k to device memory, asynchronously on `stream`. */

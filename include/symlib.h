@@ -47,7 +47,17 @@ typedef struct {
                                     instances interleaved */
     int output_groups;           /* fused form: work items per element, 0 = default (5) */
     int waves_per_eu;            /* fused form: minimum waves per SIMD to compile for, 0 = none */
+    int profile_tiles;           /* fused form: record wall-clock stamps per tile (symlib_unit_tile_profile) */
 } symlib_options;
+
+/* Fused-form program and code statistics (the autotuner's features). */
+typedef struct {
+    uint64_t num_tiles, num_outputs, num_elements, element_evaluations, contributions;
+    uint64_t lds_bytes, element_lds_bytes;         /* per workgroup: all LDS, element part */
+    uint64_t adds, muls, divs, sqrts, statements;  /* per element, over all output groups */
+    int32_t max_live;                              /* most values a work item keeps live */
+    int32_t output_groups, waves_per_eu, threads_per_block, schedule, profile_tiles;
+} symlib_fused_stats;
 
 typedef struct {
     uint64_t num_inputs;        /* input values */
@@ -96,6 +106,19 @@ int symlib_unit_output_shape(symlib_unit_t u, int k, int* rows, int* cols, uint6
 /* CSC outer (cols + 1) and inner (nnz) arrays of output container k */
 int symlib_unit_output_pattern(symlib_unit_t u, int k, int* outer, int* inner);
 int symlib_unit_get_info(symlib_unit_t u, symlib_unit_info* info);
+
+/* Fused form only.  retune: apply output_groups, waves_per_eu, threads_per_block, schedule and
+ * profile_tiles without rebuilding the tiles (tile_lds_bytes must be 0 or unchanged) and
+ * recompile if compiled; fused_source: the source those options would emit (valid until the
+ * next call on this unit); tile_stats: TILE_STATS (8) floats per tile in launch order (owned
+ * outputs, element contributions, elements, staged doubles, element LDS values, recipe
+ * words, output constants, runs); tile_profile: 4 wall-clock stamps per tile in launch order
+ * (start, staging done, elements done, outputs written) of the last profiled launch. */
+int symlib_unit_retune(symlib_unit_t u, const symlib_options* opt);
+const char* symlib_unit_fused_source(symlib_unit_t u, const symlib_options* opt);
+int symlib_unit_fused_stats(symlib_unit_t u, symlib_fused_stats* stats);
+int symlib_unit_tile_stats(symlib_unit_t u, float* out, size_t n);
+int symlib_unit_tile_profile(symlib_unit_t u, uint64_t* out, size_t n);
 /* emitted HIP source (valid until destroy) */
 const char* symlib_unit_source(symlib_unit_t u);
 /* runtime handle for device-side access (NULL before compile) */

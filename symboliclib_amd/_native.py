@@ -18,7 +18,8 @@ class Options(ctypes.Structure):
     _fields_ = [("threads_per_block", ctypes.c_int), ("decomposition_threshold", ctypes.c_int),
                 ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int), ("tiled", ctypes.c_int),
                 ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int), ("fused", ctypes.c_int),
-                ("schedule", ctypes.c_int), ("output_groups", ctypes.c_int), ("waves_per_eu", ctypes.c_int)]
+                ("schedule", ctypes.c_int), ("output_groups", ctypes.c_int), ("waves_per_eu", ctypes.c_int),
+                ("profile_tiles", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):
@@ -28,6 +29,14 @@ class UnitInfo(ctypes.Structure):
                 ("source_bytes", ctypes.c_uint64), ("frontend_seconds", ctypes.c_double),
                 ("build_seconds", ctypes.c_double), ("compile_seconds", ctypes.c_double), ("tiled", ctypes.c_uint32),
                 ("tile_pool_entries", ctypes.c_uint64), ("tiling", ctypes.c_char * 128)]
+
+
+class FusedStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in ("num_tiles", "num_outputs", "num_elements", "element_evaluations",
+                                                "contributions", "lds_bytes", "element_lds_bytes", "adds", "muls",
+                                                "divs", "sqrts", "statements")] + \
+               [(f, ctypes.c_int32) for f in ("max_live", "output_groups", "waves_per_eu", "threads_per_block",
+                                               "schedule", "profile_tiles")]
 
 
 class Launch(ctypes.Structure):
@@ -58,6 +67,12 @@ SIGNATURES = {
     "symhip_compile": (_I, [_VP, ctypes.c_char_p, ctypes.c_char_p]),
     "symhip_compile_to_file": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
     "symhip_cache_compile": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "symhip_update_tiles": (_I, [_VP, _U64, _U64, _VP]),
+    "symhip_set_launch": (_I, [_VP, _U32, ctypes.POINTER(Launch)]),
+    "symhip_set_work_words": (_I, [_VP, _U64]),
+    "symhip_read_work": (_I, [_VP, _U64, _U64, _VP]),
+    "symhip_launch_info": (_I, [_VP, _U32, _IP, _IP, _IP]),
+    "symhip_wall_clock_khz": (_I, [_IP]),
     "symhip_set_arg": (_I, [_VP, _I, _DP, _SZ]),
     "symhip_set_arg_device": (_I, [_VP, _I, _VP, _SZ, _VP]),
     "symhip_run": (_I, [_VP, _VP]),
@@ -87,10 +102,30 @@ SIGNATURES = {
     "symlib_unit_output_shape": (_I, [_VP, _I, _IP, _IP, ctypes.POINTER(_U64)]),
     "symlib_unit_output_pattern": (_I, [_VP, _I, _VP, _VP]),
     "symlib_unit_get_info": (_I, [_VP, ctypes.POINTER(UnitInfo)]),
+    "symlib_unit_retune": (_I, [_VP, ctypes.POINTER(Options)]),
+    "symlib_unit_fused_source": (ctypes.c_char_p, [_VP, ctypes.POINTER(Options)]),
+    "symlib_unit_fused_stats": (_I, [_VP, ctypes.POINTER(FusedStats)]),
+    "symlib_unit_tile_stats": (_I, [_VP, _VP, _SZ]),
+    "symlib_unit_tile_profile": (_I, [_VP, _VP, _SZ]),
     "symlib_unit_source": (ctypes.c_char_p, [_VP]),
     "symlib_unit_runtime": (_VP, [_VP]),
     "symlib_unit_destroy": (None, [_VP]),
     "symlib_tutorial_known_answers": (_I, [_DP]),
+    # symcost.h
+    "symcost_create": (_I, [ctypes.POINTER(_VP), _I, _I, _U64]),
+    "symcost_train": (_I, [_VP, _VP, _VP, _SZ, _I, _I, ctypes.c_float, _U64, _VP]),
+    "symcost_predict": (_I, [_VP, _VP, _SZ, _VP]),
+    "symcost_num_params": (_I, [_VP, ctypes.POINTER(_SZ)]),
+    "symcost_get_params": (_I, [_VP, _VP, _SZ]),
+    "symcost_set_params": (_I, [_VP, _VP, _SZ]),
+    "symcost_get_scaling": (_I, [_VP, _VP, _VP, _VP]),
+    "symcost_set_scaling": (_I, [_VP, _VP, _VP, _VP]),
+    "symcost_step": (_I, [_VP, _VP, _VP, _SZ, ctypes.c_float]),
+    "symcost_get_gradient": (_I, [_VP, _VP, _SZ]),
+    "symcost_save": (_I, [_VP, ctypes.c_char_p]),
+    "symcost_load": (_I, [ctypes.POINTER(_VP), ctypes.c_char_p]),
+    "symcost_destroy": (None, [_VP]),
+    "symcost_last_error": (ctypes.c_char_p, []),
     "symlib_last_error": (ctypes.c_char_p, []),
 }
 
@@ -115,5 +150,6 @@ def lib():
 def check(rc, what):
     if rc != 0:
         L = lib()
-        msg = (L.symlib_last_error() or b"").decode() or (L.symhip_last_error() or b"").decode()
+        msg = ((L.symlib_last_error() or b"").decode() or (L.symhip_last_error() or b"").decode()
+               or (L.symcost_last_error() or b"").decode())
         raise SymlibError(f"{what} failed ({rc}): {msg}")

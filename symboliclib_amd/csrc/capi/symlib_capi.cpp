@@ -48,6 +48,7 @@ Sym::Device makeDevice(const symlib_options* opt) {
     d.schedule = o.schedule;
     if (o.output_groups > 0) d.outputGroups = o.output_groups;
     d.wavesPerEU = o.waves_per_eu;
+    d.profileTiles = o.profile_tiles != 0;
     return d;
 }
 
@@ -55,6 +56,7 @@ Sym::Device makeDevice(const symlib_options* opt) {
 
 struct symlib_unit {
     std::unique_ptr<Sym::ComputeUnit<double>> unit;
+    std::string scratch;  // fused_source result
     std::vector<Sym::SparseMatrix<double>> patterns;  // output patterns (values unused)
     symlib_unit_info info{};
 };
@@ -151,6 +153,7 @@ void symlib_default_options(symlib_options* opt) {
     opt->schedule = 1;
     opt->output_groups = 0;
     opt->waves_per_eu = 0;
+    opt->profile_tiles = 0;
 }
 
 int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {
@@ -266,6 +269,92 @@ int symlib_unit_execute(symlib_unit_t u, int num_inputs, const double* const* in
 int symlib_unit_run(symlib_unit_t u, void* stream) {
     if (!u) return 1;
     return guarded([&] { u->unit->run(stream); });
+}
+
+namespace {
+
+Sym::FusedOptions fusedOptionsFrom(const symlib_unit* u, const symlib_options* opt) {
+    if (!u->unit->isFused()) throw std::invalid_argument("not a fused-form unit");
+    Sym::FusedOptions fo = u->unit->getFusedOptions();
+    if (!opt) return fo;
+    if (opt->tile_lds_bytes > 0 && (size_t)opt->tile_lds_bytes != fo.ldsBytes)
+        throw std::invalid_argument("the tile LDS budget cannot change without rebuilding the unit");
+    if (opt->threads_per_block > 0) fo.threadsPerBlock = opt->threads_per_block;
+    if (opt->output_groups > 0) fo.outputGroups = opt->output_groups;
+    if (opt->waves_per_eu < 0 || opt->waves_per_eu > 8) throw std::invalid_argument("waves per EU must be 0..8");
+    fo.wavesPerEU = opt->waves_per_eu;
+    if (opt->schedule < 0 || opt->schedule > 4) throw std::invalid_argument("unknown schedule");
+    fo.schedule = (Sym::FusedSchedule)opt->schedule;
+    fo.profile = opt->profile_tiles != 0;
+    return fo;
+}
+
+}  // namespace
+
+int symlib_unit_retune(symlib_unit_t u, const symlib_options* opt) {
+    if (!u) return 1;
+    return guarded([&] {
+        u->unit->retuneFused(fusedOptionsFrom(u, opt));
+        u->info.source_bytes = u->unit->getCode().size();
+    });
+}
+
+const char* symlib_unit_fused_source(symlib_unit_t u, const symlib_options* opt) {
+    if (!u) return nullptr;
+    const char* out = nullptr;
+    int rc = guarded([&] {
+        u->scratch = u->unit->fusedSource(fusedOptionsFrom(u, opt));
+        out = u->scratch.c_str();
+    });
+    return rc ? nullptr : out;
+}
+
+int symlib_unit_fused_stats(symlib_unit_t u, symlib_fused_stats* st) {
+    if (!u || !st) return 1;
+    return guarded([&] {
+        if (!u->unit->isFused()) throw std::invalid_argument("not a fused-form unit");
+        const auto& F = u->unit->getFusedProgram();
+        const auto& C = u->unit->getFusedStats();
+        const auto& O = u->unit->getFusedOptions();
+        *st = symlib_fused_stats{};
+        st->num_tiles = F.numTiles;
+        st->num_outputs = F.numResults;
+        st->num_elements = F.numElements;
+        st->element_evaluations = F.elementEvaluations;
+        st->contributions = F.numContributions;
+        st->lds_bytes = F.ldsBytes;
+        st->element_lds_bytes = F.elementLdsBytes;
+        st->adds = C.adds;
+        st->muls = C.muls;
+        st->divs = C.divs;
+        st->sqrts = C.sqrts;
+        st->statements = C.statements;
+        st->max_live = C.maxLive;
+        st->output_groups = O.outputGroups;
+        st->waves_per_eu = O.wavesPerEU;
+        st->threads_per_block = O.threadsPerBlock;
+        st->schedule = (int32_t)O.schedule;
+        st->profile_tiles = O.profile ? 1 : 0;
+    });
+}
+
+int symlib_unit_tile_stats(symlib_unit_t u, float* out, size_t n) {
+    if (!u) return 1;
+    return guarded([&] {
+        if (!u->unit->isFused()) throw std::invalid_argument("not a fused-form unit");
+        const auto st = Sym::tileStats(u->unit->getFusedProgram());
+        if (n != st.size()) throw std::invalid_argument("tile stats: n must be 8 x the number of tiles");
+        std::copy(st.begin(), st.end(), out);
+    });
+}
+
+int symlib_unit_tile_profile(symlib_unit_t u, uint64_t* out, size_t n) {
+    if (!u) return 1;
+    return guarded([&] {
+        const auto st = u->unit->tileProfile();
+        if (n != st.size()) throw std::invalid_argument("tile profile: n must be 4 x the number of tiles");
+        std::copy(st.begin(), st.end(), out);
+    });
 }
 
 int symlib_unit_run_graph(symlib_unit_t u, void* stream) {

@@ -148,6 +148,7 @@ struct symhip_program {
     uint32_t* o = nullptr;
     uint32_t* tiles = nullptr;
     uint32_t* work = nullptr;
+    uint64_t numTilesData = 0, numWork = 0;
     std::vector<symhip_launch> launches;
     std::vector<uint32_t> grids;  // resolved workgroup counts
     hipGraph_t graph = nullptr;
@@ -231,6 +232,8 @@ int symhip_create(symhip_t* out, const symhip_program_desc* d) {
         symhip_destroy(h);
         return rc;
     }
+    h->numTilesData = d->num_tiles_data;
+    h->numWork = d->num_work_words;
     *out = h;
     return SYMHIP_SUCCESS;
 }
@@ -394,6 +397,82 @@ int symhip_run(symhip_t h, void* stream) {
         int rc = symhip_launch_one(h, i, stream);
         if (rc) return rc;
     }
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_update_tiles(symhip_t h, uint64_t offset, uint64_t n, const uint32_t* data) {
+    if (!h || (n && !data) || offset + n > h->numTilesData) return fail(SYMHIP_E_INVALID, "symhip_update_tiles: range outside the tile pool");
+    if (n) HIP_TRY(hipMemcpy(h->tiles + offset, data, sizeof(uint32_t) * n, hipMemcpyHostToDevice), "hipMemcpy(tiles)");
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_set_launch(symhip_t h, uint32_t idx, const symhip_launch* launch) {
+    if (!h || !launch || idx >= h->launches.size()) return fail(SYMHIP_E_INVALID, "symhip_set_launch: bad launch index");
+    h->names[idx] = launch->name ? launch->name : "";
+    h->launches[idx] = *launch;
+    h->functions.clear();  // compile again before running
+    if (h->graphExec) {
+        (void)hipGraphExecDestroy(h->graphExec);
+        h->graphExec = nullptr;
+    }
+    if (h->graph) {
+        (void)hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_set_work_words(symhip_t h, uint64_t n) {
+    if (!h) return fail(SYMHIP_E_INVALID, "symhip_set_work_words: null handle");
+    if (n == h->numWork) return SYMHIP_SUCCESS;
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    HIP_TRY(hipFree(h->work), "hipFree(work)");
+    h->work = nullptr;
+    h->numWork = 0;
+    if (n) {
+        HIP_TRY(hipMalloc((void**)&h->work, sizeof(uint32_t) * n), "hipMalloc(work)");
+        HIP_TRY(hipMemset(h->work, 0, sizeof(uint32_t) * n), "hipMemset(work)");
+    }
+    h->numWork = n;
+    if (h->graphExec) {
+        (void)hipGraphExecDestroy(h->graphExec);
+        h->graphExec = nullptr;
+    }
+    if (h->graph) {
+        (void)hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_read_work(symhip_t h, uint64_t offset, uint64_t n, uint32_t* out) {
+    if (!h || (n && !out) || offset + n > h->numWork) return fail(SYMHIP_E_INVALID, "symhip_read_work: range outside the work words");
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (n) HIP_TRY(hipMemcpy(out, h->work + offset, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "hipMemcpy(work)");
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_launch_info(symhip_t h, uint32_t idx, int* vgprs, int* blocks_per_cu, int* cus) {
+    if (!h || idx >= h->launches.size()) return fail(SYMHIP_E_INVALID, "symhip_launch_info: bad launch index");
+    if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
+    int dev = 0, n = 0, per = 0, regs = 0;
+    HIP_TRY(hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, h->functions[idx]), "hipFuncGetAttribute");
+    HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per, h->functions[idx], (int)h->launches[idx].block,
+                                                               h->launches[idx].shared),
+            "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    HIP_TRY(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    if (vgprs) *vgprs = regs;
+    if (blocks_per_cu) *blocks_per_cu = per;
+    if (cus) *cus = n;
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_wall_clock_khz(int* khz) {
+    if (!khz) return fail(SYMHIP_E_INVALID, "symhip_wall_clock_khz: null argument");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    HIP_TRY(hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
     return SYMHIP_SUCCESS;
 }
 

@@ -87,8 +87,10 @@ void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
         if (device.wavesPerEU < 0 || device.wavesPerEU > 8) throw std::invalid_argument("ComputeUnit: waves per EU must be 0..8");
         fo.outputGroups = device.outputGroups;
         fo.wavesPerEU = device.wavesPerEU;
+        fo.profile = device.profileTiles;
         fusedProgram = assembly.buildFused(opt, fo, io, assemblyPatterns);
-        code = emitHipFused(fusedProgram, fo, launches);
+        fusedOptions = fo;
+        code = emitHipFused(fusedProgram, fo, launches, &fusedStats);
         fusedForm = true;
         tiledForm = false;
         tilingNote = fusedProgram.summary();
@@ -172,7 +174,7 @@ void ComputeUnitBase::compileProgram() {
         d.num_tiles_data = fusedProgram.pool.size();
         d.tiles_data = fusedProgram.pool.data();
     }
-    d.num_work_words = 2;
+    d.num_work_words = fusedForm && fusedOptions.profile ? tileProfileWords(fusedProgram.numTiles) : 2;
     d.num_launches = (uint32_t)ls.size();
     d.launches = ls.data();
 
@@ -197,6 +199,41 @@ void ComputeUnitBase::setArgImpl(int i, const double* d, size_t n) {
 void ComputeUnitBase::runImpl(void* stream) {
     if (!rt) throw std::runtime_error("ComputeUnit: run before compile()");
     check(symhip_run(rt, stream), "symhip_run");
+}
+
+void ComputeUnitBase::retuneFused(const FusedOptions& fo) {
+    if (!fusedForm) throw std::invalid_argument("ComputeUnit: retune applies to the fused form");
+    if (fo.ldsBytes != fusedOptions.ldsBytes || fo.maxOutputs != fusedOptions.maxOutputs)
+        throw std::invalid_argument("ComputeUnit: the tile LDS budget cannot change without rebuilding the tiles");
+    std::vector<LaunchDesc> ls;
+    FusedCodeStats st;
+    std::string src = emitHipFused(fusedProgram, fo, ls, &st);
+    orderTiles(fusedProgram, fo.schedule);
+    if (rt) {
+        check(symhip_update_tiles(rt, 1, fusedProgram.numTiles, fusedProgram.pool.data() + 1), "symhip_update_tiles");
+        symhip_launch L{ls[0].name.c_str(), ls[0].grid, ls[0].block, ls[0].shared};
+        check(symhip_set_launch(rt, 0, &L), "symhip_set_launch");
+        check(symhip_set_work_words(rt, fo.profile ? tileProfileWords(fusedProgram.numTiles) : 2), "symhip_set_work_words");
+        check(symhip_compile(rt, src.c_str(), nullptr), "symhip_compile");
+    }
+    code = std::move(src);
+    launches = std::move(ls);
+    fusedOptions = fo;
+    fusedStats = st;
+}
+
+std::string ComputeUnitBase::fusedSource(const FusedOptions& fo) const {
+    if (!fusedForm) throw std::invalid_argument("ComputeUnit: not a fused-form unit");
+    std::vector<LaunchDesc> ls;
+    return emitHipFused(fusedProgram, fo, ls);
+}
+
+std::vector<uint64_t> ComputeUnitBase::tileProfile() const {
+    if (!rt || !fusedForm || !fusedOptions.profile) throw std::runtime_error("ComputeUnit: no profiled fused launch");
+    std::vector<uint64_t> stamps(4 * fusedProgram.numTiles);
+    check(symhip_read_work(rt, 2, 8 * fusedProgram.numTiles, reinterpret_cast<uint32_t*>(stamps.data())),
+          "symhip_read_work");
+    return stamps;
 }
 
 void ComputeUnitBase::runGraph(void* stream) {

@@ -78,6 +78,7 @@ public:
     int schedule = (int)FusedSchedule::Queue;
     int outputGroups = 5;
     int wavesPerEU = 0;
+    bool profileTiles = false;
 
 private:
     void set(const DecompositionThreshold& nt) { decompositionThreshold = (unsigned)nt.x; }
@@ -138,6 +139,8 @@ protected:
     std::vector<uint32_t> tilePool;
     bool tiledForm = false;
     FusedProgram fusedProgram;
+    FusedOptions fusedOptions;
+    FusedCodeStats fusedStats;
     bool fusedForm = false;
     std::string tilingNote;
     symhip_t rt = nullptr;
@@ -170,6 +173,15 @@ public:
     bool isTiled() const { return tiledForm; }
     bool isFused() const { return fusedForm; }
     const FusedProgram& getFusedProgram() const { return fusedProgram; }
+    const FusedOptions& getFusedOptions() const { return fusedOptions; }
+    const FusedCodeStats& getFusedStats() const { return fusedStats; }
+    // Fused form: new output groups, waves per EU, block size, schedule or profiling without
+    // rebuilding the tiles (ldsBytes must stay); recompiles when already compiled.
+    void retuneFused(const FusedOptions& fo);
+    // Fused form: the source the options would emit (for compiling ahead in parallel).
+    std::string fusedSource(const FusedOptions& fo) const;
+    // Profiled fused launch: four wall-clock stamps per tile in launch order.
+    std::vector<uint64_t> tileProfile() const;
     // output patterns of a unit built from a HessianAssembly (one per instance)
     const std::vector<SparseMatrix<double>>& outputPatterns() const { return assemblyPatterns; }
     const std::string& tilingInfo() const { return tilingNote; }

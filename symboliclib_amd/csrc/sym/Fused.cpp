@@ -20,7 +20,7 @@ namespace Sym {
 namespace {
 
 constexpr uint32_t NONE = 0xffffffffu;
-constexpr int HEADER = 10;  // fixed header words before the per-template triples
+constexpr int HEADER = 12;  // fixed header words before the per-template triples
 
 uint32_t u32(size_t v, const char* what) {
     if (v > std::numeric_limits<uint32_t>::max()) throw std::length_error(std::string("fused form: ") + what + " exceeds 32 bits");
@@ -36,6 +36,8 @@ struct TemplateLayout {
     std::vector<char> uniform;
     std::vector<int> gatherNode, gatherDof;
     int numNodes = 0, numParams = 0, numOutputs = 0;
+    // LDS doubles per element: staged arguments and parameters, then its outputs
+    int ldsPerElement() const { return (int)gatherNode.size() + numParams + numOutputs; }
 };
 
 // Tiles of one instance, pool offsets relative to the instance's chunk.
@@ -44,15 +46,16 @@ struct InstanceTiles {
     std::vector<double> consts;
     std::vector<size_t> header;  // tile header offsets in pool
     std::vector<double> cost;    // per tile: outputs + element values
-    size_t evaluations = 0, contributions = 0, lds = 0, outputs = 0;
+    size_t evaluations = 0, contributions = 0, lds = 0, elementLds = 0, outputs = 0;
 };
 
 }  // namespace
 
 std::string FusedProgram::summary() const {
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "fused tiles=%zu lds=%zu recompute=%.3f templates=%zu contributions=%zu pool=%zu",
-                  numTiles, ldsBytes, recompute(), templates.size(), numContributions, pool.size());
+    std::snprintf(buf, sizeof(buf),
+                  "fused tiles=%zu lds=%zu element_lds=%zu recompute=%.3f templates=%zu contributions=%zu pool=%zu",
+                  numTiles, ldsBytes, elementLdsBytes, recompute(), templates.size(), numContributions, pool.size());
     return buf;
 }
 
@@ -148,7 +151,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         ft.argDof = tl.gatherDof;
         ft.numParams = tl.numParams;
         ft.numOutputs = tl.numOutputs;
-        if ((size_t)tl.numOutputs * 8 > fo.ldsBytes)
+        if ((size_t)tl.ldsPerElement() * 8 > fo.ldsBytes)
             throw std::invalid_argument("fused form: one element needs more LDS than a tile holds");
         fp.templates.push_back(std::move(ft));
     }
@@ -254,7 +257,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 }
                 size_t newSlots = 0;
                 for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q)
-                    if (stamp[inc[q]] != tileId) newSlots += (size_t)L[elemT[e0 + inc[q]]].numOutputs;
+                    if (stamp[inc[q]] != tileId) newSlots += (size_t)L[elemT[e0 + inc[q]]].ldsPerElement();
                 if (!tNodes.empty() && (slots + newSlots > ldsValues || outs + outCount[v] > fo.maxOutputs)) break;
                 if (slots + newSlots > ldsValues)
                     throw std::invalid_argument("fused form: the elements of one node need more LDS than a tile holds");
@@ -363,6 +366,8 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 pool[H + 8] = NONE;
             }
             pool[H + 9] = 0;  // result base, set at merge
+            pool[H + 10] = chunksOff + 3 * (uint32_t)nch - runsOff;
+            pool[H + 11] = (uint32_t)cvals.size() * (anyConst ? 1u : 0u);
             size_t evals = 0;
             for (size_t t = 0; t < NT; ++t) {
                 const auto& te = tElems[t];
@@ -384,10 +389,14 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                         if (tl.argParam[g] >= 0) out.consts[pb + (size_t)tl.argParam[g] * cnt + q] = a[g].value;
                 }
             }
+            size_t staged = 0;
+            for (size_t t = 0; t < NT; ++t) staged += tElems[t].size() * (size_t)(L[t].gatherNode.size() + L[t].numParams);
+            const size_t tileLds = 8 * (ldsBase[NT] + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * (size_t)pool[H + 11];
             out.cost.push_back((double)m + (double)ldsBase[NT]);
             out.evaluations += evals;
             out.contributions += slotList.size();
-            out.lds = std::max(out.lds, ldsBase[NT] * 8);
+            out.lds = std::max(out.lds, tileLds);
+            out.elementLds = std::max(out.elementLds, 8 * (ldsBase[NT] + staged));
             out.outputs += m;
             ++tileId;
         }
@@ -416,6 +425,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         fp.elementEvaluations += tiles[k].evaluations;
         fp.numContributions += tiles[k].contributions;
         fp.ldsBytes = std::max(fp.ldsBytes, tiles[k].lds);
+        fp.elementLdsBytes = std::max(fp.elementLdsBytes, tiles[k].elementLds);
     }
     u32(pb, "tile pool");
     u32(cb, "constants");
@@ -452,25 +462,76 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         patterns.push_back(std::move(pats[k]));
     }
     fp.numResults = rb;
-    switch (fo.schedule) {
-        case FusedSchedule::QueueLargest:
-            std::stable_sort(refs.begin(), refs.end(), [](const TileRef& a, const TileRef& b) { return a.cost > b.cost; });
-            break;
-        case FusedSchedule::Interleaved:
-            std::stable_sort(refs.begin(), refs.end(), [](const TileRef& a, const TileRef& b) {
-                return a.index != b.index ? a.index < b.index : a.inst < b.inst;
-            });
-            break;
-        default:
-            break;
+    fp.tileHeader.resize(nTiles);
+    fp.tileInstance.resize(nTiles);
+    fp.tileIndex.resize(nTiles);
+    fp.tileCost.resize(nTiles);
+    for (size_t t = 0; t < nTiles; ++t) {
+        fp.tileHeader[t] = refs[t].header;
+        fp.tileInstance[t] = refs[t].inst;
+        fp.tileIndex[t] = refs[t].index;
+        fp.tileCost[t] = (float)refs[t].cost;
     }
-    for (size_t t = 0; t < nTiles; ++t) fp.pool[1 + t] = refs[t].header;
+    orderTiles(fp, fo.schedule);
     fp.numTiles = nTiles;
     if (fp.ldsBytes == 0) fp.ldsBytes = 8;
     timer("fused: merge");
     validateFused(fp);
     timer("fused: validate");
     return fp;
+}
+
+std::vector<float> tileStats(const FusedProgram& fp) {
+    const size_t NT = fp.templates.size();
+    const auto& P = fp.pool;
+    std::vector<float> out(fp.numTiles * TILE_STATS);
+    for (size_t t = 0; t < fp.numTiles; ++t) {
+        const size_t H = P[1 + t];
+        const uint32_t m = P[H + 2];
+        const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[H + P[H + 5]]);
+        size_t S = 0;
+        for (size_t o = 0; o < m; ++o) S += cnts[o] & 0x7fu;
+        size_t elems = 0, staged = 0, vals = 0, runs = 0;
+        for (size_t q = 0; q < NT; ++q) {
+            const size_t cnt = P[H + HEADER + 3 * q];
+            elems += cnt;
+            staged += cnt * (fp.templates[q].argNode.size() + (size_t)fp.templates[q].numParams);
+            vals += cnt * (size_t)fp.templates[q].numOutputs;
+        }
+        while (P[H + P[H + 4] + 2 * runs + 1] != m) ++runs;
+        float* f = &out[t * TILE_STATS];
+        f[0] = (float)m;
+        f[1] = (float)S;
+        f[2] = (float)elems;
+        f[3] = (float)staged;
+        f[4] = (float)vals;
+        f[5] = (float)P[H + 10];
+        f[6] = (float)P[H + 11];
+        f[7] = (float)runs;
+    }
+    return out;
+}
+
+void orderTiles(FusedProgram& fp, FusedSchedule schedule) {
+    const size_t n = fp.tileHeader.size();
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    switch (schedule) {
+        case FusedSchedule::QueueLargest:
+            std::stable_sort(order.begin(), order.end(),
+                             [&](uint32_t a, uint32_t b) { return fp.tileCost[a] > fp.tileCost[b]; });
+            break;
+        case FusedSchedule::Interleaved:
+        case FusedSchedule::Strided:
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                return fp.tileIndex[a] != fp.tileIndex[b] ? fp.tileIndex[a] < fp.tileIndex[b]
+                                                          : fp.tileInstance[a] < fp.tileInstance[b];
+            });
+            break;
+        default:
+            break;
+    }
+    for (size_t t = 0; t < n; ++t) fp.pool[1 + t] = fp.tileHeader[order[t]];
 }
 
 void validateFused(const FusedProgram& fp) {
@@ -490,7 +551,10 @@ void validateFused(const FusedProgram& fp) {
         const uint32_t cOff = P[H + 8], rbase = P[H + 9];
         need(chunks + 3 * (size_t)nch <= P.size() && counts + ((size_t)m + 3) / 4 <= P.size(), "tables outside the pool");
         // LDS of the tile and element data
-        size_t lds = 0;
+        size_t lds = 0, staged = 0;
+        need(P[H + 10] == P[H + 7] + 3 * nch - P[H + 4] && P[H + 4] <= P[H + 5] && P[H + 5] <= P[H + 6] &&
+                 P[H + 6] <= P[H + 7],
+             "recipe tables not contiguous");
         for (size_t q = 0; q < NT; ++q) {
             const FusedTemplate& ft = fp.templates[q];
             const size_t cnt = P[H + HEADER + 3 * q], nodes = H + P[H + HEADER + 3 * q + 1], prm = P[H + HEADER + 3 * q + 2];
@@ -500,8 +564,9 @@ void validateFused(const FusedProgram& fp) {
             for (size_t g = 0; g < ft.argDof.size(); ++g)
                 need((size_t)B + (size_t)ft.argDof[g] * N + N <= fp.numInputs, "gather outside the inputs");
             lds += cnt * (size_t)ft.numOutputs;
+            staged += cnt * (ft.argNode.size() + (size_t)ft.numParams);
         }
-        need(lds * 8 <= fp.ldsBytes && lds <= 65536, "tile LDS exceeds the launch's LDS");
+        need(lds <= 65536, "tile LDS values exceed 16-bit slots");
         const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[counts]);
         size_t S = 0, numConst = 0;
         for (size_t o = 0; o < m; ++o) {
@@ -509,6 +574,9 @@ void validateFused(const FusedProgram& fp) {
             numConst += cnts[o] >> 7;
         }
         if (numConst) need(cOff != 0xffffffffu && (size_t)cOff + numConst <= fp.consts.size(), "output constants outside the constants");
+        need(P[H + 11] == numConst, "output constant count");
+        need(8 * (lds + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * numConst <= fp.ldsBytes,
+             "tile LDS exceeds the launch's LDS");
         need(slots + (S + 1) / 2 <= P.size(), "slot list outside the pool");
         const uint16_t* sl = reinterpret_cast<const uint16_t*>(&P[slots]);
         for (size_t s = 0; s < S; ++s) need(sl[s] < lds, "contribution outside the tile's LDS");

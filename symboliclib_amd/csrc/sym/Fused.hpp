@@ -25,10 +25,15 @@
 //     H[7] rel chunks: (first slot, run, first constant) per 64-output chunk
 //     H[8] output constants in c[], one per output with bit 7 set, in output order, or
 //          0xffffffff
-//     H[9 + 3 t], H[10 + 3 t], H[11 + 3 t]: template t's element count, rel node list
+//     H[9] result base of the instance
+//     H[10] words from runs to the end of chunks (the recipe, contiguous)
+//     H[11] number of output constants
+//     H[12 + 3 t], H[13 + 3 t], H[14 + 3 t]: template t's element count, rel node list
 //       (numNodes_t columns of count), parameter block in c[] (numParams_t columns of count)
-//   LDS of a tile: template t's outputs at L_t + slot * count_t + i, L_t = sum over t' < t
-//   of count_t' * numOutputs_t'.
+//   LDS of a tile, in doubles: element values (template t's outputs at V_t + slot * count_t
+//   + i, V_t = sum over t' < t of count_t' * numOutputs_t'), then the staged element
+//   arguments and parameters (template t: argument k at A_t + k * count_t + i), then the
+//   recipe (u32 words copied from H[4]), then the output constants.
 #pragma once
 
 #include <cstdint>
@@ -44,7 +49,8 @@ enum class FusedSchedule : int {
     Static = 0,       // one workgroup per tile, tiles in build order
     Queue = 1,        // persistent workgroups pull tiles from an atomic counter, build order
     QueueLargest = 2, // persistent, largest tiles first
-    Interleaved = 3   // persistent, tiles of different instances interleaved round-robin
+    Interleaved = 3,  // persistent, tiles of different instances interleaved round-robin
+    Strided = 4       // persistent without a counter: workgroup b takes tiles b, b + grid, ...
 };
 
 struct FusedOptions {
@@ -54,7 +60,14 @@ struct FusedOptions {
     FusedSchedule schedule = FusedSchedule::Queue;
     int outputGroups = 5;             // work items per element (outputs split between them)
     int wavesPerEU = 0;               // minimum waves per SIMD asked of the compiler (0 = none)
+    bool profile = false;             // record wall-clock stamps per tile (see tileProfileWords)
+    bool shareReciprocals = true;     // one 1/d per repeated denominator d (x/d -> x*(1/d))
 };
+
+// Profiled launches record, per tile in launch order, four 64-bit wall-clock stamps (tile
+// start, staging done, element evaluation done, outputs written) after the two counter words
+// of `work`.
+inline size_t tileProfileWords(size_t numTiles) { return 2 + 8 * numTiles; }
 
 struct FusedTemplate {
     Symbolic code;                 // ASSIGN(VAR(s,-2), ...) over VAR(a,-3) arguments, VAR(q,-4) parameters
@@ -68,19 +81,40 @@ struct FusedProgram {
     std::vector<FusedTemplate> templates;
     std::vector<uint32_t> pool;
     std::vector<double> consts;
+    // per tile in build order: header offset, instance, index within the instance, cost
+    // estimate (outputs + element values); the launch order in pool[1..] is derived from it
+    std::vector<uint32_t> tileHeader, tileInstance, tileIndex;
+    std::vector<float> tileCost;
     size_t numInputs = 0, numResults = 0;
     std::vector<size_t> outStart, outLength;
-    size_t numTiles = 0, ldsBytes = 0;
+    size_t numTiles = 0, ldsBytes = 0, elementLdsBytes = 0;  // per workgroup: all LDS, element part
     size_t elementEvaluations = 0, numElements = 0, numContributions = 0;
     double recompute() const { return numElements ? (double)elementEvaluations / (double)numElements : 0.0; }
     std::string summary() const;
 };
 
+// Per tile in launch order, TILE_STATS floats: owned outputs, element contributions, elements,
+// staged doubles, element LDS values, recipe words, output constants, runs.
+constexpr int TILE_STATS = 8;
+std::vector<float> tileStats(const FusedProgram& fp);
+
+// Write the tile table pool[1..numTiles] in the order of `schedule`.
+void orderTiles(FusedProgram& fp, FusedSchedule schedule);
+
 // Walks the pool as the kernel does and checks every index against its buffer; throws
 // std::logic_error on the first violation.
 void validateFused(const FusedProgram& fp);
 
-// Generated source: one __global__ symlib_fused and one device function per template.
-std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches);
+// What the emitted element code does per element, summed over the templates and their output
+// groups: FP64 operations, statements, and the most values any one work item keeps live.
+struct FusedCodeStats {
+    size_t adds = 0, muls = 0, divs = 0, sqrts = 0, statements = 0;
+    int maxLive = 0;
+};
+
+// Generated source: one __global__ symlib_fused, per template a staging function and one
+// device function per output group.
+std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches,
+                         FusedCodeStats* stats = nullptr);
 
 }  // namespace Sym

@@ -221,9 +221,50 @@ struct Stmt {
 };
 constexpr int ARG = 1 << 24, PARAM = 1 << 25;
 
-std::vector<Stmt> flatten(const Symbolic& block) {
+Symbolic withReciprocals(const Symbolic& y, const std::map<hash_t, int>& rcp) {
+    if (y.numChilds() == 0) return y;
+    if (y.op() == DIV) {
+        auto it = rcp.find(y[1].ahash());
+        if (it != rcp.end()) return Symbolic(MUL, withReciprocals(y[0], rcp), Symbolic(it->second, LOCAL_VAR));
+    }
+    std::vector<Symbolic> ch;
+    for (auto& c : y) ch.push_back(withReciprocals(c, rcp));
+    return Symbolic(y.op(), ch);
+}
+
+// A denominator divided by two or more times gets one reciprocal, defined once
+// (x / d -> x * r, r = 1 / d): an FP64 division costs about ten multiplies on CDNA.
+void shareReciprocals(std::vector<Symbolic>& seq) {
+    std::map<hash_t, int> count;
+    std::map<hash_t, Symbolic> denom;
+    int nextLocal = 0;
+    for (auto& b : seq) {
+        if (b.op() == DEFINE) nextLocal = std::max(nextLocal, b[0].variable()[0] + 1);
+        preOrderTraverse(b[1], [&](const Symbolic& y) {
+            if (y.op() == VAR && y.variable()[1] == LOCAL_VAR) nextLocal = std::max(nextLocal, y.variable()[0] + 1);
+            if (y.op() == DIV) {
+                ++count[y[1].ahash()];
+                denom.emplace(y[1].ahash(), y[1]);
+            }
+            return true;
+        });
+    }
+    std::map<hash_t, int> rcp;
+    std::vector<Symbolic> defs;
+    for (auto& kv : count)
+        if (kv.second >= 2) {
+            rcp[kv.first] = nextLocal;
+            defs.emplace_back(DEFINE, Symbolic(nextLocal, LOCAL_VAR), Symbolic(DIV, Symbolic(1.0), denom.at(kv.first)));
+            ++nextLocal;
+        }
+    if (rcp.empty()) return;
+    std::vector<Symbolic> out(defs.begin(), defs.end());
+    for (auto& b : seq) out.emplace_back(b.op(), b[0], withReciprocals(b[1], rcp));
+    seq.swap(out);
+}
+
+std::vector<Stmt> flatten(const Symbolic& block, bool reciprocals = false) {
     std::vector<Stmt> out;
-    std::vector<const Symbolic*> st{&block};
     std::vector<Symbolic> seq;
     // BLOCKs nest only at the top; keep statement order
     std::function<void(const Symbolic&)> walk = [&](const Symbolic& b) {
@@ -234,6 +275,7 @@ std::vector<Stmt> flatten(const Symbolic& block) {
         seq.push_back(b);
     };
     walk(block);
+    if (reciprocals) shareReciprocals(seq);
     for (auto& b : seq) {
         Stmt s;
         s.s = b;
@@ -312,6 +354,39 @@ std::vector<size_t> lowPressureOrder(const std::vector<Stmt>& S, const std::vect
             if (--indeg[j] == 0) ready.push_back(j);
     }
     return order;
+}
+
+// Largest number of values live at once when the statements run in `order` (arguments count
+// from their first use), and FP64 operation counts of those statements.
+void scheduleStats(const std::vector<Stmt>& S, const std::vector<size_t>& order, FusedCodeStats& st) {
+    std::map<int, size_t> last;
+    for (size_t pos = 0; pos < order.size(); ++pos)
+        for (int u : S[order[pos]].uses) last[u] = pos;
+    std::set<int> live;
+    size_t mx = 0;
+    for (size_t pos = 0; pos < order.size(); ++pos) {
+        const Stmt& s = S[order[pos]];
+        for (int u : s.uses) live.insert(u);
+        if (s.def >= 0 && last.count(s.def)) live.insert(s.def);
+        mx = std::max(mx, live.size());
+        for (auto it = live.begin(); it != live.end();)
+            it = (last.count(*it) && last[*it] <= pos) ? live.erase(it) : std::next(it);
+        preOrderTraverse(s.s[1], [&](const Symbolic& y) {
+            const unsigned nc = y.numChilds();
+            switch (y.op()) {
+                case ADD: st.adds += nc - 1; break;
+                case MUL:
+                case MULC: st.muls += nc - 1; break;
+                case DIV:
+                case RECIPROCAL: ++st.divs; break;
+                case SQRT: ++st.sqrts; break;
+                default: break;
+            }
+            return true;
+        });
+        ++st.statements;
+    }
+    st.maxLive = std::max(st.maxLive, (int)mx);
 }
 
 }  // namespace
@@ -403,27 +478,57 @@ std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const Emit
     return os.str();
 }
 
-// Fused assembly form (tile pool layout in Fused.hpp).  Phase A: the workgroup's threads
-// evaluate the tile's elements, one element per thread, into LDS.  Phase B: each wave takes
-// 64 consecutive owned outputs, finds its lanes' contribution lists by a wave prefix sum of
-// their counts, sums the LDS values and writes the result to its CSC position.  Persistent
-// schedules loop over tiles taken from an atomic counter in `work`; the last workgroup to
-// finish clears the counter, so consecutive launches (and graph replays) start from zero.
-std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches) {
+// Fused assembly form (tile pool layout in Fused.hpp).  Per tile, three phases separated by
+// barriers:
+//   A0  every thread stages elements' arguments (gathered coordinates) and parameters into
+//       LDS, and the workgroup copies the tile's recipe tables and output constants to LDS
+//       (all loads independent, so they are in flight together);
+//   A1  work items (element, output group) evaluate the element templates from LDS into the
+//       tile's LDS values (groups split the outputs so a work item keeps few values live);
+//   B   each wave takes 64 consecutive owned outputs, finds its lanes' contribution lists by
+//       one wave prefix sum of the packed (count, constant flag) bytes, sums LDS values and
+//       writes the result to its CSC position.
+// Persistent schedules loop over tiles taken from an atomic counter in `work`; the last
+// workgroup to finish clears the counter, so consecutive launches (and graph replays) start
+// from zero.
+std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches,
+                         FusedCodeStats* stats) {
+    FusedCodeStats localStats;
+    FusedCodeStats& cs = stats ? *stats : localStats;
+    cs = FusedCodeStats();
     const int T = opt.threadsPerBlock;
     if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("emitHipFused: threads per block must be a multiple of 64 up to 1024");
     const bool persistent = opt.schedule != FusedSchedule::Static;
+    const bool counter = persistent && opt.schedule != FusedSchedule::Strided;
     static const Program noProgram;
     static const ProgramKernel noKernel;
+    const size_t NT = fp.templates.size();
+    auto stamp = [&](std::ostringstream& o, int k) {
+        if (opt.profile)
+            o << "        if (threadIdx.x == 0) prof[4u * it + " << k << "u] = wall_clock64();\n";
+    };
     std::ostringstream os;
     os << "// generated by symboliclib_amd: fused assembly form (LDS-staged element tiles), FP64\n";
     os << "typedef unsigned int INDEX;\n\n";
-    // per template: outputs split into groups evaluated by separate work items (fewer live
-    // values per work item), statements of each group in a low-pressure order
-    std::vector<int> groupsOf(fp.templates.size());
-    for (size_t t = 0; t < fp.templates.size(); ++t) {
+    std::vector<int> groupsOf(NT);
+    for (size_t t = 0; t < NT; ++t) {
         const FusedTemplate& ft = fp.templates[t];
-        const std::vector<Stmt> S = flatten(ft.code);
+        const size_t nargs = ft.argNode.size();
+        // A0: stage arguments and parameters of one element
+        os << "__device__ __forceinline__ void symlib_stage" << t
+           << "(const double* __restrict__ x, unsigned B, unsigned N, const INDEX* __restrict__ nd,\n"
+           << "    const double* __restrict__ pr, unsigned cnt, unsigned i, double* __restrict__ A) {\n";
+        for (int j = 0; j < ft.numNodes; ++j) os << "    const unsigned n" << j << " = nd[" << j << "u * cnt + i];\n";
+        for (size_t g = 0; g < nargs; ++g) {
+            os << "    A[" << g << "u * cnt + i] = x[B + n" << ft.argNode[g];
+            if (ft.argDof[g]) os << " + " << ft.argDof[g] << "u * N";
+            os << "];\n";
+        }
+        for (int q = 0; q < ft.numParams; ++q)
+            os << "    A[" << nargs + (size_t)q << "u * cnt + i] = pr[" << q << "u * cnt + i];\n";
+        os << "}\n\n";
+        // A1: output groups, statements in a low-pressure order
+        const std::vector<Stmt> S = flatten(ft.code, opt.shareReciprocals);
         std::vector<size_t> outs;
         for (size_t i = 0; i < S.size(); ++i)
             if (S[i].out >= 0) outs.push_back(i);
@@ -454,26 +559,16 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                     for (int u : S[i].uses)
                         if (u >= ARG) leaves.insert(u);
             os << "__device__ __forceinline__ void symlib_elem" << t << "_" << g
-               << "(const double* __restrict__ x, unsigned B, unsigned N, const INDEX* __restrict__ nd,\n"
-               << "    const double* __restrict__ pr, unsigned cnt, unsigned i, double* __restrict__ L) {\n";
-            std::set<int> nodes;
-            for (int u : leaves)
-                if (u < PARAM) nodes.insert(ft.argNode[(size_t)(u - ARG)]);
-            for (int j : nodes) os << "    const unsigned n" << j << " = nd[" << j << "u * cnt + i];\n";
+               << "(const double* __restrict__ A, unsigned cnt, unsigned i, double* __restrict__ L) {\n";
             for (int u : leaves) {
-                if (u < PARAM) {
-                    const size_t a = (size_t)(u - ARG);
-                    os << "    const double a" << a << " = x[B + n" << ft.argNode[a];
-                    if (ft.argDof[a]) os << " + " << ft.argDof[a] << "u * N";
-                    os << "];\n";
-                } else {
-                    const int q = u - PARAM;
-                    os << "    const double q" << q << " = pr[" << q << "u * cnt + i];\n";
-                }
+                if (u < PARAM) os << "    const double a" << u - ARG << " = A[" << u - ARG << "u * cnt + i];\n";
+                else os << "    const double q" << u - PARAM << " = A[" << nargs + (size_t)(u - PARAM) << "u * cnt + i];\n";
             }
             KernelEmitter ke{noProgram, noKernel, false};
             ke.fusedT = &ft;
-            for (size_t i : lowPressureOrder(S, keep)) ke.statements(S[i].s, os, "    ");
+            const std::vector<size_t> order = lowPressureOrder(S, keep);
+            for (size_t i : order) ke.statements(S[i].s, os, "    ");
+            scheduleStats(S, order, cs);
             os << "}\n\n";
         }
     }
@@ -491,36 +586,75 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
        << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
        << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
     os << "    extern __shared__ double lds[];\n";
-    if (persistent) os << "    __shared__ unsigned s_next;\n";
+    if (counter) os << "    __shared__ unsigned s_next;\n";
+    if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 2);\n";
     os << "    const unsigned numTiles = tiles[0];\n";
     os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
     os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
+    stamp(os, 0);
     os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
-    os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3];\n";
-    os << "        {\n            unsigned Lb = 0;\n";
-    for (size_t t = 0; t < fp.templates.size(); ++t) {
-        const size_t h = 10 + 3 * t;
+    os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10], nconst = H[11];\n";
+    // LDS carve-up from the element counts
+    std::string vsum = "0u", ssum = "0u";
+    for (size_t t = 0; t < NT; ++t) {
+        const size_t h = 12 + 3 * t;
+        const FusedTemplate& ft = fp.templates[t];
+        os << "        const unsigned cnt" << t << " = H[" << h << "];\n";
+        vsum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.numOutputs) + "u";
+        ssum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.argNode.size() + (size_t)ft.numParams) + "u";
+    }
+    os << "        const unsigned nvals = " << vsum << ", nstaged = " << ssum << ";\n";
+    os << "        double* __restrict__ Ast = lds + nvals;\n";
+    os << "        INDEX* __restrict__ R = (INDEX*)(Ast + nstaged);\n";
+    os << "        double* __restrict__ K = (double*)(R + ((words + 1u) & ~1u));\n";
+    // A0
+    os << "        {\n            unsigned Sb = 0;\n";
+    for (size_t t = 0; t < NT; ++t) {
+        const size_t h = 12 + 3 * t;
+        const FusedTemplate& ft = fp.templates[t];
         os << "            {\n"
-           << "                const unsigned cnt = H[" << h << "];\n"
            << "                const INDEX* __restrict__ nd = H + H[" << h + 1 << "];\n"
            << "                const double* __restrict__ pr = c + H[" << h + 2 << "];\n"
+           << "                for (unsigned i = threadIdx.x; i < cnt" << t << "; i += " << T << "u) symlib_stage" << t
+           << "(x, B, N, nd, pr, cnt" << t << ", i, Ast + Sb);\n"
+           << "                Sb += cnt" << t << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n"
+           << "            }\n";
+    }
+    os << "            const INDEX* __restrict__ src = H + H[4];\n"
+       << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
+       << "            if (nconst) {\n"
+       << "                const double* __restrict__ cs = c + H[8];\n"
+       << "                for (unsigned q = threadIdx.x; q < nconst; q += " << T << "u) K[q] = cs[q];\n"
+       << "            }\n"
+       << "        }\n";
+    os << "        __syncthreads();\n";
+    stamp(os, 1);
+    // A1
+    os << "        {\n            unsigned Lb = 0, Sb = 0;\n";
+    for (size_t t = 0; t < NT; ++t) {
+        const FusedTemplate& ft = fp.templates[t];
+        os << "            {\n"
+           << "                const unsigned cnt = cnt" << t << ";\n"
            << "                for (unsigned w = threadIdx.x; w < " << groupsOf[t] << "u * cnt; w += " << T << "u) {\n"
            << "                    const unsigned g = w / cnt, i = w - g * cnt;\n";
         for (int g = 0; g < groupsOf[t]; ++g)
             os << "                    " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
-               << "symlib_elem" << t << "_" << g << "(x, B, N, nd, pr, cnt, i, lds + Lb);\n";
+               << "symlib_elem" << t << "_" << g << "(Ast + Sb, cnt, i, lds + Lb);\n";
         os << "                }\n"
-           << "                Lb += cnt * " << fp.templates[t].numOutputs << "u;\n"
+           << "                Lb += cnt * " << ft.numOutputs << "u;\n"
+           << "                Sb += cnt * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n"
            << "            }\n";
     }
     os << "        }\n";
     os << "        __syncthreads();\n";
+    stamp(os, 2);
+    // B
     os << "        {\n"
-       << "            const INDEX* __restrict__ R = H + H[4];\n"
-       << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(H + H[5]);\n"
-       << "            const unsigned short* __restrict__ S = (const unsigned short*)(H + H[6]);\n"
-       << "            const INDEX* __restrict__ CH = H + H[7];\n"
-       << "            const unsigned cOff = H[8];\n"
+       << "            const unsigned r0 = H[4];\n"
+       << "            const INDEX* __restrict__ Rn = R;\n"
+       << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (H[5] - r0));\n"
+       << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (H[6] - r0));\n"
+       << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
        << "            double* __restrict__ res = out + H[9];\n"
        << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
        << "                const unsigned oi = ch * 64u + lane;\n"
@@ -532,23 +666,26 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
        << "                double s = 0.0;\n"
        << "                for (unsigned q = 0; q < k; ++q) s += lds[S[base + q]];\n"
        << "                if (act) {\n"
-       << "                    if (hc) s += c[cOff + CH[3u * ch + 2u] + (ex >> 16)];\n"
+       << "                    if (hc) s += K[CH[3u * ch + 2u] + (ex >> 16)];\n"
        << "                    unsigned r = CH[3u * ch + 1u];\n"
-       << "                    while (R[2u * r + 3u] <= oi) ++r;\n"
-       << "                    res[R[2u * r] + (oi - R[2u * r + 1u])] = s;\n"
+       << "                    while (Rn[2u * r + 3u] <= oi) ++r;\n"
+       << "                    res[Rn[2u * r] + (oi - Rn[2u * r + 1u])] = s;\n"
        << "                }\n"
        << "            }\n"
        << "        }\n";
-    if (persistent) {
-        os << "        __syncthreads();\n"
-           << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work, 1u);\n"
+    os << "        __syncthreads();\n";
+    stamp(os, 3);
+    if (counter) {
+        os << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work, 1u);\n"
            << "        __syncthreads();\n"
            << "        it = s_next;\n";
+    } else if (persistent) {
+        os << "        it += gridDim.x;\n";
     } else {
         os << "        break;\n";
     }
     os << "    }\n";
-    if (persistent)
+    if (counter)
         os << "    if (threadIdx.x == 0) {\n"
            << "        __threadfence();\n"
            << "        if (atomicAdd(work + 1, 1u) == gridDim.x - 1u) {\n"
@@ -558,8 +695,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
            << "    }\n";
     os << "}\n";
     launches.clear();
-    const uint32_t grid = persistent ? LaunchDesc::GRID_FILL : (uint32_t)fp.numTiles;
     if (!persistent && fp.numTiles > 0xffffffffull) throw std::length_error("emitHipFused: grid too large");
+    const uint32_t grid = persistent ? LaunchDesc::GRID_FILL : (uint32_t)fp.numTiles;
     launches.push_back({"symlib_fused", grid, (uint32_t)T, (uint32_t)fp.ldsBytes});
     return os.str();
 }

@@ -16,11 +16,12 @@ from . import _native
 from .workloads import to_colmajor
 
 
-SCHEDULES = {"static": 0, "queue": 1, "largest": 2, "interleaved": 3}
+SCHEDULES = {"static": 0, "queue": 1, "largest": 2, "interleaved": 3, "strided": 4}
 
 
 def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True, tiled=False,
-          tile_lds_bytes=0, direct_assembly=False, fused=False, schedule="queue", output_groups=0, waves_per_eu=0):
+          tile_lds_bytes=0, direct_assembly=False, fused=False, schedule="queue", output_groups=0, waves_per_eu=0,
+          profile_tiles=False):
     o = _native.Options()
     _native.lib().symlib_default_options(ctypes.byref(o))
     o.threads_per_block = int(threads_per_block)
@@ -34,6 +35,7 @@ def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, shar
     o.schedule = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     o.output_groups = int(output_groups)
     o.waves_per_eu = int(waves_per_eu)
+    o.profile_tiles = int(bool(profile_tiles))
     return o
 
 
@@ -154,6 +156,44 @@ class Unit:
     def num_launches(self):
         rt = self.runtime()
         return _native.lib().symhip_num_launches(rt) if rt else 0
+
+    # ---- fused form: tuning and profiling
+    def retune(self, **opt):
+        """New output groups / waves per EU / block size / schedule / profiling, same tiles."""
+        o = _opts(**opt)
+        _native.check(_native.lib().symlib_unit_retune(self._h, ctypes.byref(o)), "symlib_unit_retune")
+        return self
+
+    def fused_source(self, **opt):
+        o = _opts(**opt)
+        src = _native.lib().symlib_unit_fused_source(self._h, ctypes.byref(o))
+        if src is None:
+            _native.check(1, "symlib_unit_fused_source")
+        return src.decode()
+
+    def fused_stats(self):
+        st = _native.FusedStats()
+        _native.check(_native.lib().symlib_unit_fused_stats(self._h, ctypes.byref(st)), "symlib_unit_fused_stats")
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def tile_stats(self):
+        n = int(self.fused_stats()["num_tiles"])
+        out = np.empty((n, 8), np.float32)
+        _native.check(_native.lib().symlib_unit_tile_stats(self._h, out.ctypes.data, out.size), "symlib_unit_tile_stats")
+        return out
+
+    def tile_profile(self):
+        n = int(self.fused_stats()["num_tiles"])
+        out = np.empty((n, 4), np.uint64)
+        _native.check(_native.lib().symlib_unit_tile_profile(self._h, out.ctypes.data, out.size),
+                      "symlib_unit_tile_profile")
+        return out
+
+    def launch_info(self, idx=0):
+        v, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _native.check(_native.lib().symhip_launch_info(self.runtime(), idx, ctypes.byref(v), ctypes.byref(b),
+                                                       ctypes.byref(c)), "symhip_launch_info")
+        return {"vgprs": v.value, "blocks_per_cu": b.value, "cus": c.value}
 
     def launch_grid(self, idx=0):
         rt = self.runtime()

@@ -157,3 +157,21 @@ def c4_sides(count=C4_COUNT, lo=C4_MIN_SIDE, hi=C4_MAX_SIDE, seed=C4_SEED):
 def c4_cloths(count=C4_COUNT, lo=C4_MIN_SIDE, hi=C4_MAX_SIDE, seed=C4_SEED):
     """Batched cloths: one (rest, F, X) per instance, each with its own deformation seed."""
     return [cloth(s, seed * 100003 + k) for k, s in enumerate(c4_sides(count, lo, hi, seed))]
+
+
+def interior_edge_count(F):
+    """Edges shared by exactly two faces (the bending stencils of a cloth)."""
+    F = np.asarray(F, np.int64)
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    e.sort(axis=1)
+    _, counts = np.unique(e[:, 0] * (int(F.max()) + 1) + e[:, 1], return_counts=True)
+    return int((counts == 2).sum())
+
+
+def bw_algorithmic_counts(num_vertices, F):
+    """Inputs, per-element constants and connectivity indices of one cloth Hessian evaluation:
+    3 coordinates per vertex; 5 rest constants per face (area, 2x2 inverse) and 10 bending
+    constants per interior edge; 3 vertex indices per face and 4 per interior edge."""
+    ne = interior_edge_count(F)
+    return {"n_in": 3 * num_vertices, "n_const": 5 * len(F) + 10 * ne, "n_conn": 3 * len(F) + 4 * ne,
+            "interior_edges": ne}

@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol():
     L = _native.lib()
     declared = []
-    for h in ("symhip.h", "symlib.h"):
+    for h in ("symhip.h", "symlib.h", "symcost.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         declared += re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(sym\w+)\s*\(", text, flags=re.M)
@@ -179,7 +179,8 @@ def test_fused_pattern_and_tiles(side, lds):
         assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
         f = _fields(info)
         assert int(f["tiles"]) >= 1
-        assert int(f["lds"]) <= (lds or 32768)
+        assert int(f["element_lds"]) <= (lds or 32768)
+        assert int(f["element_lds"]) < int(f["lds"]) <= 160 * 1024
         # every face evaluated at least once; compact patches keep re-evaluation bounded
         assert 1.0 <= float(f["recompute"]) < 2.5
         assert info["num_results"] == len(ri)

@@ -9,7 +9,6 @@ absolute value of every factor and of every additive term (products A(k,i) A(k,j
 per-face Hessian entry from the edge vectors on, plus |k_bend Q| per bending constant).  The
 maximum and median errors are printed for every case.
 """
-import ctypes
 import os
 
 import numpy as np
@@ -186,20 +185,17 @@ def test_bw_hessian_batch_fused(require_device, schedule):
 
 
 def test_fused_graph_replays_match_eager(require_device):
-    import torch
-
     sides = [12, 30, 7]
     cloths = [W.cloth(s, seed=300 + s) for s in sides]
     with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True) as u:
         u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
         eager = [u.get_results(k) for k in range(len(sides))]
-        stream = torch.cuda.current_stream()
+        for k, (rest, F, X) in enumerate(cloths):
+            _check(f"graph test eager cloth {k}", u, O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND), eager[k], k)
         for rep in range(4):
-            u.run_graph(ctypes.c_void_p(stream.cuda_stream))
-            torch.cuda.synchronize()
+            u.run_graph()  # null stream; get_results synchronises the device
             for k in range(len(sides)):
                 assert np.array_equal(u.get_results(k), eager[k]), f"graph replay {rep} differs on cloth {k}"
             u.run()  # eager launch between replays
-            torch.cuda.synchronize()
             for k in range(len(sides)):
                 assert np.array_equal(u.get_results(k), eager[k]), f"eager launch after replay {rep} differs"
