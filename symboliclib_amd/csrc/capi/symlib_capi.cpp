@@ -116,6 +116,25 @@ void finishAssembly(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0
     u->patterns = u->unit->outputPatterns();
 }
 
+void finishProducts(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, const Sym::ProductSum& ps) {
+    const Clock::time_point t1 = Clock::now();
+    u->unit.reset(new Sym::ComputeUnit<double>(dev, ps));
+    const Clock::time_point t2 = Clock::now();
+    const auto& F = u->unit->getFusedProgram();
+    u->info.num_inputs = F.numInputs;
+    u->info.num_results = F.numResults;
+    u->info.num_const = 0;
+    u->info.num_kernels = 0;
+    u->info.num_waves = 1;
+    u->info.tiled = 2;
+    u->info.tile_pool_entries = F.pool.size();
+    u->info.source_bytes = u->unit->getCode().size();
+    std::snprintf(u->info.tiling, sizeof(u->info.tiling), "%s", u->unit->tilingInfo().c_str());
+    u->info.frontend_seconds = seconds(t0, t1);
+    u->info.build_seconds = seconds(t1, t2);
+    u->patterns = u->unit->outputPatterns();
+}
+
 Sym::workloads::Faces toFaces(int nf, const int* faces) {
     Sym::workloads::Faces F((size_t)nf);
     for (int i = 0; i < nf; ++i) F[i] = {faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]};
@@ -165,11 +184,16 @@ int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const i
             if (colptr[j + 1] < colptr[j]) throw std::invalid_argument("column pointers must not decrease");
         std::unique_ptr<symlib_unit> u(new symlib_unit());
         const auto t0 = Clock::now();
-        Sym::SparseMatrix<double> A(n, n, colptr, rowidx, nullptr);
         for (int j = 0; j < n; ++j)
             for (int p = colptr[j]; p < colptr[j + 1]; ++p)
                 if (rowidx[p] < 0 || rowidx[p] >= n || (p > colptr[j] && rowidx[p] <= rowidx[p - 1]))
                     throw std::invalid_argument("row indices must be in range, sorted and unique per column");
+        if (opt && opt->fused) {
+            finishProducts(u.get(), makeDevice(opt), t0, Sym::ProductSum{n, colptr, rowidx});
+            *out = u.release();
+            return;
+        }
+        Sym::SparseMatrix<double> A(n, n, colptr, rowidx, nullptr);
         auto AS = Sym::makeSymbolic(A, 0);
         auto BS = Sym::workloads::ataPlusA(AS);
         const auto t1 = Clock::now();

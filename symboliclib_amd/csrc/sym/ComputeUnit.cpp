@@ -106,6 +106,37 @@ void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
     finalize();
 }
 
+void ComputeUnitBase::initFromProducts(const ProductSum& ps) {
+    FusedOptions fo;
+    fo.ldsBytes = device.fusedLdsBytes;
+    fo.threadsPerBlock = device.threadsPerBlock();
+    if (device.schedule < 0 || device.schedule > 4) throw std::invalid_argument("ComputeUnit: unknown schedule");
+    fo.schedule = (FusedSchedule)device.schedule;
+    fo.wavesPerEU = device.wavesPerEU;
+    fo.profile = device.profileTiles;
+    std::vector<int> outer, inner;
+    fusedProgram = buildProductTiles(ps.n, ps.colptr, ps.rowidx, fo, outer, inner);
+    fusedOptions = fo;
+    code = emitHipFused(fusedProgram, fo, launches, &fusedStats);
+    fusedForm = true;
+    tiledForm = false;
+    tilingNote = fusedProgram.summary();
+    io = ProgramInputs();
+    io.inputSizes.push_back((size_t)ps.colptr[ps.n]);
+    io.inputOffsets[0] = 0;
+    io.inputLengths[0] = (size_t)ps.colptr[ps.n];
+    io.inputDataSize = (size_t)ps.colptr[ps.n];
+    io.outputSizes.push_back(fusedProgram.numResults);
+    program = Program();
+    program.numInputs = fusedProgram.numInputs;
+    program.outStart.push_back(0);
+    program.outLength.push_back(fusedProgram.numResults);
+    SparseMatrix<double> pat(ps.n, ps.n);
+    pat.setPattern(outer, inner);
+    assemblyPatterns.clear();
+    assemblyPatterns.push_back(std::move(pat));
+}
+
 void ComputeUnitBase::finalize() {
     EmitOptions eo;
     eo.threadsPerBlock = device.threadsPerBlock();

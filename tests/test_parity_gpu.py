@@ -199,3 +199,35 @@ def test_fused_graph_replays_match_eager(require_device):
             u.run()  # eager launch between replays
             for k in range(len(sides)):
                 assert np.array_equal(u.get_results(k), eager[k]), f"eager launch after replay {rep} differs"
+
+
+@pytest.mark.parametrize("subdiv,schedule,lds", [(1, "static", 0), (5, "queue", 0), (5, "strided", 65536)])
+def test_ata_plus_a_product_tiles(require_device, subdiv, schedule, lds):
+    n, cp, ri, a = W.c1_matrix(subdiv=subdiv)
+    with Unit.ata_plus_a(cp, ri, fused=True, schedule=schedule, tile_lds_bytes=lds) as u:
+        assert u.info()["tiled"] == 2
+        u.compile().execute(a)
+        _check(f"A^T A + A icosphere({subdiv}) product tiles {schedule}", u, O.ata_plus_a(cp, ri, a), u.get_results())
+        a2 = W.random_values(len(ri), seed=77)
+        u.execute(a2)
+        _check(f"A^T A + A icosphere({subdiv}) product tiles second values", u, O.ata_plus_a(cp, ri, a2),
+               u.get_results())
+
+
+def test_sphere_mtx_golden_product_tiles(require_device):
+    n, cp, ri, a = W.load_mtx(os.path.join(ROOT, "tests", "golden", "sphere.mtx"))
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sphere_ata_plus_a.npz"))
+    with Unit.ata_plus_a(cp, ri, fused=True) as u:
+        u.compile().execute(a)
+        _, _, _, sc = O.ata_plus_a(cp, ri, a)
+        _check("sphere.mtx A^T A + A product tiles", u, (g["indptr"], g["indices"], g["values"], sc), u.get_results())
+
+
+@pytest.mark.parametrize("rank,lds", [(0, 32768), (20, 65536)])
+def test_synthetic_pattern_product_tiles(require_device, rank, lds):
+    # power-law rows, band plus random fill: the cost-model patterns (smallest sizes)
+    idx = int(np.argsort(W.c5_sizes())[rank])
+    n, cp, ri, a = W.c5_pattern(idx)
+    with Unit.ata_plus_a(cp, ri, fused=True, tile_lds_bytes=lds, schedule="queue") as u:
+        u.compile().execute(a)
+        _check(f"synthetic pattern {idx} (n={n}) product tiles", u, O.ata_plus_a(cp, ri, a), u.get_results())
