@@ -248,7 +248,11 @@ def write_db(path, program, records):
 
 
 def read_db(path):
-    with open(path) as f:
+    """Timing database: JSON lines, optionally gzip-compressed (.gz)."""
+    import gzip
+
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "rt") as f:
         return [json.loads(line) for line in f if line.strip()]
 
 
