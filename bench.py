@@ -50,9 +50,9 @@ def fused_options(args, tuned=None):
     o = {"fused": True}
     if tuned:
         o.update(tile_lds_bytes=tuned["lds"], output_groups=tuned["groups"], waves_per_eu=tuned["wpe"],
-                 threads_per_block=tuned["tpb"], schedule=tuned["schedule"])
+                 threads_per_block=tuned["tpb"], schedule=tuned["schedule"], split=bool(tuned.get("split", 0)))
     for k, v in (("tile_lds_bytes", args.tile_lds), ("output_groups", args.groups), ("waves_per_eu", args.wpe),
-                 ("threads_per_block", args.tpb), ("schedule", args.schedule)):
+                 ("threads_per_block", args.tpb), ("schedule", args.schedule), ("split", args.split)):
         if v is not None:
             o[k] = v
     return o
@@ -281,7 +281,11 @@ def run_batched(args, dev, stream, sp):
     torch.cuda.synchronize()
     ms_per_step = (time.perf_counter() - ts) * 1e3 / args.steps
     log(f"[bench] batched: {args.steps} steps, {ms_per_step:.4f} ms/step")
-    kernel_ms = time_events(lambda: batched.launch_one(0, sp), stream, max(5, min(args.steps, 50)))
+    # each launch of the step on its own (the split form has two: elements, then sums)
+    names = ["symlib_fused_elem", "symlib_fused_sum"] if opt.get("split") else ["symlib_fused"]
+    launch_ms = [time_events(lambda i=i: batched.launch_one(i, sp), stream, max(5, min(args.steps, 50)))
+                 for i in range(batched.num_launches())]
+    kernel_ms = sum(launch_ms)
     balg = sum(b_alg(c, 0) for c in counts) + 12 * n_out
 
     # plain single-instance form, one program per distinct side, launched once per instance
@@ -340,12 +344,14 @@ def run_batched(args, dev, stream, sp):
                                f"32..256 (seed 4), stretch+shear AD + bend",
                    "instances": len(sides), "vertices": int(sum(len(r) for r, _, _ in cloths)),
                    "faces": int(sum(len(F) for _, F, _ in cloths)), "output_nnz": n_out,
-                   "form": "fused assembly, one persistent launch replayed from a HIP graph",
+                   "form": ("fused assembly in two launches (element images, then sums)" if opt.get("split")
+                            else "fused assembly, one launch") + ", replayed from a HIP graph",
                    "options": opt, "tuned_from": os.path.relpath(TUNED, ROOT) if tuned else None,
                    "tiling": info["tiling"], "grid": batched.launch_grid(0), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "symlib_fused",
-                     "algorithmic_bytes": balg, "kernel_ms": kernel_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "+".join(names),
+                     "algorithmic_bytes": balg, "kernel_ms": kernel_ms,
+                     "launch_ms": dict(zip(names, launch_ms))},
         "plain_per_instance": plain_line,
         "speedup_vs_plain": (value / plain_line["value"]) if plain_line else None,
         "copy_bandwidth_gbs": copy_gbs,
@@ -389,7 +395,9 @@ def main():
     ap.add_argument("--tpb", type=int, default=None, help="threads per block")
     ap.add_argument("--groups", type=int, default=None, help="fused: work items per element")
     ap.add_argument("--wpe", type=int, default=None, help="fused: minimum waves per SIMD")
-    ap.add_argument("--schedule", default=None, choices=["static", "queue", "largest", "interleaved"])
+    ap.add_argument("--schedule", default=None, choices=["static", "queue", "largest", "interleaved", "strided"])
+    ap.add_argument("--split", type=int, default=None, choices=[0, 1],
+                    help="fused: 1 = element images in HBM, then sums (two launches)")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -139,6 +139,8 @@ uint32_t symhip_launch_grid(symhip_t h, uint32_t idx);
  * the work words (zeroed), read work words back (after synchronising). */
 int symhip_update_tiles(symhip_t h, uint64_t offset, uint64_t n, const uint32_t* data);
 int symhip_set_launch(symhip_t h, uint32_t idx, const symhip_launch* launch);
+/* Replace the whole launch list (its length may change); takes effect at the next compile. */
+int symhip_set_launches(symhip_t h, uint32_t n, const symhip_launch* launches);
 int symhip_set_work_words(symhip_t h, uint64_t n);
 int symhip_read_work(symhip_t h, uint64_t offset, uint64_t n, uint32_t* out);
 /* Registers and residency of launch `idx` after compile: VGPRs per work-item, workgroups per

@@ -48,6 +48,7 @@ typedef struct {
     int output_groups;           /* fused form: work items per element, 0 = default (5) */
     int waves_per_eu;            /* fused form: minimum waves per SIMD to compile for, 0 = none */
     int profile_tiles;           /* fused form: record wall-clock stamps per tile (symlib_unit_tile_profile) */
+    int split;                   /* fused cloth form: 1 = two launches (element images in HBM, then sums) */
 } symlib_options;
 
 /* Fused-form program and code statistics (the autotuner's features). */

@@ -19,7 +19,7 @@ class Options(ctypes.Structure):
                 ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int), ("tiled", ctypes.c_int),
                 ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int), ("fused", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("output_groups", ctypes.c_int), ("waves_per_eu", ctypes.c_int),
-                ("profile_tiles", ctypes.c_int)]
+                ("profile_tiles", ctypes.c_int), ("split", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):
@@ -69,6 +69,7 @@ SIGNATURES = {
     "symhip_cache_compile": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
     "symhip_update_tiles": (_I, [_VP, _U64, _U64, _VP]),
     "symhip_set_launch": (_I, [_VP, _U32, ctypes.POINTER(Launch)]),
+    "symhip_set_launches": (_I, [_VP, _U32, ctypes.POINTER(Launch)]),
     "symhip_set_work_words": (_I, [_VP, _U64]),
     "symhip_read_work": (_I, [_VP, _U64, _U64, _VP]),
     "symhip_launch_info": (_I, [_VP, _U32, _IP, _IP, _IP]),

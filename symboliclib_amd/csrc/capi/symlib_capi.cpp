@@ -49,6 +49,7 @@ Sym::Device makeDevice(const symlib_options* opt) {
     if (o.output_groups > 0) d.outputGroups = o.output_groups;
     d.wavesPerEU = o.waves_per_eu;
     d.profileTiles = o.profile_tiles != 0;
+    d.fusedSplit = o.split != 0;
     return d;
 }
 
@@ -173,6 +174,7 @@ void symlib_default_options(symlib_options* opt) {
     opt->output_groups = 0;
     opt->waves_per_eu = 0;
     opt->profile_tiles = 0;
+    opt->split = 0;
 }
 
 int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {
@@ -310,6 +312,7 @@ Sym::FusedOptions fusedOptionsFrom(const symlib_unit* u, const symlib_options* o
     if (opt->schedule < 0 || opt->schedule > 4) throw std::invalid_argument("unknown schedule");
     fo.schedule = (Sym::FusedSchedule)opt->schedule;
     fo.profile = opt->profile_tiles != 0;
+    fo.split = opt->split != 0 && u->unit->getFusedProgram().kind == 0;
     return fo;
 }
 

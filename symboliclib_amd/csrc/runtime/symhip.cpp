@@ -422,6 +422,27 @@ int symhip_set_launch(symhip_t h, uint32_t idx, const symhip_launch* launch) {
     return SYMHIP_SUCCESS;
 }
 
+int symhip_set_launches(symhip_t h, uint32_t n, const symhip_launch* launches) {
+    if (!h || (n && !launches)) return fail(SYMHIP_E_INVALID, "symhip_set_launches: null argument");
+    h->names.clear();
+    h->launches.clear();
+    for (uint32_t i = 0; i < n; ++i) {
+        h->names.push_back(launches[i].name ? launches[i].name : "");
+        h->launches.push_back(launches[i]);
+    }
+    h->functions.clear();  // compile again before running
+    h->grids.clear();
+    if (h->graphExec) {
+        (void)hipGraphExecDestroy(h->graphExec);
+        h->graphExec = nullptr;
+    }
+    if (h->graph) {
+        (void)hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return SYMHIP_SUCCESS;
+}
+
 int symhip_set_work_words(symhip_t h, uint64_t n) {
     if (!h) return fail(SYMHIP_E_INVALID, "symhip_set_work_words: null handle");
     if (n == h->numWork) return SYMHIP_SUCCESS;

@@ -81,13 +81,14 @@ void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
         FusedOptions fo;
         fo.ldsBytes = device.fusedLdsBytes;
         fo.threadsPerBlock = device.threadsPerBlock();
-        if (device.schedule < 0 || device.schedule > 3) throw std::invalid_argument("ComputeUnit: unknown schedule");
+        if (device.schedule < 0 || device.schedule > 4) throw std::invalid_argument("ComputeUnit: unknown schedule");
         fo.schedule = (FusedSchedule)device.schedule;
         if (device.outputGroups < 1 || device.outputGroups > 64) throw std::invalid_argument("ComputeUnit: output groups must be 1..64");
         if (device.wavesPerEU < 0 || device.wavesPerEU > 8) throw std::invalid_argument("ComputeUnit: waves per EU must be 0..8");
         fo.outputGroups = device.outputGroups;
         fo.wavesPerEU = device.wavesPerEU;
         fo.profile = device.profileTiles;
+        fo.split = device.fusedSplit;
         fusedProgram = assembly.buildFused(opt, fo, io, assemblyPatterns);
         fusedOptions = fo;
         code = emitHipFused(fusedProgram, fo, launches, &fusedStats);
@@ -197,7 +198,9 @@ void ComputeUnitBase::compileProgram() {
         d.num_index = 0;
     }
     if (fusedForm) {
-        d.num_values = fusedProgram.numInputs;
+        // element images of the split form live after the inputs (allocated for both forms so a
+        // retune can switch between them)
+        d.num_values = fusedProgram.numInputs + fusedProgram.imageDoubles;
         d.num_index = 0;
         d.num_outpos = 0;
         d.num_const = fusedProgram.consts.size();
@@ -205,7 +208,7 @@ void ComputeUnitBase::compileProgram() {
         d.num_tiles_data = fusedProgram.pool.size();
         d.tiles_data = fusedProgram.pool.data();
     }
-    d.num_work_words = fusedForm && fusedOptions.profile ? tileProfileWords(fusedProgram.numTiles) : 2;
+    d.num_work_words = fusedForm && fusedOptions.profile ? tileProfileWords(fusedProgram.numTiles) : WORK_COUNTER_WORDS;
     d.num_launches = (uint32_t)ls.size();
     d.launches = ls.data();
 
@@ -242,9 +245,11 @@ void ComputeUnitBase::retuneFused(const FusedOptions& fo) {
     orderTiles(fusedProgram, fo.schedule);
     if (rt) {
         check(symhip_update_tiles(rt, 1, fusedProgram.numTiles, fusedProgram.pool.data() + 1), "symhip_update_tiles");
-        symhip_launch L{ls[0].name.c_str(), ls[0].grid, ls[0].block, ls[0].shared};
-        check(symhip_set_launch(rt, 0, &L), "symhip_set_launch");
-        check(symhip_set_work_words(rt, fo.profile ? tileProfileWords(fusedProgram.numTiles) : 2), "symhip_set_work_words");
+        std::vector<symhip_launch> L;
+        for (auto& l : ls) L.push_back({l.name.c_str(), l.grid, l.block, l.shared});
+        check(symhip_set_launches(rt, (uint32_t)L.size(), L.data()), "symhip_set_launches");
+        check(symhip_set_work_words(rt, fo.profile ? tileProfileWords(fusedProgram.numTiles) : WORK_COUNTER_WORDS),
+              "symhip_set_work_words");
         check(symhip_compile(rt, src.c_str(), nullptr), "symhip_compile");
     }
     code = std::move(src);
@@ -262,7 +267,7 @@ std::string ComputeUnitBase::fusedSource(const FusedOptions& fo) const {
 std::vector<uint64_t> ComputeUnitBase::tileProfile() const {
     if (!rt || !fusedForm || !fusedOptions.profile) throw std::runtime_error("ComputeUnit: no profiled fused launch");
     std::vector<uint64_t> stamps(4 * fusedProgram.numTiles);
-    check(symhip_read_work(rt, 2, 8 * fusedProgram.numTiles, reinterpret_cast<uint32_t*>(stamps.data())),
+    check(symhip_read_work(rt, WORK_COUNTER_WORDS, 8 * fusedProgram.numTiles, reinterpret_cast<uint32_t*>(stamps.data())),
           "symhip_read_work");
     return stamps;
 }

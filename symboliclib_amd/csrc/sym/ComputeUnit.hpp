@@ -61,6 +61,8 @@ using Schedule = NamedType<int, struct ScheduleTag>;
 // waves per SIMD the kernel is compiled for (0 = compiler's choice).
 using OutputGroups = NamedType<int, struct OutputGroupsTag>;
 using WavesPerEU = NamedType<int, struct WavesPerEUTag>;
+// Fused form in two launches: element values of every tile to HBM, then the sums.
+struct UseSplit {};
 
 class Device {
 public:
@@ -79,6 +81,7 @@ public:
     int outputGroups = 5;
     int wavesPerEU = 0;
     bool profileTiles = false;
+    bool fusedSplit = false;
 
 private:
     void set(const DecompositionThreshold& nt) { decompositionThreshold = (unsigned)nt.x; }
@@ -97,6 +100,7 @@ private:
     void set(const Schedule& s) { schedule = s.x; }
     void set(const OutputGroups& g) { outputGroups = g.x; }
     void set(const WavesPerEU& w) { wavesPerEU = w.x; }
+    void set(const UseSplit&) { fusedSplit = true; }
     void set(const UseHIP&) {
         hipDevice = true;
         cudaDevice = false;

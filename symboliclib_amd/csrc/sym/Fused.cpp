@@ -20,7 +20,7 @@ namespace Sym {
 namespace {
 
 constexpr uint32_t NONE = 0xffffffffu;
-constexpr int HEADER = 12;  // fixed header words before the per-template triples
+constexpr int HEADER = FUSED_HEADER;  // fixed header words before the per-template triples
 
 uint32_t u32(size_t v, const char* what) {
     if (v > std::numeric_limits<uint32_t>::max()) throw std::length_error(std::string("fused form: ") + what + " exceeds 32 bits");
@@ -46,7 +46,8 @@ struct InstanceTiles {
     std::vector<double> consts;
     std::vector<size_t> header;  // tile header offsets in pool
     std::vector<double> cost;    // per tile: outputs + element values
-    size_t evaluations = 0, contributions = 0, lds = 0, elementLds = 0, outputs = 0;
+    std::vector<size_t> values;  // per tile: element values (its image in the split form)
+    size_t evaluations = 0, contributions = 0, lds = 0, elementLds = 0, stagedLds = 0, outputs = 0;
 };
 
 }  // namespace
@@ -368,6 +369,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             pool[H + 9] = 0;  // result base, set at merge
             pool[H + 10] = chunksOff + 3 * (uint32_t)nch - runsOff;
             pool[H + 11] = (uint32_t)cvals.size() * (anyConst ? 1u : 0u);
+            pool[H + 12] = 0;  // element image, set at merge
             size_t evals = 0;
             for (size_t t = 0; t < NT; ++t) {
                 const auto& te = tElems[t];
@@ -393,6 +395,8 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             for (size_t t = 0; t < NT; ++t) staged += tElems[t].size() * (size_t)(L[t].gatherNode.size() + L[t].numParams);
             const size_t tileLds = 8 * (ldsBase[NT] + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * (size_t)pool[H + 11];
             out.cost.push_back((double)m + (double)ldsBase[NT]);
+            out.values.push_back(ldsBase[NT]);
+            out.stagedLds = std::max(out.stagedLds, 8 * staged);
             out.evaluations += evals;
             out.contributions += slotList.size();
             out.lds = std::max(out.lds, tileLds);
@@ -406,12 +410,17 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
 
     // ---- merge instances (offsets first, then copies in parallel); order the tile list
     const size_t K = inst_.size();
-    std::vector<size_t> poolBase(K), constBase(K), resBase(K), tileBase(K);
-    size_t nTiles = 0, pb = 0, cb = 0, rb = 0;
+    std::vector<size_t> poolBase(K), constBase(K), resBase(K), tileBase(K), imageBase(K);
+    size_t nTiles = 0, pb = 0, cb = 0, rb = 0, ib = fp.numInputs;
     for (size_t k = 0; k < K; ++k) {
         tileBase[k] = nTiles;
         nTiles += tiles[k].header.size();
+        imageBase[k] = ib;
+        for (size_t v : tiles[k].values) ib += v;
+        fp.stagedLdsBytes = std::max(fp.stagedLdsBytes, tiles[k].stagedLds);
     }
+    u32(ib, "element images");
+    fp.imageDoubles = ib - fp.numInputs;
     pb = 1 + nTiles;
     for (size_t k = 0; k < K; ++k) {
         if (tiles[k].outputs != pats[k].nonZeros())
@@ -442,10 +451,13 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
     parallelFor(K, [&](size_t k) {
         InstanceTiles& it = tiles[k];
         const uint32_t cbase = (uint32_t)constBase[k];
+        size_t img = imageBase[k];
         for (size_t q = 0; q < it.header.size(); ++q) {
             uint32_t* H = &it.pool[it.header[q]];
             if (H[8] != NONE) H[8] += cbase;
             H[9] = (uint32_t)resBase[k];
+            H[12] = (uint32_t)img;
+            img += it.values[q];
             for (size_t t = 0; t < NT; ++t) H[HEADER + 3 * t + 2] += cbase;
             refs[tileBase[k] + q] = {(uint32_t)(poolBase[k] + it.header[q]), (uint32_t)k, (uint32_t)q, it.cost[q]};
         }
@@ -615,6 +627,9 @@ void validateFused(const FusedProgram& fp) {
             staged += cnt * (ft.argNode.size() + (size_t)ft.numParams);
         }
         need(lds <= 65536, "tile LDS values exceed 16-bit slots");
+        need((size_t)P[H + 12] >= fp.numInputs && (size_t)P[H + 12] + lds <= fp.numInputs + fp.imageDoubles,
+             "element image outside the value buffer");
+        need(8 * staged <= fp.stagedLdsBytes, "staged values exceed the element launch's LDS");
         const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[counts]);
         size_t S = 0, numConst = 0;
         for (size_t o = 0; o < m; ++o) {

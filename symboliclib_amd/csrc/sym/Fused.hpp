@@ -28,7 +28,8 @@
 //     H[9] result base of the instance
 //     H[10] words from runs to the end of chunks (the recipe, contiguous)
 //     H[11] number of output constants
-//     H[12 + 3 t], H[13 + 3 t], H[14 + 3 t]: template t's element count, rel node list
+//     H[12] the tile's element image in the value buffer (split form: first value, in doubles)
+//     H[13 + 3 t], H[14 + 3 t], H[15 + 3 t]: template t's element count, rel node list
 //       (numNodes_t columns of count), parameter block in c[] (numParams_t columns of count)
 //   LDS of a tile, in doubles: element values (template t's outputs at V_t + slot * count_t
 //   + i, V_t = sum over t' < t of count_t' * numOutputs_t'), then the staged element
@@ -62,12 +63,17 @@ struct FusedOptions {
     int wavesPerEU = 0;               // minimum waves per SIMD asked of the compiler (0 = none)
     bool profile = false;             // record wall-clock stamps per tile (see tileProfileWords)
     bool shareReciprocals = true;     // one 1/d per repeated denominator d (x/d -> x*(1/d))
+    // split form: one launch evaluates every tile's elements into its image in HBM (no LDS for
+    // values, high occupancy), a second launch stages images and recipes in LDS and sums
+    bool split = false;
 };
 
-// Profiled launches record, per tile in launch order, four 64-bit wall-clock stamps (tile
-// start, staging done, element evaluation done, outputs written) after the two counter words
-// of `work`.
-inline size_t tileProfileWords(size_t numTiles) { return 2 + 8 * numTiles; }
+// `work` starts with two work-queue counters (next tile, finished workgroups) per launch of an
+// evaluation (the split form has two launches).  Profiled launches record after them, per tile
+// in launch order, four 64-bit wall-clock stamps (tile start, staging done, element evaluation
+// done, outputs written; in the split form the last stamp comes from the sum launch).
+constexpr size_t WORK_COUNTER_WORDS = 4;
+inline size_t tileProfileWords(size_t numTiles) { return WORK_COUNTER_WORDS + 8 * numTiles; }
 
 struct FusedTemplate {
     Symbolic code;                 // ASSIGN(VAR(s,-2), ...) over VAR(a,-3) arguments, VAR(q,-4) parameters
@@ -89,6 +95,8 @@ struct FusedProgram {
     size_t numInputs = 0, numResults = 0;
     std::vector<size_t> outStart, outLength;
     size_t numTiles = 0, ldsBytes = 0, elementLdsBytes = 0;  // per workgroup: all LDS, element part
+    size_t stagedLdsBytes = 0;  // split form, element launch: staged arguments and parameters
+    size_t imageDoubles = 0;    // split form: element images of all tiles (after the inputs)
     size_t elementEvaluations = 0, numElements = 0, numContributions = 0;
     double recompute() const { return numElements ? (double)elementEvaluations / (double)numElements : 0.0; }
     std::string summary() const;
@@ -97,6 +105,7 @@ struct FusedProgram {
 // Per tile in launch order, TILE_STATS floats: owned outputs, element contributions, elements,
 // staged doubles, element LDS values, recipe words, output constants, runs.
 constexpr int TILE_STATS = 8;
+constexpr int FUSED_HEADER = 13;  // fixed header words of an element tile
 std::vector<float> tileStats(const FusedProgram& fp);
 
 // Write the tile table pool[1..numTiles] in the order of `schedule`.

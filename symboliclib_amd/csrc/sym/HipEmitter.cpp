@@ -521,7 +521,7 @@ std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std
        << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
     os << "    extern __shared__ double lds[];\n";
     if (counter) os << "    __shared__ unsigned s_next;\n";
-    if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 2);\n";
+    if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 4);\n";
     os << "    const unsigned numTiles = tiles[0];\n";
     os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
     os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
@@ -673,125 +673,163 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
        << "        if (lane >= d) s += u;\n"
        << "    }\n"
        << "    return s - v;\n}\n\n";
-    os << "extern \"C\" __global__ void __launch_bounds__(" << T;
-    if (opt.wavesPerEU > 0) os << ", " << opt.wavesPerEU;
-    os << ") symlib_fused"
-       << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
-       << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
-       << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
-    os << "    extern __shared__ double lds[];\n";
-    if (counter) os << "    __shared__ unsigned s_next;\n";
-    if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 2);\n";
-    os << "    const unsigned numTiles = tiles[0];\n";
-    os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
-    os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
-    stamp(os, 0);
-    os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
-    os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10], nconst = H[11];\n";
-    // LDS carve-up from the element counts
-    std::string vsum = "0u", ssum = "0u";
-    for (size_t t = 0; t < NT; ++t) {
-        const size_t h = 12 + 3 * t;
-        const FusedTemplate& ft = fp.templates[t];
-        os << "        const unsigned cnt" << t << " = H[" << h << "];\n";
-        vsum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.numOutputs) + "u";
-        ssum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.argNode.size() + (size_t)ft.numParams) + "u";
-    }
-    os << "        const unsigned nvals = " << vsum << ", nstaged = " << ssum << ";\n";
-    os << "        double* __restrict__ Ast = lds + nvals;\n";
-    os << "        INDEX* __restrict__ R = (INDEX*)(Ast + nstaged);\n";
-    os << "        double* __restrict__ K = (double*)(R + ((words + 1u) & ~1u));\n";
-    // A0
-    os << "        {\n            unsigned Sb = 0;\n";
-    for (size_t t = 0; t < NT; ++t) {
-        const size_t h = 12 + 3 * t;
-        const FusedTemplate& ft = fp.templates[t];
-        os << "            {\n"
-           << "                const INDEX* __restrict__ nd = H + H[" << h + 1 << "];\n"
-           << "                const double* __restrict__ pr = c + H[" << h + 2 << "];\n"
-           << "                for (unsigned i = threadIdx.x; i < cnt" << t << "; i += " << T << "u) symlib_stage" << t
-           << "(x, B, N, nd, pr, cnt" << t << ", i, Ast + Sb);\n"
-           << "                Sb += cnt" << t << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n"
-           << "            }\n";
-    }
-    os << "            const INDEX* __restrict__ src = H + H[4];\n"
-       << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
-       << "            if (nconst) {\n"
-       << "                const double* __restrict__ cs = c + H[8];\n"
-       << "                for (unsigned q = threadIdx.x; q < nconst; q += " << T << "u) K[q] = cs[q];\n"
-       << "            }\n"
-       << "        }\n";
-    os << "        __syncthreads();\n";
-    stamp(os, 1);
-    // A1
-    os << "        {\n            unsigned Lb = 0, Sb = 0;\n";
-    for (size_t t = 0; t < NT; ++t) {
-        const FusedTemplate& ft = fp.templates[t];
-        os << "            {\n"
-           << "                const unsigned cnt = cnt" << t << ";\n"
-           << "                for (unsigned w = threadIdx.x; w < " << groupsOf[t] << "u * cnt; w += " << T << "u) {\n"
-           << "                    const unsigned g = w / cnt, i = w - g * cnt;\n";
-        for (int g = 0; g < groupsOf[t]; ++g)
-            os << "                    " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
-               << "symlib_elem" << t << "_" << g << "(Ast + Sb, cnt, i, lds + Lb);\n";
-        os << "                }\n"
-           << "                Lb += cnt * " << ft.numOutputs << "u;\n"
-           << "                Sb += cnt * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n"
-           << "            }\n";
-    }
-    os << "        }\n";
-    os << "        __syncthreads();\n";
-    stamp(os, 2);
-    // B
-    os << "        {\n"
-       << "            const unsigned r0 = H[4];\n"
-       << "            const INDEX* __restrict__ Rn = R;\n"
-       << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (H[5] - r0));\n"
-       << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (H[6] - r0));\n"
-       << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
-       << "            double* __restrict__ res = out + H[9];\n"
-       << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
-       << "                const unsigned oi = ch * 64u + lane;\n"
-       << "                const bool act = oi < m;\n"
-       << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
-       << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
-       << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
-       << "                const unsigned base = CH[3u * ch] + (ex & 0xffffu);\n"
-       << "                double s = 0.0;\n"
-       << "                for (unsigned q = 0; q < k; ++q) s += lds[S[base + q]];\n"
-       << "                if (act) {\n"
-       << "                    if (hc) s += K[CH[3u * ch + 2u] + (ex >> 16)];\n"
-       << "                    unsigned r = CH[3u * ch + 1u];\n"
-       << "                    while (Rn[2u * r + 3u] <= oi) ++r;\n"
-       << "                    res[Rn[2u * r] + (oi - Rn[2u * r + 1u])] = s;\n"
-       << "                }\n"
-       << "            }\n"
-       << "        }\n";
-    os << "        __syncthreads();\n";
-    stamp(os, 3);
-    if (counter) {
-        os << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work, 1u);\n"
-           << "        __syncthreads();\n"
-           << "        it = s_next;\n";
-    } else if (persistent) {
-        os << "        it += gridDim.x;\n";
-    } else {
-        os << "        break;\n";
-    }
-    os << "    }\n";
-    if (counter)
-        os << "    if (threadIdx.x == 0) {\n"
-           << "        __threadfence();\n"
-           << "        if (atomicAdd(work + 1, 1u) == gridDim.x - 1u) {\n"
-           << "            atomicExch(work, 0u);\n"
-           << "            atomicExch(work + 1, 0u);\n"
-           << "        }\n"
-           << "    }\n";
-    os << "}\n";
+    // Kernel variants: 0 fused (stage, evaluate into LDS, sum), 1 split elements (stage, evaluate
+    // into the tile's image in HBM), 2 split sums (copy image, recipe and constants to LDS, sum).
+    auto kernel = [&](const char* name, int variant) {
+        const bool stage = variant != 2, eval = variant != 2, sums = variant != 1;
+        const unsigned ctr = variant == 2 ? 2u : 0u;  // work-queue counter words
+        os << "extern \"C\" __global__ void __launch_bounds__(" << T;
+        if (opt.wavesPerEU > 0) os << ", " << opt.wavesPerEU;
+        os << ") " << name
+           << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+           << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
+           << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
+        os << "    extern __shared__ double lds[];\n";
+        if (counter) os << "    __shared__ unsigned s_next;\n";
+        if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 4);\n";
+        os << "    const unsigned numTiles = tiles[0];\n";
+        os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
+        os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
+        if (variant != 2) stamp(os, 0);
+        os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
+        os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10], nconst = H[11];\n";
+        std::string vsum = "0u", ssum = "0u";
+        for (size_t t = 0; t < NT; ++t) {
+            const size_t h = FUSED_HEADER + 3 * t;
+            const FusedTemplate& ft = fp.templates[t];
+            os << "        const unsigned cnt" << t << " = H[" << h << "];\n";
+            vsum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.numOutputs) + "u";
+            ssum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.argNode.size() + (size_t)ft.numParams) + "u";
+        }
+        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)lane; (void)wv;\n";
+        os << "        const unsigned nvals = " << vsum << ", nstaged = " << ssum << ";\n";
+        if (variant == 0) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals;\n";
+        if (variant == 1) os << "        double* __restrict__ V = y + H[12];\n        double* __restrict__ Ast = lds;\n";
+        if (variant == 2) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals;\n";
+        os << "        INDEX* __restrict__ R = (INDEX*)(Ast + " << (variant == 2 ? "0u" : "nstaged") << ");\n";
+        os << "        double* __restrict__ K = (double*)(R + ((words + 1u) & ~1u));\n";
+        os << "        (void)nvals; (void)nstaged; (void)R; (void)K; (void)V;\n";
+        os << "        {\n";
+        if (stage) {
+            // A0: one loop over the elements of every template (an if-chain picks the template),
+            // so a tile's templates share one pass of the workgroup
+            std::string total = "0u";
+            for (size_t t = 0; t < NT; ++t) total += " + cnt" + std::to_string(t);
+            os << "            for (unsigned e = threadIdx.x; e < " << total << "; e += " << T << "u) {\n"
+               << "                unsigned i = e, Sb = 0;\n";
+            for (size_t t = 0; t < NT; ++t) {
+                const size_t h = FUSED_HEADER + 3 * t;
+                const FusedTemplate& ft = fp.templates[t];
+                os << "                if (i < cnt" << t << ") {\n"
+                   << "                    symlib_stage" << t << "(x, B, N, H + H[" << h + 1 << "], c + H[" << h + 2 << "], cnt" << t
+                   << ", i, Ast + Sb);\n"
+                   << "                    continue;\n"
+                   << "                }\n"
+                   << "                i -= cnt" << t << ";\n"
+                   << "                Sb += cnt" << t << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
+            }
+            os << "            }\n";
+        }
+        if (variant == 2)
+            os << "            const double* __restrict__ img = x + H[12];\n"
+               << "            for (unsigned q = threadIdx.x; q < nvals; q += " << T << "u) V[q] = img[q];\n";
+        if (sums)
+            os << "            const INDEX* __restrict__ src = H + H[4];\n"
+               << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
+               << "            if (nconst) {\n"
+               << "                const double* __restrict__ cs = c + H[8];\n"
+               << "                for (unsigned q = threadIdx.x; q < nconst; q += " << T << "u) K[q] = cs[q];\n"
+               << "            }\n";
+        os << "        }\n";
+        os << "        __syncthreads();\n";
+        if (variant != 2) stamp(os, 1);
+        if (eval) {
+            // A1: one loop over the (template, group, element) work items of the tile, elements
+            // fastest, so lanes of a wave share their template and group except at run boundaries
+            std::string total = "0u";
+            for (size_t t = 0; t < NT; ++t) total += " + " + std::to_string(groupsOf[t]) + "u * cnt" + std::to_string(t);
+            os << "        {\n"
+               << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
+               << "                unsigned j = w, Lb = 0, Sb = 0;\n";
+            for (size_t t = 0; t < NT; ++t) {
+                const FusedTemplate& ft = fp.templates[t];
+                const std::string cnt = "cnt" + std::to_string(t);
+                os << "                if (j < " << groupsOf[t] << "u * " << cnt << ") {\n"
+                   << "                    const unsigned g = j / " << cnt << ", i = j - g * " << cnt << ";\n";
+                for (int g = 0; g < groupsOf[t]; ++g)
+                    os << "                    " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
+                       << "symlib_elem" << t << "_" << g << "(Ast + Sb, " << cnt << ", i, V + Lb);\n";
+                os << "                    continue;\n"
+                   << "                }\n"
+                   << "                j -= " << groupsOf[t] << "u * " << cnt << ";\n"
+                   << "                Lb += " << cnt << " * " << ft.numOutputs << "u;\n"
+                   << "                Sb += " << cnt << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
+            }
+            os << "            }\n"
+               << "        }\n";
+            if (sums) os << "        __syncthreads();\n";
+            stamp(os, 2);
+        }
+        if (sums) {
+            os << "        {\n"
+               << "            const unsigned r0 = H[4];\n"
+               << "            const INDEX* __restrict__ Rn = R;\n"
+               << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (H[5] - r0));\n"
+               << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (H[6] - r0));\n"
+               << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
+               << "            double* __restrict__ res = out + H[9];\n"
+               << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
+               << "                const unsigned oi = ch * 64u + lane;\n"
+               << "                const bool act = oi < m;\n"
+               << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
+               << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
+               << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
+               << "                const unsigned base = CH[3u * ch] + (ex & 0xffffu);\n"
+               << "                double s = 0.0;\n"
+               << "                for (unsigned q = 0; q < k; ++q) s += V[S[base + q]];\n"
+               << "                if (act) {\n"
+               << "                    if (hc) s += K[CH[3u * ch + 2u] + (ex >> 16)];\n"
+               << "                    unsigned r = CH[3u * ch + 1u];\n"
+               << "                    while (Rn[2u * r + 3u] <= oi) ++r;\n"
+               << "                    res[Rn[2u * r] + (oi - Rn[2u * r + 1u])] = s;\n"
+               << "                }\n"
+               << "            }\n"
+               << "        }\n";
+        }
+        os << "        __syncthreads();\n";
+        if (variant != 1) stamp(os, 3);
+        if (counter) {
+            os << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work + " << ctr << "u, 1u);\n"
+               << "        __syncthreads();\n"
+               << "        it = s_next;\n";
+        } else if (persistent) {
+            os << "        it += gridDim.x;\n";
+        } else {
+            os << "        break;\n";
+        }
+        os << "    }\n";
+        if (counter)
+            os << "    if (threadIdx.x == 0) {\n"
+               << "        __threadfence();\n"
+               << "        if (atomicAdd(work + " << ctr + 1 << "u, 1u) == gridDim.x - 1u) {\n"
+               << "            atomicExch(work + " << ctr << "u, 0u);\n"
+               << "            atomicExch(work + " << ctr + 1 << "u, 0u);\n"
+               << "        }\n"
+               << "    }\n";
+        os << "}\n\n";
+    };
     launches.clear();
     if (!persistent && fp.numTiles > 0xffffffffull) throw std::length_error("emitHipFused: grid too large");
     const uint32_t grid = persistent ? LaunchDesc::GRID_FILL : (uint32_t)fp.numTiles;
-    launches.push_back({"symlib_fused", grid, (uint32_t)T, (uint32_t)fp.ldsBytes});
+    if (opt.split) {
+        kernel("symlib_fused_elem", 1);
+        kernel("symlib_fused_sum", 2);
+        launches.push_back({"symlib_fused_elem", grid, (uint32_t)T, (uint32_t)std::max<size_t>(8, fp.stagedLdsBytes)});
+        launches.push_back({"symlib_fused_sum", grid, (uint32_t)T, (uint32_t)fp.ldsBytes});
+    } else {
+        kernel("symlib_fused", 0);
+        launches.push_back({"symlib_fused", grid, (uint32_t)T, (uint32_t)fp.ldsBytes});
+    }
     return os.str();
 }
 

@@ -196,15 +196,22 @@ def test_fused_batch_patterns():
             assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
 
 
-@pytest.mark.parametrize("groups,wpe,schedule", [(1, 0, "static"), (9, 5, "queue")])
-def test_fused_source_compiles_for_gfx950(tmp_path, groups, wpe, schedule):
+@pytest.mark.parametrize("groups,wpe,schedule,split", [(1, 0, "static", False), (9, 5, "queue", False),
+                                                      (3, 0, "queue", True), (5, 4, "static", True)])
+def test_fused_source_compiles_for_gfx950(tmp_path, groups, wpe, schedule, split):
     L = _native.lib()
     rest, F, X = W.c3_cloth(side=6)
     with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, output_groups=groups,
-                         waves_per_eu=wpe, schedule=schedule) as u:
+                         waves_per_eu=wpe, schedule=schedule, split=split) as u:
         src = u.source()
-        assert "symlib_fused" in src and "symlib_elem0_0" in src
+        assert "symlib_elem0_0" in src
+        kernels = ["symlib_fused_elem", "symlib_fused_sum"] if split else ["symlib_fused("]
+        assert all(k in src for k in kernels)
+        assert ("symlib_fused_elem" in src) == split
         assert ("atomicAdd" in src) == (schedule != "static")
+        # the retuned source of the other form
+        other = u.fused_source(output_groups=groups, waves_per_eu=wpe, schedule=schedule, split=not split)
+        assert ("symlib_fused_elem" in other) == (not split)
         out = str(tmp_path / "fused.co")
         assert L.symhip_compile_to_file(src.encode(), None, b"gfx950", out.encode()) == 0, L.symhip_last_error()
 

@@ -145,26 +145,31 @@ def test_bw_hessian_batch(require_device, tiled):
 
 
 FUSED_CASES = [
-    # (side, schedule, lds bytes, output groups, waves per EU)
-    (4, "queue", 0, 0, 0),
-    (16, "static", 0, 0, 0),
-    (16, "queue", 8192, 1, 0),
-    (33, "largest", 16384, 9, 5),
-    (64, "interleaved", 49152, 3, 4),
-    (256, "queue", 0, 9, 5),
+    # (side, schedule, lds bytes, output groups, waves per EU, split)
+    (4, "queue", 0, 0, 0, False),
+    (16, "static", 0, 0, 0, False),
+    (16, "queue", 8192, 1, 0, False),
+    (33, "largest", 16384, 9, 5, False),
+    (64, "interleaved", 49152, 3, 4, False),
+    (256, "queue", 0, 9, 5, False),
+    (4, "static", 0, 0, 0, True),
+    (33, "queue", 16384, 3, 0, True),
+    (64, "strided", 49152, 5, 4, True),
+    (256, "queue", 0, 5, 0, True),
 ]
 
 
-@pytest.mark.parametrize("side,schedule,lds,groups,wpe", FUSED_CASES)
-def test_bw_hessian_fused(require_device, side, schedule, lds, groups, wpe):
+@pytest.mark.parametrize("side,schedule,lds,groups,wpe,split", FUSED_CASES)
+def test_bw_hessian_fused(require_device, side, schedule, lds, groups, wpe, split):
     rest, F, X = W.c3_cloth(side=side)
     with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, schedule=schedule, tile_lds_bytes=lds,
-                         output_groups=groups, waves_per_eu=wpe) as u:
+                         output_groups=groups, waves_per_eu=wpe, split=split) as u:
         assert u.info()["tiled"] == 2, u.info()["tiling"]
         u.compile().execute(W.to_colmajor(X))
+        assert u.num_launches() == (2 if split else 1)
         ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
-        _check(f"Baraff-Witkin {side}x{side} fused {schedule} lds={lds} groups={groups} wpe={wpe}", u, ref,
-               u.get_results())
+        _check(f"Baraff-Witkin {side}x{side} fused {schedule} lds={lds} groups={groups} wpe={wpe} split={split}", u,
+               ref, u.get_results())
         # a second evaluation of the same launch (persistent work-queue counters start from zero again)
         X2 = W.cloth(side, seed=7)[2]
         u.execute(W.to_colmajor(X2))
@@ -172,22 +177,44 @@ def test_bw_hessian_fused(require_device, side, schedule, lds, groups, wpe):
                O.bw_hessian(rest, F, X2, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results())
 
 
-@pytest.mark.parametrize("schedule", ["static", "queue", "largest", "interleaved"])
-def test_bw_hessian_batch_fused(require_device, schedule):
+@pytest.mark.parametrize("schedule,split", [("static", False), ("queue", False), ("largest", False),
+                                            ("interleaved", False), ("static", True), ("queue", True)])
+def test_bw_hessian_batch_fused(require_device, schedule, split):
     sides = [5, 17, 32, 9, 40, 3]
     cloths = [W.cloth(s, seed=200 + s) for s in sides]
     with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True,
-                               schedule=schedule) as u:
+                               schedule=schedule, split=split) as u:
         u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
         for k, (rest, F, X) in enumerate(cloths):
             ref = O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
-            _check(f"fused batch {schedule} cloth {k} ({sides[k]}x{sides[k]})", u, ref, u.get_results(k), k)
+            _check(f"fused batch {schedule} split={split} cloth {k} ({sides[k]}x{sides[k]})", u, ref, u.get_results(k), k)
 
 
-def test_fused_graph_replays_match_eager(require_device):
+def test_fused_retune_between_one_and_two_launches(require_device):
+    sides = [20, 7, 33]
+    cloths = [W.cloth(s, seed=250 + s) for s in sides]
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True) as u:
+        u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
+        one = [u.get_results(k) for k in range(len(sides))]
+        for k, (rest, F, X) in enumerate(cloths):
+            _check(f"one launch cloth {k}", u, O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND), one[k], k)
+        for step, (split, schedule) in enumerate([(True, "queue"), (True, "static"), (False, "strided")]):
+            u.retune(split=split, schedule=schedule)
+            assert u.num_launches() == (2 if split else 1)
+            # new positions, so every output must be rewritten by the retuned launches
+            Xs = [W.cloth(s, seed=260 + 10 * step + s)[2] for s in sides]
+            u.execute(*[W.to_colmajor(X) for X in Xs])
+            for k, (rest, F, _) in enumerate(cloths):
+                _check(f"retuned split={split} {schedule} cloth {k}", u,
+                       O.bw_hessian(rest, F, Xs[k], W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results(k), k)
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_fused_graph_replays_match_eager(require_device, split):
     sides = [12, 30, 7]
     cloths = [W.cloth(s, seed=300 + s) for s in sides]
-    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True) as u:
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True,
+                               split=split) as u:
         u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
         eager = [u.get_results(k) for k in range(len(sides))]
         for k, (rest, F, X) in enumerate(cloths):
