@@ -49,6 +49,8 @@ typedef struct {
     int waves_per_eu;            /* fused form: minimum waves per SIMD to compile for, 0 = none */
     int profile_tiles;           /* fused form: record wall-clock stamps per tile (symlib_unit_tile_profile) */
     int split;                   /* fused cloth form: 1 = two launches (element images in HBM, then sums) */
+    int register_staging;        /* fused forms: 1 = stage tiles through registers instead of
+                                    direct global-to-LDS loads (default 0) */
 } symlib_options;
 
 /* Fused-form program and code statistics (the autotuner's features). */

@@ -19,7 +19,8 @@ class Options(ctypes.Structure):
                 ("simplify", ctypes.c_int), ("share_templates", ctypes.c_int), ("tiled", ctypes.c_int),
                 ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int), ("fused", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("output_groups", ctypes.c_int), ("waves_per_eu", ctypes.c_int),
-                ("profile_tiles", ctypes.c_int), ("split", ctypes.c_int)]
+                ("profile_tiles", ctypes.c_int), ("split", ctypes.c_int),
+                ("register_staging", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):

@@ -50,6 +50,7 @@ Sym::Device makeDevice(const symlib_options* opt) {
     d.wavesPerEU = o.waves_per_eu;
     d.profileTiles = o.profile_tiles != 0;
     d.fusedSplit = o.split != 0;
+    d.ldsDma = o.register_staging == 0;
     return d;
 }
 
@@ -175,6 +176,7 @@ void symlib_default_options(symlib_options* opt) {
     opt->waves_per_eu = 0;
     opt->profile_tiles = 0;
     opt->split = 0;
+    opt->register_staging = 0;
 }
 
 int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {
@@ -313,6 +315,7 @@ Sym::FusedOptions fusedOptionsFrom(const symlib_unit* u, const symlib_options* o
     fo.schedule = (Sym::FusedSchedule)opt->schedule;
     fo.profile = opt->profile_tiles != 0;
     fo.split = opt->split != 0 && u->unit->getFusedProgram().kind == 0;
+    fo.ldsDma = opt->register_staging == 0;
     return fo;
 }
 

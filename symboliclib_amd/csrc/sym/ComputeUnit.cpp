@@ -89,6 +89,7 @@ void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
         fo.wavesPerEU = device.wavesPerEU;
         fo.profile = device.profileTiles;
         fo.split = device.fusedSplit;
+        fo.ldsDma = device.ldsDma;
         fusedProgram = assembly.buildFused(opt, fo, io, assemblyPatterns);
         fusedOptions = fo;
         code = emitHipFused(fusedProgram, fo, launches, &fusedStats);
@@ -115,6 +116,7 @@ void ComputeUnitBase::initFromProducts(const ProductSum& ps) {
     fo.schedule = (FusedSchedule)device.schedule;
     fo.wavesPerEU = device.wavesPerEU;
     fo.profile = device.profileTiles;
+    fo.ldsDma = device.ldsDma;
     std::vector<int> outer, inner;
     fusedProgram = buildProductTiles(ps.n, ps.colptr, ps.rowidx, fo, outer, inner);
     fusedOptions = fo;

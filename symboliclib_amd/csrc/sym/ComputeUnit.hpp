@@ -63,6 +63,8 @@ using OutputGroups = NamedType<int, struct OutputGroupsTag>;
 using WavesPerEU = NamedType<int, struct WavesPerEUTag>;
 // Fused form in two launches: element values of every tile to HBM, then the sums.
 struct UseSplit {};
+// Fused form: stage tiles through registers instead of direct global-to-LDS loads.
+struct UseRegisterStaging {};
 
 class Device {
 public:
@@ -82,6 +84,7 @@ public:
     int wavesPerEU = 0;
     bool profileTiles = false;
     bool fusedSplit = false;
+    bool ldsDma = true;
 
 private:
     void set(const DecompositionThreshold& nt) { decompositionThreshold = (unsigned)nt.x; }
@@ -101,6 +104,7 @@ private:
     void set(const OutputGroups& g) { outputGroups = g.x; }
     void set(const WavesPerEU& w) { wavesPerEU = w.x; }
     void set(const UseSplit&) { fusedSplit = true; }
+    void set(const UseRegisterStaging&) { ldsDma = false; }
     void set(const UseHIP&) {
         hipDevice = true;
         cudaDevice = false;

@@ -37,7 +37,8 @@ struct TemplateLayout {
     std::vector<int> gatherNode, gatherDof;
     int numNodes = 0, numParams = 0, numOutputs = 0;
     // LDS doubles per element: staged arguments and parameters, then its outputs
-    int ldsPerElement() const { return (int)gatherNode.size() + numParams + numOutputs; }
+    int valueStride() const { return numOutputs | 1; }
+    int ldsPerElement() const { return (int)gatherNode.size() + numParams + valueStride(); }
 };
 
 // Tiles of one instance, pool offsets relative to the instance's chunk.
@@ -152,6 +153,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         ft.argDof = tl.gatherDof;
         ft.numParams = tl.numParams;
         ft.numOutputs = tl.numOutputs;
+        ft.valueStride = tl.valueStride();
         if ((size_t)tl.ldsPerElement() * 8 > fo.ldsBytes)
             throw std::invalid_argument("fused form: one element needs more LDS than a tile holds");
         fp.templates.push_back(std::move(ft));
@@ -282,7 +284,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             for (size_t t = 0; t < NT; ++t) {
                 std::sort(tElems[t].begin(), tElems[t].end());
                 for (size_t q = 0; q < tElems[t].size(); ++q) local[tElems[t][q]] = (uint32_t)q;
-                ldsBase[t + 1] = ldsBase[t] + tElems[t].size() * (size_t)L[t].numOutputs;
+                ldsBase[t + 1] = ldsBase[t] + tElems[t].size() * (size_t)L[t].valueStride();
             }
             if (ldsBase[NT] > 65536) throw std::logic_error("fused form: tile LDS index exceeds 16 bits");
             std::vector<uint32_t> runs;  // (position, first output)
@@ -310,7 +312,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                             const size_t slot = (size_t)(s & 0xffff);
                             if (stamp[i] != tileId) throw std::logic_error("fused form: contribution from an element outside the tile");
                             const size_t t = (size_t)elemT[e0 + i];
-                            slotList.push_back((uint16_t)(ldsBase[t] + slot * tElems[t].size() + local[i]));
+                            slotList.push_back((uint16_t)(ldsBase[t] + (size_t)local[i] * (size_t)L[t].valueStride() + slot));
                         }
                         if (P.hasConst[q]) {
                             cvals.push_back(P.cval[q]);
@@ -511,7 +513,7 @@ std::vector<float> tileStats(const FusedProgram& fp) {
             const size_t cnt = P[H + HEADER + 3 * q];
             elems += cnt;
             staged += cnt * (fp.templates[q].argNode.size() + (size_t)fp.templates[q].numParams);
-            vals += cnt * (size_t)fp.templates[q].numOutputs;
+            vals += cnt * (size_t)fp.templates[q].valueStride;
         }
         while (P[H + P[H + 4] + 2 * runs + 1] != m) ++runs;
         float* f = &out[t * TILE_STATS];
@@ -623,7 +625,7 @@ void validateFused(const FusedProgram& fp) {
             for (size_t i = 0; i < (size_t)ft.numNodes * cnt; ++i) need(P[nodes + i] < N, "node outside its instance");
             for (size_t g = 0; g < ft.argDof.size(); ++g)
                 need((size_t)B + (size_t)ft.argDof[g] * N + N <= fp.numInputs, "gather outside the inputs");
-            lds += cnt * (size_t)ft.numOutputs;
+            lds += cnt * (size_t)ft.valueStride;
             staged += cnt * (ft.argNode.size() + (size_t)ft.numParams);
         }
         need(lds <= 65536, "tile LDS values exceed 16-bit slots");

@@ -31,8 +31,8 @@
 //     H[12] the tile's element image in the value buffer (split form: first value, in doubles)
 //     H[13 + 3 t], H[14 + 3 t], H[15 + 3 t]: template t's element count, rel node list
 //       (numNodes_t columns of count), parameter block in c[] (numParams_t columns of count)
-//   LDS of a tile, in doubles: element values (template t's outputs at V_t + slot * count_t
-//   + i, V_t = sum over t' < t of count_t' * numOutputs_t'), then the staged element
+//   LDS of a tile, in doubles: element values (template t's outputs at V_t + i * stride_t
+//   + slot, V_t = sum over t' < t of count_t' * stride_t', stride = valueStride), then the staged element
 //   arguments and parameters (template t: argument k at A_t + k * count_t + i), then the
 //   recipe (u32 words copied from H[4]), then the output constants.
 #pragma once
@@ -66,6 +66,9 @@ struct FusedOptions {
     // split form: one launch evaluates every tile's elements into its image in HBM (no LDS for
     // values, high occupancy), a second launch stages images and recipes in LDS and sums
     bool split = false;
+    // stage tiles with direct global-to-LDS loads (no register round trip, no wait per load);
+    // false: through registers
+    bool ldsDma = true;
 };
 
 // `work` starts with two work-queue counters (next tile, finished workgroups) per launch of an
@@ -80,7 +83,10 @@ struct FusedTemplate {
     int numNodes = 0;              // nodes per element
     std::vector<int> argNode, argDof;  // per gathered argument
     int numParams = 0;             // parameter columns (non-uniform constants)
-    int numOutputs = 0;            // LDS values per element
+    int numOutputs = 0;            // values per element
+    // LDS doubles between consecutive elements' values (numOutputs rounded up to odd, so the
+    // 64 lanes of a wave storing one output of 64 consecutive elements hit distinct banks)
+    int valueStride = 0;
 };
 
 struct FusedProgram {

@@ -52,13 +52,13 @@ struct KernelEmitter {
     // tiled form: per-tile table D (operands | ids | outpos), cnt instances, LDS base L
     const TiledProgram* tiled = nullptr;
     size_t kernelIndex = 0;
-    // fused form: element template evaluated into LDS (L[slot * cnt + i])
+    // fused form: element template evaluated into LDS (L[slot], L = the element's values)
     const FusedTemplate* fusedT = nullptr;
 
     std::string fusedLeaf(const std::array<int, 2>& v) const {
         switch (v[1]) {
             case -2:
-                return "L[" + std::to_string(v[0]) + "u * cnt + i]";
+                return "L[" + std::to_string(v[0]) + "]";
             case -3:
                 return "a" + std::to_string(v[0]);
             case -4:
@@ -480,9 +480,12 @@ std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const Emit
 
 // Fused assembly form (tile pool layout in Fused.hpp).  Per tile, three phases separated by
 // barriers:
-//   A0  every thread stages elements' arguments (gathered coordinates) and parameters into
-//       LDS, and the workgroup copies the tile's recipe tables and output constants to LDS
-//       (all loads independent, so they are in flight together);
+//   A0  the workgroup stages elements' arguments (gathered coordinates) and parameters, the
+//       tile's recipe tables and its output constants into LDS.  With direct global-to-LDS
+//       loads (gfx950 global_load_lds_dword: lane l of a wave writes dword l after an LDS base
+//       held in M0) every copy is issued without a register round trip and without waiting,
+//       and one wait precedes the barrier; a gathered double takes two lanes (low and high
+//       dword).  Without them every thread loads into registers and stores to LDS;
 //   A1  work items (element, output group) evaluate the element templates from LDS into the
 //       tile's LDS values (groups split the outputs so a work item keeps few values live);
 //   B   each wave takes 64 consecutive owned outputs, finds its lanes' contribution lists by
@@ -492,6 +495,53 @@ std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const Emit
 // workgroup to finish clears the counter, so consecutive launches (and graph replays) start
 // from zero.
 namespace {
+
+// Exclusive prefix sum over the 64 lanes of a wave with DPP moves (register crossbar, no LDS
+// round trip): shifts by 1, 2, 4, 8 within each row of 16 lanes, then the row_bcast:15 and
+// row_bcast:31 steps carry the sums of rows 0 and 0-1 into the rows above.  Lanes whose
+// source is outside the row, or whose row is masked off, add 0 (`old` operand).
+void emitWaveScan(std::ostream& os) {
+    os << "__device__ __forceinline__ unsigned symlib_excl_scan64(unsigned v, unsigned lane) {\n"
+       << "    (void)lane;\n"
+       << "    unsigned s = v;\n"
+       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x111, 0xf, 0xf, false);\n"
+       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x112, 0xf, 0xf, false);\n"
+       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x114, 0xf, 0xf, false);\n"
+       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x118, 0xf, 0xf, false);\n"
+       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x142, 0xa, 0xf, false);\n"
+       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x143, 0xc, 0xf, false);\n"
+       << "    return s - v;\n}\n\n";
+}
+
+// Direct global-to-LDS copy helpers for a workgroup of T threads: symlib_lds_copy32 moves n
+// dwords (64 per wave instruction); symlib_lds_gather64 moves doubles x[idx[i]], i < n, to
+// dst[i] (32 per wave instruction, four index loads in flight per wave).  Neither waits for
+// the data; the caller waits (s_waitcnt 0) before its barrier.
+void emitLdsDmaHelpers(std::ostream& os, int T) {
+    os << "#define SYMLIB_LDS_DMA4(src, dst) __builtin_amdgcn_global_load_lds((const void*)(src), "
+          "(__attribute__((address_space(3))) void*)(dst), 4, 0, 0)\n\n";
+    os << "__device__ __forceinline__ void symlib_lds_copy32(const unsigned* __restrict__ src, unsigned* dst, unsigned n,\n"
+       << "    unsigned wv, unsigned lane) {\n"
+       << "    for (unsigned b = wv * 64u; b < n; b += " << T << "u)\n"
+       << "        if (b + lane < n) SYMLIB_LDS_DMA4(src + b + lane, dst + b);\n"
+       << "}\n\n";
+    os << "__device__ __forceinline__ void symlib_lds_gather64(const double* __restrict__ x, const INDEX* __restrict__ idx,\n"
+       << "    double* dst, unsigned n, unsigned wv, unsigned lane) {\n"
+       << "    for (unsigned b = wv * 128u; b < n; b += " << 2 * T << "u) {\n"
+       << "        unsigned a[4];\n"
+       << "#pragma unroll\n"
+       << "        for (unsigned u = 0; u < 4u; ++u) {\n"
+       << "            const unsigned i = b + 32u * u + (lane >> 1);\n"
+       << "            a[u] = i < n ? idx[i] : 0u;\n"
+       << "        }\n"
+       << "#pragma unroll\n"
+       << "        for (unsigned u = 0; u < 4u; ++u) {\n"
+       << "            const unsigned i = b + 32u * u + (lane >> 1);\n"
+       << "            if (i < n) SYMLIB_LDS_DMA4((const unsigned*)(x + a[u]) + (lane & 1u), dst + b + 32u * u);\n"
+       << "        }\n"
+       << "    }\n"
+       << "}\n\n";
+}
 
 // Product-sum tiles (buildProductTiles): A0 stages the tile's values of A (one gather per
 // staged value, all independent) and its recipe in LDS; B sums, per output, A(i, j) and the products of LDS pairs.  Schedules and
@@ -506,13 +556,8 @@ std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std
     };
     os << "// generated by symboliclib_amd: product-sum tiles (A^T A + A), FP64\n";
     os << "typedef unsigned int INDEX;\n\n";
-    os << "__device__ __forceinline__ unsigned symlib_excl_scan64(unsigned v, unsigned lane) {\n"
-       << "    unsigned s = v;\n"
-       << "    for (unsigned d = 1; d < 64u; d <<= 1) {\n"
-       << "        const unsigned u = __shfl_up(s, d, 64);\n"
-       << "        if (lane >= d) s += u;\n"
-       << "    }\n"
-       << "    return s - v;\n}\n\n";
+    if (opt.ldsDma) emitLdsDmaHelpers(os, T);
+    emitWaveScan(os);
     os << "extern \"C\" __global__ void __launch_bounds__(" << T;
     if (opt.wavesPerEU > 0) os << ", " << opt.wavesPerEU;
     os << ") symlib_fused"
@@ -529,12 +574,17 @@ std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std
     os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
     os << "        const unsigned m = H[2], nch = H[3], words = H[10], nst = H[12];\n";
     os << "        INDEX* __restrict__ R = (INDEX*)(lds + nst);\n";
-    os << "        {\n"
-       << "            const INDEX* __restrict__ PS = H + H[13];\n"
-       << "            for (unsigned i = threadIdx.x; i < nst; i += " << T << "u) lds[i] = x[PS[i]];\n"
-       << "            const INDEX* __restrict__ src = H + H[4];\n"
-       << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
-       << "        }\n";
+    if (opt.ldsDma)
+        os << "        symlib_lds_gather64(x, H + H[13], lds, nst, wv, lane);\n"
+           << "        symlib_lds_copy32(H + H[4], R, words, wv, lane);\n"
+           << "        __builtin_amdgcn_s_waitcnt(0);\n";
+    else
+        os << "        {\n"
+           << "            const INDEX* __restrict__ PS = H + H[13];\n"
+           << "            for (unsigned i = threadIdx.x; i < nst; i += " << T << "u) lds[i] = x[PS[i]];\n"
+           << "            const INDEX* __restrict__ src = H + H[4];\n"
+           << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
+           << "        }\n";
     os << "        __syncthreads();\n";
     stamp(1);
     stamp(2);
@@ -560,7 +610,7 @@ std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std
     if (counter)
         os << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work, 1u);\n"
            << "        __syncthreads();\n"
-           << "        it = s_next;\n";
+           << "        it = __builtin_amdgcn_readfirstlane(s_next);  // uniform: header loads stay scalar\n";
     else if (persistent)
         os << "        it += gridDim.x;\n";
     else
@@ -604,6 +654,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
     std::ostringstream os;
     os << "// generated by symboliclib_amd: fused assembly form (LDS-staged element tiles), FP64\n";
     os << "typedef unsigned int INDEX;\n\n";
+    if (opt.ldsDma) emitLdsDmaHelpers(os, T);
     std::vector<int> groupsOf(NT);
     for (size_t t = 0; t < NT; ++t) {
         const FusedTemplate& ft = fp.templates[t];
@@ -621,6 +672,28 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         for (int q = 0; q < ft.numParams; ++q)
             os << "    A[" << nargs + (size_t)q << "u * cnt + i] = pr[" << q << "u * cnt + i];\n";
         os << "}\n\n";
+        if (opt.ldsDma) {
+            // one wave stages node slot j of 32 consecutive elements: its lanes load the node
+            // numbers (two lanes per element) and issue one direct load per gathered argument
+            // of that slot (low and high dwords of 32 doubles)
+            os << "__device__ __forceinline__ void symlib_stagew" << t
+               << "(const double* __restrict__ x, unsigned B, unsigned N, const INDEX* __restrict__ nd,\n"
+               << "    unsigned cnt, unsigned nb, unsigned v, double* A, unsigned lane) {\n"
+               << "    const unsigned j = v / nb, i0 = (v - j * nb) << 5, i = i0 + (lane >> 1);\n"
+               << "    if (i >= cnt) return;\n"
+               << "    const unsigned* s = (const unsigned*)(x + B + nd[j * cnt + i]) + (lane & 1u);\n";
+            for (int j = 0; j < ft.numNodes; ++j) {
+                os << "    " << (j ? "else " : "") << "if (j == " << j << "u) {\n";
+                for (size_t g = 0; g < nargs; ++g)
+                    if (ft.argNode[g] == j) {
+                        os << "        SYMLIB_LDS_DMA4(s";
+                        if (ft.argDof[g]) os << " + " << 2 * ft.argDof[g] << "u * N";
+                        os << ", A + " << g << "u * cnt + i0);\n";
+                    }
+                os << "    }\n";
+            }
+            os << "}\n\n";
+        }
         // A1: output groups, statements in a low-pressure order
         const std::vector<Stmt> S = flatten(ft.code, opt.shareReciprocals);
         std::vector<size_t> outs;
@@ -666,13 +739,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             os << "}\n\n";
         }
     }
-    os << "__device__ __forceinline__ unsigned symlib_excl_scan64(unsigned v, unsigned lane) {\n"
-       << "    unsigned s = v;\n"
-       << "    for (unsigned d = 1; d < 64u; d <<= 1) {\n"
-       << "        const unsigned u = __shfl_up(s, d, 64);\n"
-       << "        if (lane >= d) s += u;\n"
-       << "    }\n"
-       << "    return s - v;\n}\n\n";
+    emitWaveScan(os);
     // Kernel variants: 0 fused (stage, evaluate into LDS, sum), 1 split elements (stage, evaluate
     // into the tile's image in HBM), 2 split sums (copy image, recipe and constants to LDS, sum).
     auto kernel = [&](const char* name, int variant) {
@@ -698,7 +765,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             const size_t h = FUSED_HEADER + 3 * t;
             const FusedTemplate& ft = fp.templates[t];
             os << "        const unsigned cnt" << t << " = H[" << h << "];\n";
-            vsum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.numOutputs) + "u";
+            vsum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.valueStride) + "u";
             ssum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.argNode.size() + (size_t)ft.numParams) + "u";
         }
         os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)lane; (void)wv;\n";
@@ -710,7 +777,40 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         os << "        double* __restrict__ K = (double*)(R + ((words + 1u) & ~1u));\n";
         os << "        (void)nvals; (void)nstaged; (void)R; (void)K; (void)V;\n";
         os << "        {\n";
-        if (stage) {
+        if (stage && opt.ldsDma) {
+            // A0 with direct loads: one loop over (template, node slot, 32 elements) units, one
+            // wave each, then the parameter columns of each template (contiguous in `c`)
+            std::string total = "0u";
+            for (size_t t = 0; t < NT; ++t) {
+                os << "            const unsigned nb" << t << " = (cnt" << t << " + 31u) >> 5;\n";
+                total += " + " + std::to_string(fp.templates[t].numNodes) + "u * nb" + std::to_string(t);
+            }
+            os << "            for (unsigned u = wv; u < " << total << "; u += " << T / 64 << "u) {\n"
+               << "                unsigned v = u, Sb = 0;\n";
+            for (size_t t = 0; t < NT; ++t) {
+                const size_t h = FUSED_HEADER + 3 * t;
+                const FusedTemplate& ft = fp.templates[t];
+                const std::string units = std::to_string(ft.numNodes) + "u * nb" + std::to_string(t);
+                os << "                if (v < " << units << ") {\n"
+                   << "                    symlib_stagew" << t << "(x, B, N, H + H[" << h + 1 << "], cnt" << t << ", nb" << t
+                   << ", v, Ast + Sb, lane);\n"
+                   << "                    continue;\n"
+                   << "                }\n"
+                   << "                v -= " << units << ";\n"
+                   << "                Sb += cnt" << t << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
+            }
+            os << "            }\n";
+            std::string sb = "0u";
+            for (size_t t = 0; t < NT; ++t) {
+                const size_t h = FUSED_HEADER + 3 * t;
+                const FusedTemplate& ft = fp.templates[t];
+                if (ft.numParams > 0)
+                    os << "            symlib_lds_copy32((const unsigned*)(c + H[" << h + 2 << "]), (unsigned*)(Ast + " << sb
+                       << " + cnt" << t << " * " << ft.argNode.size() << "u), " << 2 * ft.numParams << "u * cnt" << t
+                       << ", wv, lane);\n";
+                sb += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.argNode.size() + (size_t)ft.numParams) + "u";
+            }
+        } else if (stage) {
             // A0: one loop over the elements of every template (an if-chain picks the template),
             // so a tile's templates share one pass of the workgroup
             std::string total = "0u";
@@ -730,16 +830,26 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             }
             os << "            }\n";
         }
-        if (variant == 2)
-            os << "            const double* __restrict__ img = x + H[12];\n"
-               << "            for (unsigned q = threadIdx.x; q < nvals; q += " << T << "u) V[q] = img[q];\n";
-        if (sums)
-            os << "            const INDEX* __restrict__ src = H + H[4];\n"
-               << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
-               << "            if (nconst) {\n"
-               << "                const double* __restrict__ cs = c + H[8];\n"
-               << "                for (unsigned q = threadIdx.x; q < nconst; q += " << T << "u) K[q] = cs[q];\n"
-               << "            }\n";
+        if (opt.ldsDma) {
+            if (variant == 2) os << "            symlib_lds_copy32((const unsigned*)(x + H[12]), (unsigned*)V, 2u * nvals, wv, lane);\n";
+            // with element evaluation in the launch, the recipe and constants are copied during
+            // it instead (A1), off the staging phase's critical path
+            if (sums && !eval)
+                os << "            symlib_lds_copy32(H + H[4], R, words, wv, lane);\n"
+                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + H[8]), (unsigned*)K, 2u * nconst, wv, lane);\n";
+            os << "            __builtin_amdgcn_s_waitcnt(0);\n";
+        } else {
+            if (variant == 2)
+                os << "            const double* __restrict__ img = x + H[12];\n"
+                   << "            for (unsigned q = threadIdx.x; q < nvals; q += " << T << "u) V[q] = img[q];\n";
+            if (sums)
+                os << "            const INDEX* __restrict__ src = H + H[4];\n"
+                   << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
+                   << "            if (nconst) {\n"
+                   << "                const double* __restrict__ cs = c + H[8];\n"
+                   << "                for (unsigned q = threadIdx.x; q < nconst; q += " << T << "u) K[q] = cs[q];\n"
+                   << "            }\n";
+        }
         os << "        }\n";
         os << "        __syncthreads();\n";
         if (variant != 2) stamp(os, 1);
@@ -748,8 +858,11 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             // fastest, so lanes of a wave share their template and group except at run boundaries
             std::string total = "0u";
             for (size_t t = 0; t < NT; ++t) total += " + " + std::to_string(groupsOf[t]) + "u * cnt" + std::to_string(t);
-            os << "        {\n"
-               << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
+            os << "        {\n";
+            if (opt.ldsDma && sums)
+                os << "            symlib_lds_copy32(H + H[4], R, words, wv, lane);\n"
+                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + H[8]), (unsigned*)K, 2u * nconst, wv, lane);\n";
+            os << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
                << "                unsigned j = w, Lb = 0, Sb = 0;\n";
             for (size_t t = 0; t < NT; ++t) {
                 const FusedTemplate& ft = fp.templates[t];
@@ -758,15 +871,17 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                    << "                    const unsigned g = j / " << cnt << ", i = j - g * " << cnt << ";\n";
                 for (int g = 0; g < groupsOf[t]; ++g)
                     os << "                    " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
-                       << "symlib_elem" << t << "_" << g << "(Ast + Sb, " << cnt << ", i, V + Lb);\n";
+                       << "symlib_elem" << t << "_" << g << "(Ast + Sb, " << cnt << ", i, V + Lb + i * " << ft.valueStride
+                       << "u);\n";
                 os << "                    continue;\n"
                    << "                }\n"
                    << "                j -= " << groupsOf[t] << "u * " << cnt << ";\n"
-                   << "                Lb += " << cnt << " * " << ft.numOutputs << "u;\n"
+                   << "                Lb += " << cnt << " * " << ft.valueStride << "u;\n"
                    << "                Sb += " << cnt << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
             }
             os << "            }\n"
                << "        }\n";
+            if (opt.ldsDma && sums) os << "        __builtin_amdgcn_s_waitcnt(0);\n";
             if (sums) os << "        __syncthreads();\n";
             stamp(os, 2);
         }
@@ -786,6 +901,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
                << "                const unsigned base = CH[3u * ch] + (ex & 0xffffu);\n"
                << "                double s = 0.0;\n"
+               << "#pragma unroll 4\n"
                << "                for (unsigned q = 0; q < k; ++q) s += V[S[base + q]];\n"
                << "                if (act) {\n"
                << "                    if (hc) s += K[CH[3u * ch + 2u] + (ex >> 16)];\n"
@@ -801,7 +917,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         if (counter) {
             os << "        if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(work + " << ctr << "u, 1u);\n"
                << "        __syncthreads();\n"
-               << "        it = s_next;\n";
+               << "        it = __builtin_amdgcn_readfirstlane(s_next);  // uniform: header loads stay scalar\n";
         } else if (persistent) {
             os << "        it += gridDim.x;\n";
         } else {
