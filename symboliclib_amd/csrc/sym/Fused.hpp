@@ -63,6 +63,7 @@ struct FusedOptions {
     int wavesPerEU = 0;               // minimum waves per SIMD asked of the compiler (0 = none)
     bool profile = false;             // record wall-clock stamps per tile (see tileProfileWords)
     bool shareReciprocals = true;     // one 1/d per repeated denominator d (x/d -> x*(1/d))
+    bool foldCoefficients = true;     // carry constant factors through the statements
     // split form: one launch evaluates every tile's elements into its image in HBM (no LDS for
     // values, high occupancy), a second launch stages images and recipes in LDS and sums
     bool split = false;

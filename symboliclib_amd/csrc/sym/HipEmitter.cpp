@@ -250,20 +250,323 @@ void shareReciprocals(std::vector<Symbolic>& seq) {
         });
     }
     std::map<hash_t, int> rcp;
-    std::vector<Symbolic> defs;
     for (auto& kv : count)
-        if (kv.second >= 2) {
-            rcp[kv.first] = nextLocal;
-            defs.emplace_back(DEFINE, Symbolic(nextLocal, LOCAL_VAR), Symbolic(DIV, Symbolic(1.0), denom.at(kv.first)));
-            ++nextLocal;
-        }
+        if (kv.second >= 2) rcp[kv.first] = nextLocal++;
     if (rcp.empty()) return;
+    // 1 / (u u) as (1 / u)^2 when 1 / u is shared too (a multiply instead of a division)
+    std::map<hash_t, Symbolic> defOf;
+    for (auto& b : seq)
+        if (b.op() == DEFINE) defOf.emplace(b[0].ahash(), b[1]);
+    std::vector<Symbolic> defs;
+    for (auto& kv : rcp) {
+        const Symbolic& d = denom.at(kv.first);
+        Symbolic value(DIV, Symbolic(1.0), d);
+        auto def = defOf.find(d.ahash());
+        if (d.op() == VAR && def != defOf.end() && def->second.op() == MUL && def->second.numChilds() == 2 &&
+            def->second[0].ahash() == def->second[1].ahash()) {
+            auto u = rcp.find(def->second[0].ahash());
+            if (u != rcp.end()) value = Symbolic(MUL, Symbolic(u->second, LOCAL_VAR), Symbolic(u->second, LOCAL_VAR));
+        }
+        defs.emplace_back(DEFINE, Symbolic(kv.second, LOCAL_VAR), value);
+    }
     std::vector<Symbolic> out(defs.begin(), defs.end());
     for (auto& b : seq) out.emplace_back(b.op(), b[0], withReciprocals(b[1], rcp));
     seq.swap(out);
 }
 
-std::vector<Stmt> flatten(const Symbolic& block, bool reciprocals = false) {
+// Constant coefficients carried through a straight-line statement list.  Every value is kept
+// as c * t with a numeric coefficient c and a "monic" expression t (t = 1 for constants):
+// products multiply coefficients (negations flip them), quotients and reciprocals divide
+// them, a sum (terms in canonical order) takes its smallest coefficient out of every term,
+// sqrt and pow take theirs out when it is positive.  A definition whose monic expression equals an earlier one (or is
+// a single variable) defines nothing new: it becomes an alias c * (earlier local).  Outputs
+// multiply their coefficient back in.  Differentiated templates are full of x * 2, -x and
+// 1 / (2 x) chains; carrying the constants removes most of those multiplies and the copies
+// that only differ by a constant factor.  Coefficient products are exact for the small
+// integer and power-of-two constants these chains carry; otherwise they reassociate a product
+// of constants, a change within a few units in the last place.
+class CoefficientFolding {
+    struct CT {
+        double c = 1.0;
+        Symbolic t;  // empty: the value is the constant c
+        bool one() const { return t.op() == NOOP; }
+    };
+    struct Alias {
+        double c;
+        Symbolic base;  // VAR of a defining local, argument or parameter; empty for a constant
+    };
+    std::map<int, Alias> local_;
+    std::map<hash_t, Symbolic> byTerm_;  // monic expression -> local holding it
+    std::map<hash_t, Symbolic> termOf_;  // local (VAR hash) -> its monic expression
+
+    static Symbolic materialize(const CT& x) {
+        if (x.one()) return Symbolic(x.c);
+        if (x.c == 1.0) return x.t;
+        if (x.c == -1.0) return Symbolic(NEG, x.t);
+        return Symbolic(MUL, Symbolic(x.c), x.t);
+    }
+    static CT plain(const Symbolic& t) { return CT{1.0, t}; }
+
+    CT norm(const Symbolic& y) {
+        const OpType op = y.op();
+        switch (op) {
+            case CONST:
+                return CT{y.constant(), Symbolic()};
+            case VAR: {
+                const auto v = y.variable();
+                if (v[1] == LOCAL_VAR) {
+                    auto it = local_.find(v[0]);
+                    if (it != local_.end()) return CT{it->second.c, it->second.base};
+                }
+                return plain(y);
+            }
+            case NEG: {
+                CT x = norm(y[0]);
+                x.c = -x.c;
+                return x;
+            }
+            case MUL:
+            case MULC: {
+                double c = 1.0;
+                std::vector<Symbolic> ts;
+                for (const auto& ch : y) {
+                    CT x = norm(ch);
+                    c *= x.c;
+                    if (!x.one()) ts.push_back(x.t);
+                }
+                if (c == 0.0 || ts.empty()) return CT{c, Symbolic()};
+                return CT{c, ts.size() == 1 ? ts[0] : Symbolic(MUL, ts)};
+            }
+            case DIV:
+            case RECIPROCAL: {
+                const CT a = op == DIV ? norm(y[0]) : CT{1.0, Symbolic()};
+                const CT b = norm(op == DIV ? y[1] : y[0]);
+                if (b.c == 0.0 || a.c == 0.0) return plain(Symbolic(DIV, materialize(a), materialize(b)));
+                const double c = a.c / b.c;
+                if (b.one()) return CT{c, a.t};
+                // a / (u u) with an earlier 1 / u = r: a r r, multiplies instead of a division
+                if (b.t.op() == VAR) {
+                    auto def = termOf_.find(b.t.ahash());
+                    if (def != termOf_.end() && def->second.op() == MUL && def->second.numChilds() == 2 &&
+                        def->second[0].ahash() == def->second[1].ahash()) {
+                        auto r = byTerm_.find(Symbolic(DIV, Symbolic(1.0), def->second[0]).ahash());
+                        if (r != byTerm_.end()) {
+                            const Symbolic rr(MUL, r->second, r->second);
+                            return CT{c, a.one() ? rr : Symbolic(MUL, a.t, rr)};
+                        }
+                    }
+                }
+                return CT{c, Symbolic(DIV, a.one() ? Symbolic(1.0) : a.t, b.t)};
+            }
+            case ADD: {
+                // combine equal monic terms, then factor the first term's coefficient out
+                std::vector<CT> terms;
+                std::map<hash_t, size_t> at;
+                double k = 0.0;
+                for (const auto& ch : y) {
+                    CT x = norm(ch);
+                    if (x.one()) {
+                        k += x.c;
+                        continue;
+                    }
+                    auto it = at.find(x.t.ahash());
+                    if (it == at.end()) {
+                        at.emplace(x.t.ahash(), terms.size());
+                        terms.push_back(x);
+                    } else {
+                        terms[it->second].c += x.c;
+                    }
+                }
+                std::vector<CT> nz;
+                for (auto& x : terms)
+                    if (x.c != 0.0) nz.push_back(x);
+                if (nz.empty()) return CT{k, Symbolic()};
+                // canonical term order, so a - b and b - a share one monic form
+                std::stable_sort(nz.begin(), nz.end(), [](const CT& a, const CT& b) { return a.t.ahash() < b.t.ahash(); });
+                // factor out the smallest coefficient (ties: the first), so the ratios left on
+                // the other terms are at least 1 in magnitude (integers when the constants are)
+                double c = nz[0].c;
+                for (auto& x : nz)
+                    if (std::fabs(x.c) < std::fabs(c)) c = x.c;
+                std::vector<Symbolic> parts;
+                for (auto& x : nz) parts.push_back(materialize(CT{x.c / c, x.t}));
+                if (k != 0.0) parts.push_back(Symbolic(k / c));
+                return CT{c, parts.size() == 1 ? parts[0] : Symbolic(ADD, parts)};
+            }
+            case SQRT: {
+                const CT x = norm(y[0]);
+                if (x.one()) return CT{std::sqrt(x.c), Symbolic()};
+                if (x.c > 0.0) return CT{std::sqrt(x.c), Symbolic(SQRT, x.t)};
+                return plain(Symbolic(SQRT, materialize(x)));
+            }
+            case POW: {
+                const CT x = norm(y[0]), e = norm(y[1]);
+                if (e.one() && x.c > 0.0 && !x.one()) return CT{std::pow(x.c, e.c), Symbolic(POW, x.t, Symbolic(e.c))};
+                return plain(Symbolic(POW, materialize(x), materialize(e)));
+            }
+            default: {
+                std::vector<Symbolic> ch;
+                for (const auto& c : y) ch.push_back(materialize(norm(c)));
+                return plain(Symbolic(op, ch));
+            }
+        }
+    }
+
+public:
+    void run(std::vector<Symbolic>& seq) {
+        std::vector<Symbolic> out;
+        out.reserve(seq.size());
+        for (const auto& b : seq) {
+            const CT x = norm(b[1]);
+            if (b.op() == ASSIGN) {
+                out.emplace_back(ASSIGN, b[0], materialize(x));
+                continue;
+            }
+            const int id = b[0].variable()[0];
+            if (x.one() || x.t.op() == VAR) {
+                local_[id] = Alias{x.c, x.t};
+                continue;
+            }
+            auto it = byTerm_.find(x.t.ahash());
+            if (it != byTerm_.end()) {
+                local_[id] = Alias{x.c, it->second};
+                continue;
+            }
+            const Symbolic v(id, LOCAL_VAR);
+            byTerm_.emplace(x.t.ahash(), v);
+            termOf_.emplace(v.ahash(), x.t);
+            local_[id] = Alias{x.c, v};
+            out.emplace_back(DEFINE, v, x.t);
+        }
+        seq.swap(out);
+    }
+};
+
+// Products shared between statements: the pair of variable factors that occurs together in
+// the most products (two or more) becomes one new local p = a * b, replacing a and b in each
+// of those products; repeated until no pair repeats (greedy pair extraction).  Nested products
+// inside one expression are flattened first.  Only variable factors (locals, arguments,
+// parameters) pair; constants and compound factors stay where they are.
+class ProductSharing {
+    struct PNode {
+        OpType op = NOOP;
+        Symbolic leaf;                // CONST / VAR / other leaves
+        std::vector<int> ch;          // children (PNode indices); for MUL the non-variable factors
+        std::vector<Symbolic> vars;   // MUL only: variable factors
+    };
+    std::vector<PNode> nodes_;
+    std::vector<int> muls_;
+
+    int build(const Symbolic& y) {
+        const OpType op = y.op();
+        if (y.numChilds() == 0) {
+            PNode n;
+            n.op = op;
+            n.leaf = y;
+            nodes_.push_back(n);
+            return (int)nodes_.size() - 1;
+        }
+        if (op == MUL || op == MULC) {
+            std::vector<Symbolic> factors, st{y};
+            while (!st.empty()) {
+                Symbolic f = st.back();
+                st.pop_back();
+                if (f.op() == MUL || f.op() == MULC)
+                    for (unsigned i = f.numChilds(); i-- > 0;) st.push_back(f[i]);
+                else
+                    factors.push_back(f);
+            }
+            PNode n;
+            n.op = MUL;
+            std::vector<int> ch;
+            for (auto& f : factors) {
+                if (f.op() == VAR) n.vars.push_back(f);
+                else ch.push_back(build(f));
+            }
+            n.ch = ch;
+            nodes_.push_back(n);
+            muls_.push_back((int)nodes_.size() - 1);
+            return (int)nodes_.size() - 1;
+        }
+        PNode n;
+        n.op = op;
+        std::vector<int> ch;
+        for (const auto& c : y) ch.push_back(build(c));
+        n.ch = ch;
+        nodes_.push_back(n);
+        return (int)nodes_.size() - 1;
+    }
+
+    Symbolic emit(int i) const {
+        const PNode& n = nodes_[(size_t)i];
+        if (n.op != MUL && n.ch.empty()) return n.leaf;
+        std::vector<Symbolic> ch;
+        if (n.op == MUL)
+            for (auto& v : n.vars) ch.push_back(v);
+        for (int c : n.ch) ch.push_back(emit(c));
+        if (n.op == MUL && ch.size() == 1) return ch[0];
+        return Symbolic(n.op, ch);
+    }
+
+public:
+    void run(std::vector<Symbolic>& seq) {
+        int nextLocal = 0;
+        for (auto& b : seq) {
+            if (b.op() == DEFINE) nextLocal = std::max(nextLocal, b[0].variable()[0] + 1);
+            preOrderTraverse(b[1], [&](const Symbolic& y) {
+                if (y.op() == VAR && y.variable()[1] == LOCAL_VAR) nextLocal = std::max(nextLocal, y.variable()[0] + 1);
+                return true;
+            });
+        }
+        std::vector<int> roots;
+        for (auto& b : seq) roots.push_back(build(b[1]));
+        std::vector<Symbolic> defs;
+        for (;;) {
+            std::map<std::pair<hash_t, hash_t>, int> count;
+            std::map<hash_t, Symbolic> byHash;
+            for (int m : muls_) {
+                const auto& v = nodes_[(size_t)m].vars;
+                std::set<std::pair<hash_t, hash_t>> seen;
+                for (size_t i = 0; i < v.size(); ++i) {
+                    byHash.emplace(v[i].ahash(), v[i]);
+                    for (size_t j = i + 1; j < v.size(); ++j) {
+                        hash_t a = v[i].ahash(), b = v[j].ahash();
+                        if (b < a) std::swap(a, b);
+                        if (seen.insert({a, b}).second) ++count[{a, b}];
+                    }
+                }
+            }
+            std::pair<hash_t, hash_t> best{};
+            int bestCount = 1;
+            for (auto& kv : count)
+                if (kv.second > bestCount) {
+                    bestCount = kv.second;
+                    best = kv.first;
+                }
+            if (bestCount < 2) break;
+            const Symbolic p(nextLocal++, LOCAL_VAR);
+            defs.emplace_back(DEFINE, p, Symbolic(MUL, byHash.at(best.first), byHash.at(best.second)));
+            for (int m : muls_) {
+                auto& v = nodes_[(size_t)m].vars;
+                int ia = -1, ib = -1;
+                for (int i = 0; i < (int)v.size(); ++i) {
+                    if (ia < 0 && v[(size_t)i].ahash() == best.first) ia = i;
+                    else if (ib < 0 && v[(size_t)i].ahash() == best.second) ib = i;
+                }
+                if (ia < 0 || ib < 0) continue;
+                v.erase(v.begin() + std::max(ia, ib));
+                v.erase(v.begin() + std::min(ia, ib));
+                v.push_back(p);
+            }
+        }
+        if (defs.empty()) return;
+        std::vector<Symbolic> out(defs.begin(), defs.end());
+        for (size_t i = 0; i < seq.size(); ++i) out.emplace_back(seq[i].op(), seq[i][0], emit(roots[i]));
+        seq.swap(out);
+    }
+};
+
+std::vector<Stmt> flatten(const Symbolic& block, bool reciprocals = false, bool coefficients = false) {
     std::vector<Stmt> out;
     std::vector<Symbolic> seq;
     // BLOCKs nest only at the top; keep statement order
@@ -275,7 +578,9 @@ std::vector<Stmt> flatten(const Symbolic& block, bool reciprocals = false) {
         seq.push_back(b);
     };
     walk(block);
+    if (coefficients) CoefficientFolding().run(seq);
     if (reciprocals) shareReciprocals(seq);
+    if (coefficients) ProductSharing().run(seq);
     for (auto& b : seq) {
         Stmt s;
         s.s = b;
@@ -695,7 +1000,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             os << "}\n\n";
         }
         // A1: output groups, statements in a low-pressure order
-        const std::vector<Stmt> S = flatten(ft.code, opt.shareReciprocals);
+        const std::vector<Stmt> S = flatten(ft.code, opt.shareReciprocals, opt.foldCoefficients);
         std::vector<size_t> outs;
         for (size_t i = 0; i < S.size(); ++i)
             if (S[i].out >= 0) outs.push_back(i);
