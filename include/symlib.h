@@ -37,7 +37,8 @@ typedef struct {
     int simplify;                /* 1 = simplify kernel templates (default), 0 = skip */
     int share_templates;         /* 1 = one differentiated template per element shape (default) */
     int tiled;                   /* 1 = LDS-staged work tiles when possible (default), 0 = wave launches */
-    int tile_lds_bytes;          /* LDS budget per tile, 0 = default (64 KiB) */
+    int tile_lds_bytes;          /* LDS bytes per workgroup, 0 = default (tiled form 64 KiB; fused
+                                    forms 53760, all regions, so three workgroups share a CU) */
     int direct_assembly;         /* cloth Hessian: 1 = HessianAssembly (no per-nonzero expressions),
                                     0 = symbolic SparseMatrix through ComputeUnit (default) */
     int fused;                   /* cloth Hessian: 1 = fused assembly form (implies direct_assembly):

@@ -78,7 +78,7 @@ public:
     bool tiled = false;
     unsigned int tileLdsBytes = 64 * 1024;
     bool fused = false;
-    unsigned int fusedLdsBytes = 32 * 1024;
+    unsigned int fusedLdsBytes = 53760;
     int schedule = (int)FusedSchedule::Queue;
     int outputGroups = 5;
     int wavesPerEU = 0;

@@ -164,7 +164,12 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
     // ---- per instance: pattern and tiles
     std::vector<InstanceTiles> tiles(inst_.size());
     std::vector<SparseMatrix<double>> pats(inst_.size());
-    const size_t ldsValues = fo.ldsBytes / 8;
+    // Tiles grow until the workgroup's whole LDS (element values, staged arguments, recipe and
+    // output constants) would exceed fo.ldsBytes; recipe and constants are charged per owned
+    // node by an upper bound of what it adds (its count bytes, u16 slots, constants, two run
+    // words per dof column and its share of the 64-output chunk records), so the tile built
+    // never needs more than the budget.
+    const double ldsBudget = (double)fo.ldsBytes - 64.0;  // run terminator, chunk and word rounding
     parallelFor(inst_.size(), [&](size_t k) {
         const Instance& I = inst_[k];
         InstancePattern P;
@@ -202,8 +207,16 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 }
         }
         std::vector<uint32_t> outCount((size_t)N, 0);
-        for (int b = 0; b < N; ++b)
-            for (int d = 0; d < D; ++d) outCount[b] += (uint32_t)(colptr[b + d * N + 1] - colptr[b + d * N]);
+        std::vector<double> nodeBytes((size_t)N, 0.0);
+        for (int b = 0; b < N; ++b) {
+            double bytes = 8.0 * D;
+            for (int d = 0; d < D; ++d) {
+                const int c = b + d * N;
+                outCount[b] += (uint32_t)(colptr[c + 1] - colptr[c]);
+                for (int q = colptr[c]; q < colptr[c + 1]; ++q) bytes += 2.0 * P.count[q] + (P.hasConst[q] ? 8.0 : 0.0);
+            }
+            nodeBytes[b] = bytes + outCount[b] * (1.0 + 12.0 / 64.0);
+        }
 
         std::vector<char> assigned((size_t)N, 0);
         for (int b = 0; b < N; ++b) assigned[b] = outCount[b] == 0;
@@ -238,6 +251,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             tNodes.clear();
             for (auto& te : tElems) te.clear();
             size_t slots = 0, outs = 0;
+            double recipeBytes = 0.0;
             push((uint32_t)cursor, cost((uint32_t)cursor, tileId));
             for (;;) {
                 // pop the cheapest frontier node (FIFO within a cost)
@@ -252,7 +266,15 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                         found = true;
                         break;
                     }
-                if (!found) break;
+                if (!found) {
+                    // frontier exhausted (a pocket enclosed by earlier tiles): continue with the
+                    // next unassigned node rather than closing a small tile
+                    size_t next = cursor;
+                    while (next < (size_t)N && assigned[next]) ++next;
+                    if (next == (size_t)N) break;
+                    push((uint32_t)next, cost((uint32_t)next, tileId));
+                    continue;
+                }
                 const uint32_t cur = cost(v, tileId);
                 if (cur != popped) {
                     push(v, cur);
@@ -261,13 +283,14 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 size_t newSlots = 0;
                 for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q)
                     if (stamp[inc[q]] != tileId) newSlots += (size_t)L[elemT[e0 + inc[q]]].ldsPerElement();
-                if (!tNodes.empty() && (slots + newSlots > ldsValues || outs + outCount[v] > fo.maxOutputs)) break;
-                if (slots + newSlots > ldsValues)
-                    throw std::invalid_argument("fused form: the elements of one node need more LDS than a tile holds");
+                const bool fits = 8.0 * (double)(slots + newSlots) + recipeBytes + nodeBytes[v] <= ldsBudget;
+                if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs)) break;
+                if (!fits) throw std::invalid_argument("fused form: one node and its elements need more LDS than a tile holds");
                 assigned[v] = 1;
                 tNodes.push_back(v);
                 outs += outCount[v];
                 slots += newSlots;
+                recipeBytes += nodeBytes[v];
                 for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q) {
                     const uint32_t i = inc[q];
                     if (stamp[i] == tileId) continue;

@@ -55,7 +55,7 @@ enum class FusedSchedule : int {
 };
 
 struct FusedOptions {
-    size_t ldsBytes = 32 * 1024;      // element values per tile
+    size_t ldsBytes = 53760;          // LDS bytes per workgroup, all regions (3 fit in 160 KiB)
     size_t maxOutputs = 1u << 16;     // owned outputs per tile
     int threadsPerBlock = 256;
     FusedSchedule schedule = FusedSchedule::Queue;

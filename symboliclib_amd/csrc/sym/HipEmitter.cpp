@@ -724,7 +724,11 @@ std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<Lau
             os << "        const unsigned i = (b - " << b0 << "u) * " << T << "u + threadIdx.x;\n";
             os << "        if (i < " << K.n << "u) {\n";
             KernelEmitter ke{prog, K, wide};
-            ke.statements(K.code, os, "            ");
+            // the template's statements with constant coefficients folded, shared reciprocals
+            // and products, in a low-register-pressure order
+            const std::vector<Stmt> S = flatten(K.code, true, true);
+            const std::vector<char> all(S.size(), 1);
+            for (size_t si : lowPressureOrder(S, all)) ke.statements(S[si].s, os, "            ");
             os << "        }\n        return;\n    }\n";
         }
         os << "}\n\n";

@@ -27,7 +27,7 @@ import numpy as np
 from . import _native
 
 SPACE = {
-    "lds": [24576, 32768, 49152, 65536],
+    "lds": [40448, 53760, 65024, 81408],  # whole-workgroup LDS: 4, 3, 2, 2 workgroups per CU
     "groups": [3, 5, 9],
     "wpe": [0, 4, 5],
     "tpb": [256, 512, 1024],
