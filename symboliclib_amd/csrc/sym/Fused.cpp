@@ -215,7 +215,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 outCount[b] += (uint32_t)(colptr[c + 1] - colptr[c]);
                 for (int q = colptr[c]; q < colptr[c + 1]; ++q) bytes += 2.0 * P.count[q] + (P.hasConst[q] ? 8.0 : 0.0);
             }
-            nodeBytes[b] = bytes + outCount[b] * (1.0 + 12.0 / 64.0);
+            nodeBytes[b] = bytes + outCount[b] * (1.0 + 4.0 * FUSED_CHUNK_WORDS / 64.0);
         }
 
         std::vector<char> assigned((size_t)N, 0);
@@ -346,15 +346,20 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             runs.push_back(0);
             runs.push_back((uint32_t)m);
             const size_t nch = (m + 63) / 64;
-            std::vector<uint32_t> chunks(3 * nch);
+            const size_t CW = FUSED_CHUNK_WORDS;
+            std::vector<uint32_t> chunks(CW * nch);
             {
                 size_t sb = 0, cb = 0, r = 0;
                 for (size_t ch = 0; ch < nch; ++ch) {
                     const size_t o0 = ch * 64;
                     while (runs[2 * (r + 1) + 1] <= o0) ++r;
-                    chunks[3 * ch] = (uint32_t)sb;
-                    chunks[3 * ch + 1] = (uint32_t)r;
-                    chunks[3 * ch + 2] = (uint32_t)cb;
+                    chunks[CW * ch] = (uint32_t)sb;
+                    chunks[CW * ch + 1] = (uint32_t)r;
+                    chunks[CW * ch + 2] = (uint32_t)cb;
+                    uint64_t mask = 0;
+                    for (size_t q = r + 1; runs[2 * q + 1] < std::min(m, o0 + 64); ++q) mask |= 1ull << (runs[2 * q + 1] - o0);
+                    chunks[CW * ch + 3] = (uint32_t)mask;
+                    chunks[CW * ch + 4] = (uint32_t)(mask >> 32);
                     for (size_t o = o0; o < std::min(m, o0 + 64); ++o) {
                         sb += counts[o] & 0x7fu;
                         cb += counts[o] >> 7;
@@ -392,7 +397,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 pool[H + 8] = NONE;
             }
             pool[H + 9] = 0;  // result base, set at merge
-            pool[H + 10] = chunksOff + 3 * (uint32_t)nch - runsOff;
+            pool[H + 10] = chunksOff + (uint32_t)(CW * nch) - runsOff;
             pool[H + 11] = (uint32_t)cvals.size() * (anyConst ? 1u : 0u);
             pool[H + 12] = 0;  // element image, set at merge
             size_t evals = 0;
@@ -634,10 +639,11 @@ void validateFused(const FusedProgram& fp) {
         need(nch == (m + 63) / 64, "chunk count");
         const size_t runs = H + P[H + 4], counts = H + P[H + 5], slots = H + P[H + 6], chunks = H + P[H + 7];
         const uint32_t cOff = P[H + 8], rbase = P[H + 9];
-        need(chunks + 3 * (size_t)nch <= P.size() && counts + ((size_t)m + 3) / 4 <= P.size(), "tables outside the pool");
+        const size_t CW = FUSED_CHUNK_WORDS;
+        need(chunks + CW * (size_t)nch <= P.size() && counts + ((size_t)m + 3) / 4 <= P.size(), "tables outside the pool");
         // LDS of the tile and element data
         size_t lds = 0, staged = 0;
-        need(P[H + 10] == P[H + 7] + 3 * nch - P[H + 4] && P[H + 4] <= P[H + 5] && P[H + 5] <= P[H + 6] &&
+        need(P[H + 10] == P[H + 7] + CW * nch - P[H + 4] && P[H + 4] <= P[H + 5] && P[H + 5] <= P[H + 6] &&
                  P[H + 6] <= P[H + 7],
              "recipe tables not contiguous");
         for (size_t q = 0; q < NT; ++q) {
@@ -682,9 +688,14 @@ void validateFused(const FusedProgram& fp) {
         const size_t numRuns = r;
         size_t sb = 0, cb = 0;
         for (size_t ch = 0; ch < nch; ++ch) {
-            need(P[chunks + 3 * ch] == sb && P[chunks + 3 * ch + 2] == cb, "chunk slot or constant base");
-            const size_t rr = P[chunks + 3 * ch + 1];
+            need(P[chunks + CW * ch] == sb && P[chunks + CW * ch + 2] == cb, "chunk slot or constant base");
+            const size_t rr = P[chunks + CW * ch + 1];
             need(rr < numRuns && P[runs + 2 * rr + 1] <= ch * 64 && P[runs + 2 * rr + 3] > ch * 64, "chunk run");
+            // run-start mask: exactly the runs that begin inside the chunk after its first output
+            uint64_t mask = 0;
+            for (size_t q = rr + 1; q < numRuns && P[runs + 2 * q + 1] < std::min<size_t>(m, ch * 64 + 64); ++q)
+                mask |= 1ull << (P[runs + 2 * q + 1] - ch * 64);
+            need(P[chunks + CW * ch + 3] == (uint32_t)mask && P[chunks + CW * ch + 4] == (uint32_t)(mask >> 32), "chunk run mask");
             for (size_t o = ch * 64; o < std::min<size_t>(m, ch * 64 + 64); ++o) {
                 sb += cnts[o] & 0x7fu;
                 cb += cnts[o] >> 7;

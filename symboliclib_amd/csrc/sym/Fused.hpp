@@ -22,7 +22,9 @@
 //     H[5] rel counts: u8 per output, element contributions (bits 0-6) and whether the
 //          output adds a constant (bit 7)
 //     H[6] rel slots: u16 LDS indices, contributions of every output in order
-//     H[7] rel chunks: (first slot, run, first constant) per 64-output chunk
+//     H[7] rel chunks: (first slot, run of the first output, first constant, run-start mask
+//          low and high: bit j set when output 64 ch + j, j >= 1, starts a run) per 64-output
+//          chunk, FUSED_CHUNK_WORDS words each
 //     H[8] output constants in c[], one per output with bit 7 set, in output order, or
 //          0xffffffff
 //     H[9] result base of the instance
@@ -113,6 +115,7 @@ struct FusedProgram {
 // staged doubles, element LDS values, recipe words, output constants, runs.
 constexpr int TILE_STATS = 8;
 constexpr int FUSED_HEADER = 13;  // fixed header words of an element tile
+constexpr int FUSED_CHUNK_WORDS = 5;  // words per 64-output chunk record of an element tile
 std::vector<float> tileStats(const FusedProgram& fp);
 
 // Write the tile table pool[1..numTiles] in the order of `schedule`.

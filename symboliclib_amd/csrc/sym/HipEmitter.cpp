@@ -1205,19 +1205,34 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
                << "                const unsigned oi = ch * 64u + lane;\n"
                << "                const bool act = oi < m;\n"
+               // the chunk record and the lane's count byte in one round of LDS reads
+               << "                const INDEX* __restrict__ cr = CH + " << FUSED_CHUNK_WORDS << "u * ch;\n"
+               << "                const unsigned sb = cr[0], rr = cr[1], cb = cr[2], mlo = cr[3], mhi = cr[4];\n"
                << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
                << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
                << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
-               << "                const unsigned base = CH[3u * ch] + (ex & 0xffffu);\n"
+               << "                const unsigned base = sb + (ex & 0xffffu);\n"
+               // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
+               << "                const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
+               << "                const unsigned r = rr + (unsigned)__popcll((((unsigned long long)mhi << 32) | mlo) & below);\n"
+               << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
+               << "                double kc = 0.0;\n"
+               << "                if (hc) kc = K[cb + (ex >> 16)];\n"
+               // four slots per step: their indices in one round of reads, then their values
+               // (indices past the lane's count read the lane's first slot, and add nothing)
                << "                double s = 0.0;\n"
-               << "#pragma unroll 4\n"
-               << "                for (unsigned q = 0; q < k; ++q) s += V[S[base + q]];\n"
-               << "                if (act) {\n"
-               << "                    if (hc) s += K[CH[3u * ch + 2u] + (ex >> 16)];\n"
-               << "                    unsigned r = CH[3u * ch + 1u];\n"
-               << "                    while (Rn[2u * r + 3u] <= oi) ++r;\n"
-               << "                    res[Rn[2u * r] + (oi - Rn[2u * r + 1u])] = s;\n"
+               << "                for (unsigned q = 0; q < k; q += 4u) {\n"
+               << "                    const unsigned i0 = S[base + q];\n"
+               << "                    const unsigned i1 = S[q + 1u < k ? base + q + 1u : base];\n"
+               << "                    const unsigned i2 = S[q + 2u < k ? base + q + 2u : base];\n"
+               << "                    const unsigned i3 = S[q + 3u < k ? base + q + 3u : base];\n"
+               << "                    const double v0 = V[i0], v1 = V[i1], v2 = V[i2], v3 = V[i3];\n"
+               << "                    s += v0;\n"
+               << "                    if (q + 1u < k) s += v1;\n"
+               << "                    if (q + 2u < k) s += v2;\n"
+               << "                    if (q + 3u < k) s += v3;\n"
                << "                }\n"
+               << "                if (act) res[pos] = hc ? s + kc : s;\n"
                << "            }\n"
                << "        }\n";
         }
