@@ -577,6 +577,24 @@ void orderTiles(FusedProgram& fp, FusedSchedule schedule) {
             break;
     }
     for (size_t t = 0; t < n; ++t) fp.pool[1 + t] = fp.tileHeader[order[t]];
+    if (fp.kind != 0) return;
+    // element tiles: the headers again, in launch order at a fixed stride, with offsets made
+    // absolute (pool indices), so a workgroup reads its header in one round of loads
+    const size_t NT = fp.templates.size(), words = FUSED_HEADER + 3 * NT;
+    fp.headerStride = (words + 3) & ~size_t(3);
+    if (fp.headerTable == 0) {
+        fp.headerTable = fp.pool.size();
+        if (fp.headerTable + n * fp.headerStride > 0xffffffffull) throw std::length_error("fused form: tile pool exceeds 32-bit indices");
+        fp.pool.resize(fp.headerTable + n * fp.headerStride, 0);
+    }
+    for (size_t t = 0; t < n; ++t) {
+        const size_t H = fp.pool[1 + t];
+        uint32_t* dst = &fp.pool[fp.headerTable + t * fp.headerStride];
+        for (size_t w = 0; w < words; ++w) {
+            const bool rel = (w >= 4 && w <= 7) || (w >= (size_t)FUSED_HEADER && (w - FUSED_HEADER) % 3 == 1);
+            dst[w] = rel ? (uint32_t)(H + fp.pool[H + w]) : fp.pool[H + w];
+        }
+    }
 }
 
 namespace {
@@ -703,6 +721,16 @@ void validateFused(const FusedProgram& fp) {
         }
     }
     need(totalContrib == fp.numContributions, "contribution count");
+    // the launch-order header table
+    const size_t words = FUSED_HEADER + 3 * NT;
+    need(fp.headerStride >= words && fp.headerTable + fp.numTiles * fp.headerStride <= P.size(), "header table outside the pool");
+    for (size_t t = 0; t < fp.numTiles; ++t) {
+        const size_t H = P[1 + t];
+        for (size_t w = 0; w < words; ++w) {
+            const bool rel = (w >= 4 && w <= 7) || (w >= (size_t)FUSED_HEADER && (w - FUSED_HEADER) % 3 == 1);
+            need(P[fp.headerTable + t * fp.headerStride + w] == (rel ? H + P[H + w] : P[H + w]), "header table entry");
+        }
+    }
 }
 
 }  // namespace Sym

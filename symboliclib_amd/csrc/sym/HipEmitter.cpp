@@ -1067,7 +1067,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
         os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
         if (variant != 2) stamp(os, 0);
-        os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
+        os << "        const INDEX* __restrict__ H = tiles + " << fp.headerTable << "u + it * " << fp.headerStride << "u;  // launch-order header\n";
         os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10], nconst = H[11];\n";
         std::string vsum = "0u", ssum = "0u";
         for (size_t t = 0; t < NT; ++t) {
@@ -1101,7 +1101,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 const FusedTemplate& ft = fp.templates[t];
                 const std::string units = std::to_string(ft.numNodes) + "u * nb" + std::to_string(t);
                 os << "                if (v < " << units << ") {\n"
-                   << "                    symlib_stagew" << t << "(x, B, N, H + H[" << h + 1 << "], cnt" << t << ", nb" << t
+                   << "                    symlib_stagew" << t << "(x, B, N, tiles + H[" << h + 1 << "], cnt" << t << ", nb" << t
                    << ", v, Ast + Sb, lane);\n"
                    << "                    continue;\n"
                    << "                }\n"
@@ -1130,7 +1130,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 const size_t h = FUSED_HEADER + 3 * t;
                 const FusedTemplate& ft = fp.templates[t];
                 os << "                if (i < cnt" << t << ") {\n"
-                   << "                    symlib_stage" << t << "(x, B, N, H + H[" << h + 1 << "], c + H[" << h + 2 << "], cnt" << t
+                   << "                    symlib_stage" << t << "(x, B, N, tiles + H[" << h + 1 << "], c + H[" << h + 2 << "], cnt" << t
                    << ", i, Ast + Sb);\n"
                    << "                    continue;\n"
                    << "                }\n"
@@ -1144,7 +1144,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             // with element evaluation in the launch, the recipe and constants are copied during
             // it instead (A1), off the staging phase's critical path
             if (sums && !eval)
-                os << "            symlib_lds_copy32(H + H[4], R, words, wv, lane);\n"
+                os << "            symlib_lds_copy32(tiles + H[4], R, words, wv, lane);\n"
                    << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + H[8]), (unsigned*)K, 2u * nconst, wv, lane);\n";
             os << "            __builtin_amdgcn_s_waitcnt(0);\n";
         } else {
@@ -1152,7 +1152,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 os << "            const double* __restrict__ img = x + H[12];\n"
                    << "            for (unsigned q = threadIdx.x; q < nvals; q += " << T << "u) V[q] = img[q];\n";
             if (sums)
-                os << "            const INDEX* __restrict__ src = H + H[4];\n"
+                os << "            const INDEX* __restrict__ src = tiles + H[4];\n"
                    << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
                    << "            if (nconst) {\n"
                    << "                const double* __restrict__ cs = c + H[8];\n"
@@ -1169,7 +1169,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             for (size_t t = 0; t < NT; ++t) total += " + " + std::to_string(groupsOf[t]) + "u * cnt" + std::to_string(t);
             os << "        {\n";
             if (opt.ldsDma && sums)
-                os << "            symlib_lds_copy32(H + H[4], R, words, wv, lane);\n"
+                os << "            symlib_lds_copy32(tiles + H[4], R, words, wv, lane);\n"
                    << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + H[8]), (unsigned*)K, 2u * nconst, wv, lane);\n";
             os << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
                << "                unsigned j = w, Lb = 0, Sb = 0;\n";
