@@ -1202,37 +1202,56 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (H[6] - r0));\n"
                << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
                << "            double* __restrict__ res = out + H[9];\n"
-               << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
-               << "                const unsigned oi = ch * 64u + lane;\n"
-               << "                const bool act = oi < m;\n"
-               // the chunk record and the lane's count byte in one round of LDS reads
-               << "                const INDEX* __restrict__ cr = CH + " << FUSED_CHUNK_WORDS << "u * ch;\n"
-               << "                const unsigned sb = cr[0], rr = cr[1], cb = cr[2], mlo = cr[3], mhi = cr[4];\n"
-               << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
-               << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
-               << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
-               << "                const unsigned base = sb + (ex & 0xffffu);\n"
-               // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
-               << "                const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
-               << "                const unsigned r = rr + (unsigned)__popcll((((unsigned long long)mhi << 32) | mlo) & below);\n"
-               << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
-               << "                double kc = 0.0;\n"
-               << "                if (hc) kc = K[cb + (ex >> 16)];\n"
-               // four slots per step: their indices in one round of reads, then their values
-               // (indices past the lane's count read the lane's first slot, and add nothing)
-               << "                double s = 0.0;\n"
-               << "                for (unsigned q = 0; q < k; q += 4u) {\n"
-               << "                    const unsigned i0 = S[base + q];\n"
-               << "                    const unsigned i1 = S[q + 1u < k ? base + q + 1u : base];\n"
-               << "                    const unsigned i2 = S[q + 2u < k ? base + q + 2u : base];\n"
-               << "                    const unsigned i3 = S[q + 3u < k ? base + q + 3u : base];\n"
-               << "                    const double v0 = V[i0], v1 = V[i1], v2 = V[i2], v3 = V[i3];\n"
-               << "                    s += v0;\n"
-               << "                    if (q + 1u < k) s += v1;\n"
-               << "                    if (q + 2u < k) s += v2;\n"
-               << "                    if (q + 3u < k) s += v3;\n"
-               << "                }\n"
-               << "                if (act) res[pos] = hc ? s + kc : s;\n"
+               << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
+               << "            for (unsigned c0 = wv; c0 < nch; c0 += " << 2 * (T / 64) << "u) {\n";
+            // two chunks per step (c0 and c0 + waves) with interleaved dependency chains; the
+            // second one is clamped to the first and inactive past the last chunk
+            os << "                const bool two = c0 + " << T / 64 << "u < nch;\n";
+            for (const char* X : {"A", "B"}) {
+                const std::string x = X;
+                os << "                const unsigned ch" << x << " = " << (x == "A" ? "c0" : "two ? c0 + " + std::to_string(T / 64) + "u : c0")
+                   << ";\n"
+                   << "                const unsigned oi" << x << " = ch" << x << " * 64u + lane;\n"
+                   << "                const bool act" << x << " = oi" << x << " < m" << (x == "B" ? " && two" : "") << ";\n"
+                   << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * ch" << x << ";\n"
+                   << "                const unsigned b8" << x << " = act" << x << " ? (unsigned)C8[oi" << x << "] : 0u;\n"
+                   << "                const unsigned k" << x << " = b8" << x << " & 0x7fu, hc" << x << " = b8" << x << " >> 7;\n";
+            }
+            for (const char* X : {"A", "B"}) {
+                const std::string x = X;
+                os << "                const unsigned ex" << x << " = symlib_excl_scan64(k" << x << " | (hc" << x << " << 16), lane);\n"
+                   << "                const unsigned base" << x << " = cr" << x << "[0] + (ex" << x << " & 0xffffu);\n"
+                   // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
+                   << "                const unsigned r" << x << " = cr" << x << "[1] + (unsigned)__popcll((((unsigned long long)cr" << x
+                   << "[4] << 32) | cr" << x << "[3]) & below);\n"
+                   << "                const unsigned pos" << x << " = Rn[2u * r" << x << "] + (oi" << x << " - Rn[2u * r" << x << " + 1u]);\n"
+                   << "                double kc" << x << " = 0.0;\n"
+                   << "                if (hc" << x << ") kc" << x << " = K[cr" << x << "[2] + (ex" << x << " >> 16)];\n"
+                   << "                double s" << x << " = 0.0;\n";
+            }
+            // four slots of each chunk per step: indices in one round of reads, then values;
+            // slots past a lane's count read its first slot (in the tile's slot list) and add 0
+            os << "                const unsigned kmax = kA > kB ? kA : kB;\n"
+               << "                for (unsigned q = 0; q < kmax; q += 4u) {\n";
+            for (const char* X : {"A", "B"}) {
+                const std::string x = X;
+                for (int j = 0; j < 4; ++j)
+                    os << "                    const unsigned i" << x << j << " = q + " << j << "u < k" << x << " ? S[base" << x << " + q + "
+                       << j << "u] : 0u;\n";
+            }
+            for (const char* X : {"A", "B"}) {
+                const std::string x = X;
+                os << "                    const double v" << x << "0 = V[i" << x << "0], v" << x << "1 = V[i" << x << "1], v" << x << "2 = V[i"
+                   << x << "2], v" << x << "3 = V[i" << x << "3];\n";
+            }
+            for (const char* X : {"A", "B"}) {
+                const std::string x = X;
+                for (int j = 0; j < 4; ++j)
+                    os << "                    if (q + " << j << "u < k" << x << ") s" << x << " += v" << x << j << ";\n";
+            }
+            os << "                }\n"
+               << "                if (actA) res[posA] = hcA ? sA + kcA : sA;\n"
+               << "                if (actB) res[posB] = hcB ? sB + kcB : sB;\n"
                << "            }\n"
                << "        }\n";
         }
