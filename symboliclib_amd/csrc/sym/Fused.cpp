@@ -213,7 +213,8 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             for (int d = 0; d < D; ++d) {
                 const int c = b + d * N;
                 outCount[b] += (uint32_t)(colptr[c + 1] - colptr[c]);
-                for (int q = colptr[c]; q < colptr[c + 1]; ++q) bytes += 2.0 * P.count[q] + (P.hasConst[q] ? 8.0 : 0.0);
+                for (int q = colptr[c]; q < colptr[c + 1]; ++q)
+                    bytes += 2.0 * (P.count[q] + (P.count[q] & 1u)) + (P.hasConst[q] ? 8.0 : 0.0);
             }
             nodeBytes[b] = bytes + outCount[b] * (1.0 + 4.0 * FUSED_CHUNK_WORDS / 64.0);
         }
@@ -309,7 +310,8 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 for (size_t q = 0; q < tElems[t].size(); ++q) local[tElems[t][q]] = (uint32_t)q;
                 ldsBase[t + 1] = ldsBase[t] + tElems[t].size() * (size_t)L[t].valueStride();
             }
-            if (ldsBase[NT] > 65536) throw std::logic_error("fused form: tile LDS index exceeds 16 bits");
+            // slot ldsBase[NT] is the tile's zero (padding of odd contribution lists)
+            if (ldsBase[NT] >= 65536) throw std::logic_error("fused form: tile LDS index exceeds 16 bits");
             std::vector<uint32_t> runs;  // (position, first output)
             std::vector<uint8_t> counts;
             std::vector<uint16_t> slotList;
@@ -327,8 +329,9 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                         }
                         prevEnd = (uint32_t)q + 1;
                         const uint32_t cnt = P.count[q];
-                        if (cnt > 127) throw std::invalid_argument("fused form: an output has more than 127 element contributions");
-                        counts.push_back((uint8_t)(cnt | (P.hasConst[q] ? 0x80u : 0u)));
+                        if (cnt > 126) throw std::invalid_argument("fused form: an output has more than 126 element contributions");
+                        // lists padded to even length with the zero slot (read in pairs)
+                        counts.push_back((uint8_t)((cnt + (cnt & 1u)) | (P.hasConst[q] ? 0x80u : 0u)));
                         for (uint32_t r = 0; r < cnt; ++r) {
                             const uint64_t s = P.contrib[P.first[q] + r];
                             const size_t i = (size_t)(s >> 16) - e0;
@@ -337,6 +340,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                             const size_t t = (size_t)elemT[e0 + i];
                             slotList.push_back((uint16_t)(ldsBase[t] + (size_t)local[i] * (size_t)L[t].valueStride() + slot));
                         }
+                        if (cnt & 1u) slotList.push_back((uint16_t)ldsBase[NT]);
                         if (P.hasConst[q]) {
                             cvals.push_back(P.cval[q]);
                             anyConst = true;
@@ -423,14 +427,14 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             }
             size_t staged = 0;
             for (size_t t = 0; t < NT; ++t) staged += tElems[t].size() * (size_t)(L[t].gatherNode.size() + L[t].numParams);
-            const size_t tileLds = 8 * (ldsBase[NT] + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * (size_t)pool[H + 11];
+            const size_t tileLds = 8 * (ldsBase[NT] + 1 + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * (size_t)pool[H + 11];
             out.cost.push_back((double)m + (double)ldsBase[NT]);
             out.values.push_back(ldsBase[NT]);
             out.stagedLds = std::max(out.stagedLds, 8 * staged);
             out.evaluations += evals;
             out.contributions += slotList.size();
             out.lds = std::max(out.lds, tileLds);
-            out.elementLds = std::max(out.elementLds, 8 * (ldsBase[NT] + staged));
+            out.elementLds = std::max(out.elementLds, 8 * (ldsBase[NT] + 1 + staged));
             out.outputs += m;
             ++tileId;
         }
@@ -682,16 +686,17 @@ void validateFused(const FusedProgram& fp) {
         const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[counts]);
         size_t S = 0, numConst = 0;
         for (size_t o = 0; o < m; ++o) {
+            need((cnts[o] & 1u) == 0, "contribution list of odd length");
             S += cnts[o] & 0x7fu;
             numConst += cnts[o] >> 7;
         }
         if (numConst) need(cOff != 0xffffffffu && (size_t)cOff + numConst <= fp.consts.size(), "output constants outside the constants");
         need(P[H + 11] == numConst, "output constant count");
-        need(8 * (lds + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * numConst <= fp.ldsBytes,
+        need(8 * (lds + 1 + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * numConst <= fp.ldsBytes,
              "tile LDS exceeds the launch's LDS");
         need(slots + (S + 1) / 2 <= P.size(), "slot list outside the pool");
         const uint16_t* sl = reinterpret_cast<const uint16_t*>(&P[slots]);
-        for (size_t s = 0; s < S; ++s) need(sl[s] < lds, "contribution outside the tile's LDS");
+        for (size_t s = 0; s < S; ++s) need(sl[s] <= lds, "contribution outside the tile's LDS");  // lds: the zero slot
         totalContrib += S;
         // runs: first outputs increasing from 0 to the sentinel m; positions inside the results
         size_t r = 0;

@@ -808,15 +808,15 @@ namespace {
 // Exclusive prefix sum over the 64 lanes of a wave with DPP moves (register crossbar, no LDS
 // round trip): shifts by 1, 2, 4, 8 within each row of 16 lanes, then the row_bcast:15 and
 // row_bcast:31 steps carry the sums of rows 0 and 0-1 into the rows above.  Lanes whose
-// source is outside the row, or whose row is masked off, add 0 (`old` operand).
+// source is outside the row read 0 (bound_ctrl), lanes of masked-off rows keep `old` = 0.
 void emitWaveScan(std::ostream& os) {
     os << "__device__ __forceinline__ unsigned symlib_excl_scan64(unsigned v, unsigned lane) {\n"
        << "    (void)lane;\n"
        << "    unsigned s = v;\n"
-       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x111, 0xf, 0xf, false);\n"
-       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x112, 0xf, 0xf, false);\n"
-       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x114, 0xf, 0xf, false);\n"
-       << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x118, 0xf, 0xf, false);\n"
+       << "    s += (unsigned)__builtin_amdgcn_mov_dpp((int)s, 0x111, 0xf, 0xf, true);\n"
+       << "    s += (unsigned)__builtin_amdgcn_mov_dpp((int)s, 0x112, 0xf, 0xf, true);\n"
+       << "    s += (unsigned)__builtin_amdgcn_mov_dpp((int)s, 0x114, 0xf, 0xf, true);\n"
+       << "    s += (unsigned)__builtin_amdgcn_mov_dpp((int)s, 0x118, 0xf, 0xf, true);\n"
        << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x142, 0xa, 0xf, false);\n"
        << "    s += (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x143, 0xc, 0xf, false);\n"
        << "    return s - v;\n}\n\n";
@@ -1079,13 +1079,15 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         }
         os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)lane; (void)wv;\n";
         os << "        const unsigned nvals = " << vsum << ", nstaged = " << ssum << ";\n";
-        if (variant == 0) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals;\n";
+        // V[nvals]: the tile's zero (odd contribution lists are padded with its slot)
+        if (variant == 0) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
         if (variant == 1) os << "        double* __restrict__ V = y + H[12];\n        double* __restrict__ Ast = lds;\n";
-        if (variant == 2) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals;\n";
+        if (variant == 2) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
         os << "        INDEX* __restrict__ R = (INDEX*)(Ast + " << (variant == 2 ? "0u" : "nstaged") << ");\n";
         os << "        double* __restrict__ K = (double*)(R + ((words + 1u) & ~1u));\n";
         os << "        (void)nvals; (void)nstaged; (void)R; (void)K; (void)V;\n";
         os << "        {\n";
+        if (sums) os << "            if (threadIdx.x == 0u) V[nvals] = 0.0;\n";
         if (stage && opt.ldsDma) {
             // A0 with direct loads: one loop over (template, node slot, 32 elements) units, one
             // wave each, then the parameter columns of each template (contiguous in `c`)
@@ -1164,9 +1166,14 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         if (variant != 2) stamp(os, 1);
         if (eval) {
             // A1: one loop over the (template, group, element) work items of the tile, elements
-            // fastest, so lanes of a wave share their template and group except at run boundaries
+            // fastest; each (template, group) segment is padded to whole waves, so every wave
+            // runs one element function (no divergent pair of them) and idle lanes sit at the
+            // ends of segments
             std::string total = "0u";
-            for (size_t t = 0; t < NT; ++t) total += " + " + std::to_string(groupsOf[t]) + "u * cnt" + std::to_string(t);
+            for (size_t t = 0; t < NT; ++t) {
+                os << "        const unsigned pc" << t << " = (cnt" << t << " + 63u) & ~63u;\n";
+                total += " + " + std::to_string(groupsOf[t]) + "u * pc" + std::to_string(t);
+            }
             os << "        {\n";
             if (opt.ldsDma && sums)
                 os << "            symlib_lds_copy32(tiles + H[4], R, words, wv, lane);\n"
@@ -1175,16 +1182,20 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "                unsigned j = w, Lb = 0, Sb = 0;\n";
             for (size_t t = 0; t < NT; ++t) {
                 const FusedTemplate& ft = fp.templates[t];
-                const std::string cnt = "cnt" + std::to_string(t);
-                os << "                if (j < " << groupsOf[t] << "u * " << cnt << ") {\n"
-                   << "                    const unsigned g = j / " << cnt << ", i = j - g * " << cnt << ";\n";
+                const std::string ts = std::to_string(t), cnt = "cnt" + ts, pc = "pc" + ts;
+                os << "                if (j < " << groupsOf[t] << "u * " << pc << ") {\n"
+                   << "                    unsigned g = 0;\n";
+                for (int g = 1; g < groupsOf[t]; ++g) os << "                    g += j >= " << g << "u * " << pc << " ? 1u : 0u;\n";
+                os << "                    const unsigned i = j - g * " << pc << ";\n"
+                   << "                    if (i < " << cnt << ") {\n";
                 for (int g = 0; g < groupsOf[t]; ++g)
-                    os << "                    " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
+                    os << "                        " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
                        << "symlib_elem" << t << "_" << g << "(Ast + Sb, " << cnt << ", i, V + Lb + i * " << ft.valueStride
                        << "u);\n";
-                os << "                    continue;\n"
+                os << "                    }\n"
+                   << "                    continue;\n"
                    << "                }\n"
-                   << "                j -= " << groupsOf[t] << "u * " << cnt << ";\n"
+                   << "                j -= " << groupsOf[t] << "u * " << pc << ";\n"
                    << "                Lb += " << cnt << " * " << ft.valueStride << "u;\n"
                    << "                Sb += " << cnt << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
             }
@@ -1203,55 +1214,28 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
                << "            double* __restrict__ res = out + H[9];\n"
                << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
-               << "            for (unsigned c0 = wv; c0 < nch; c0 += " << 2 * (T / 64) << "u) {\n";
-            // two chunks per step (c0 and c0 + waves) with interleaved dependency chains; the
-            // second one is clamped to the first and inactive past the last chunk
-            os << "                const bool two = c0 + " << T / 64 << "u < nch;\n";
-            for (const char* X : {"A", "B"}) {
-                const std::string x = X;
-                os << "                const unsigned ch" << x << " = " << (x == "A" ? "c0" : "two ? c0 + " + std::to_string(T / 64) + "u : c0")
-                   << ";\n"
-                   << "                const unsigned oi" << x << " = ch" << x << " * 64u + lane;\n"
-                   << "                const bool act" << x << " = oi" << x << " < m" << (x == "B" ? " && two" : "") << ";\n"
-                   << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * ch" << x << ";\n"
-                   << "                const unsigned b8" << x << " = act" << x << " ? (unsigned)C8[oi" << x << "] : 0u;\n"
-                   << "                const unsigned k" << x << " = b8" << x << " & 0x7fu, hc" << x << " = b8" << x << " >> 7;\n";
-            }
-            for (const char* X : {"A", "B"}) {
-                const std::string x = X;
-                os << "                const unsigned ex" << x << " = symlib_excl_scan64(k" << x << " | (hc" << x << " << 16), lane);\n"
-                   << "                const unsigned base" << x << " = cr" << x << "[0] + (ex" << x << " & 0xffffu);\n"
-                   // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
-                   << "                const unsigned r" << x << " = cr" << x << "[1] + (unsigned)__popcll((((unsigned long long)cr" << x
-                   << "[4] << 32) | cr" << x << "[3]) & below);\n"
-                   << "                const unsigned pos" << x << " = Rn[2u * r" << x << "] + (oi" << x << " - Rn[2u * r" << x << " + 1u]);\n"
-                   << "                double kc" << x << " = 0.0;\n"
-                   << "                if (hc" << x << ") kc" << x << " = K[cr" << x << "[2] + (ex" << x << " >> 16)];\n"
-                   << "                double s" << x << " = 0.0;\n";
-            }
-            // four slots of each chunk per step: indices in one round of reads, then values;
-            // slots past a lane's count read its first slot (in the tile's slot list) and add 0
-            os << "                const unsigned kmax = kA > kB ? kA : kB;\n"
-               << "                for (unsigned q = 0; q < kmax; q += 4u) {\n";
-            for (const char* X : {"A", "B"}) {
-                const std::string x = X;
-                for (int j = 0; j < 4; ++j)
-                    os << "                    const unsigned i" << x << j << " = q + " << j << "u < k" << x << " ? S[base" << x << " + q + "
-                       << j << "u] : 0u;\n";
-            }
-            for (const char* X : {"A", "B"}) {
-                const std::string x = X;
-                os << "                    const double v" << x << "0 = V[i" << x << "0], v" << x << "1 = V[i" << x << "1], v" << x << "2 = V[i"
-                   << x << "2], v" << x << "3 = V[i" << x << "3];\n";
-            }
-            for (const char* X : {"A", "B"}) {
-                const std::string x = X;
-                for (int j = 0; j < 4; ++j)
-                    os << "                    if (q + " << j << "u < k" << x << ") s" << x << " += v" << x << j << ";\n";
-            }
-            os << "                }\n"
-               << "                if (actA) res[posA] = hcA ? sA + kcA : sA;\n"
-               << "                if (actB) res[posB] = hcB ? sB + kcB : sB;\n"
+               << "            const unsigned* __restrict__ S2 = (const unsigned*)S;  // slot pairs\n"
+               << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
+               << "                const unsigned oi = ch * 64u + lane;\n"
+               << "                const bool act = oi < m;\n"
+               // the chunk record and the lane's count byte in one round of LDS reads
+               << "                const INDEX* __restrict__ cr = CH + " << FUSED_CHUNK_WORDS << "u * ch;\n"
+               << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
+               << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
+               << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
+               << "                const unsigned base = cr[0] + (ex & 0xffffu);  // even: lists have even length\n"
+               // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
+               << "                const unsigned r = cr[1] + (unsigned)__popcll((((unsigned long long)cr[4] << 32) | cr[3]) & below);\n"
+               << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
+               << "                double kc = 0.0;\n"
+               << "                if (hc) kc = K[cr[2] + (ex >> 16)];\n"
+               << "                double s = 0.0;\n"
+               << "                for (unsigned q = 0; q < k; q += 2u) {\n"
+               << "                    const unsigned w = S2[(base + q) >> 1];\n"
+               << "                    s += V[w & 0xffffu];\n"
+               << "                    s += V[w >> 16];\n"
+               << "                }\n"
+               << "                if (act) res[pos] = hc ? s + kc : s;\n"
                << "            }\n"
                << "        }\n";
         }
