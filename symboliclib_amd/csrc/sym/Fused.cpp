@@ -455,7 +455,10 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
     }
     u32(ib, "element images");
     fp.imageDoubles = ib - fp.numInputs;
-    pb = 1 + nTiles;
+    // pool: tile count, launch order, launch-order header table, then the tiles
+    fp.headerStride = (FUSED_HEADER + 3 * NT + 3) & ~size_t(3);
+    fp.headerTable = 1 + nTiles;
+    pb = 1 + nTiles + nTiles * fp.headerStride;
     for (size_t k = 0; k < K; ++k) {
         if (tiles[k].outputs != pats[k].nonZeros())
             throw std::logic_error("fused form: tiles do not cover every output of an instance");
@@ -585,12 +588,7 @@ void orderTiles(FusedProgram& fp, FusedSchedule schedule) {
     // element tiles: the headers again, in launch order at a fixed stride, with offsets made
     // absolute (pool indices), so a workgroup reads its header in one round of loads
     const size_t NT = fp.templates.size(), words = FUSED_HEADER + 3 * NT;
-    fp.headerStride = (words + 3) & ~size_t(3);
-    if (fp.headerTable == 0) {
-        fp.headerTable = fp.pool.size();
-        if (fp.headerTable + n * fp.headerStride > 0xffffffffull) throw std::length_error("fused form: tile pool exceeds 32-bit indices");
-        fp.pool.resize(fp.headerTable + n * fp.headerStride, 0);
-    }
+    if (fp.headerTable != 1 + n || fp.headerStride < words) throw std::logic_error("orderTiles: no header table in the pool");
     for (size_t t = 0; t < n; ++t) {
         const size_t H = fp.pool[1 + t];
         uint32_t* dst = &fp.pool[fp.headerTable + t * fp.headerStride];

@@ -107,7 +107,8 @@ struct FusedProgram {
     size_t stagedLdsBytes = 0;  // split form, element launch: staged arguments and parameters
     size_t imageDoubles = 0;    // split form: element images of all tiles (after the inputs)
     // element tiles: copies of the headers in launch order at pool[headerTable + t * headerStride]
-    // with the relative offsets (H[4..7], node lists) replaced by absolute pool indices
+    // (headerTable = 1 + numTiles, right after the launch order) with the relative offsets
+    // (H[4..7], node lists) replaced by absolute pool indices
     size_t headerTable = 0, headerStride = 0;
     size_t elementEvaluations = 0, numElements = 0, numContributions = 0;
     double recompute() const { return numElements ? (double)elementEvaluations / (double)numElements : 0.0; }

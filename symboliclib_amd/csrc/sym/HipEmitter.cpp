@@ -1067,7 +1067,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
         os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
         if (variant != 2) stamp(os, 0);
-        os << "        const INDEX* __restrict__ H = tiles + " << fp.headerTable << "u + it * " << fp.headerStride << "u;  // launch-order header\n";
+        os << "        const INDEX* __restrict__ H = tiles + 1u + numTiles + it * " << fp.headerStride << "u;  // launch-order header\n";
         os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10], nconst = H[11];\n";
         std::string vsum = "0u", ssum = "0u";
         for (size_t t = 0; t < NT; ++t) {
