@@ -36,9 +36,9 @@ struct TemplateLayout {
     std::vector<char> uniform;
     std::vector<int> gatherNode, gatherDof;
     int numNodes = 0, numParams = 0, numOutputs = 0;
-    // LDS doubles per element: staged arguments and parameters, then its outputs
     int valueStride() const { return numOutputs | 1; }
-    int ldsPerElement() const { return (int)gatherNode.size() + numParams + valueStride(); }
+    // LDS bytes of one element besides its nodes' coordinates: parameters, values, slots
+    size_t elementBytes() const { return 8 * (size_t)(numParams + valueStride()) + 2 * (size_t)numNodes; }
 };
 
 // Tiles of one instance, pool offsets relative to the instance's chunk.
@@ -154,11 +154,14 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         ft.numParams = tl.numParams;
         ft.numOutputs = tl.numOutputs;
         ft.valueStride = tl.valueStride();
-        if ((size_t)tl.ldsPerElement() * 8 > fo.ldsBytes)
-            throw std::invalid_argument("fused form: one element needs more LDS than a tile holds");
+        for (int d : tl.gatherDof) fp.stagedDofs = std::max(fp.stagedDofs, d + 1);
         fp.templates.push_back(std::move(ft));
     }
     const size_t NT = L.size();
+    const size_t nodeLds = 8 * (size_t)fp.stagedDofs;  // LDS bytes per staged tile node
+    for (auto& tl : L)
+        if (tl.elementBytes() + nodeLds * (size_t)tl.numNodes > fo.ldsBytes)
+            throw std::invalid_argument("fused form: one element needs more LDS than a tile holds");
     timer("fused: templates");
 
     // ---- per instance: pattern and tiles
@@ -169,7 +172,8 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
     // node by an upper bound of what it adds (its count bytes, u16 slots, constants, two run
     // words per dof column and its share of the 64-output chunk records), so the tile built
     // never needs more than the budget.
-    const double ldsBudget = (double)fo.ldsBytes - 64.0;  // run terminator, chunk and word rounding
+    // margin: run terminator, chunk and word rounding, slot table padding per template
+    const double ldsBudget = (double)fo.ldsBytes - 64.0 - 16.0 * (double)L.size();
     parallelFor(inst_.size(), [&](size_t k) {
         const Instance& I = inst_[k];
         InstancePattern P;
@@ -222,6 +226,9 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         std::vector<char> assigned((size_t)N, 0);
         for (int b = 0; b < N; ++b) assigned[b] = outCount[b] == 0;
         std::vector<int64_t> stamp(E, -1);
+        std::vector<int64_t> nodeStamp((size_t)N, -1), nodeStep((size_t)N, -1);
+        std::vector<uint32_t> localNode((size_t)N, 0), tileNodeList, newNodes;
+        int64_t step = 0;
         std::vector<uint32_t> local(E, 0);
         std::vector<std::vector<uint32_t>> buckets;
         std::vector<size_t> heads;
@@ -251,8 +258,9 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             std::fill(heads.begin(), heads.end(), 0);
             tNodes.clear();
             for (auto& te : tElems) te.clear();
-            size_t slots = 0, outs = 0;
+            size_t elemBytes = 0, outs = 0;
             double recipeBytes = 0.0;
+            tileNodeList.clear();
             push((uint32_t)cursor, cost((uint32_t)cursor, tileId));
             for (;;) {
                 // pop the cheapest frontier node (FIFO within a cost)
@@ -281,17 +289,34 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                     push(v, cur);
                     continue;
                 }
-                size_t newSlots = 0;
-                for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q)
-                    if (stamp[inc[q]] != tileId) newSlots += (size_t)L[elemT[e0 + inc[q]]].ldsPerElement();
-                const bool fits = 8.0 * (double)(slots + newSlots) + recipeBytes + nodeBytes[v] <= ldsBudget;
+                // LDS the node adds: its new elements, the nodes they bring in, its outputs
+                size_t newBytes = 0;
+                newNodes.clear();
+                ++step;
+                for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q) {
+                    const uint32_t i = inc[q];
+                    if (stamp[i] == tileId) continue;
+                    newBytes += L[elemT[e0 + i]].elementBytes();
+                    for (uint32_t j = enStart[i]; j < enStart[i + 1]; ++j) {
+                        const uint32_t n = en[j];
+                        if (nodeStamp[n] == tileId || nodeStep[n] == step) continue;
+                        nodeStep[n] = step;
+                        newNodes.push_back(n);
+                    }
+                }
+                const double need = (double)(elemBytes + newBytes + nodeLds * (tileNodeList.size() + newNodes.size()));
+                const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget;
                 if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs)) break;
                 if (!fits) throw std::invalid_argument("fused form: one node and its elements need more LDS than a tile holds");
                 assigned[v] = 1;
                 tNodes.push_back(v);
                 outs += outCount[v];
-                slots += newSlots;
+                elemBytes += newBytes;
                 recipeBytes += nodeBytes[v];
+                for (uint32_t n : newNodes) {
+                    nodeStamp[n] = tileId;
+                    tileNodeList.push_back(n);
+                }
                 for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q) {
                     const uint32_t i = inc[q];
                     if (stamp[i] == tileId) continue;
@@ -304,6 +329,9 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
 
             // ---- emit the tile
             std::sort(tNodes.begin(), tNodes.end());
+            std::sort(tileNodeList.begin(), tileNodeList.end());
+            if (tileNodeList.size() > 65536) throw std::logic_error("fused form: tile nodes exceed 16-bit slots");
+            for (size_t q = 0; q < tileNodeList.size(); ++q) localNode[tileNodeList[q]] = (uint32_t)q;
             std::vector<size_t> ldsBase(NT + 1, 0);
             for (size_t t = 0; t < NT; ++t) {
                 std::sort(tElems[t].begin(), tElems[t].end());
@@ -404,18 +432,21 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             pool[H + 10] = chunksOff + (uint32_t)(CW * nch) - runsOff;
             pool[H + 11] = (uint32_t)cvals.size() * (anyConst ? 1u : 0u);
             pool[H + 12] = 0;  // element image, set at merge
+            pool[H + 13] = (uint32_t)tileNodeList.size();
+            pool[H + 14] = appendWords(tileNodeList.data(), tileNodeList.size() * 4);
             size_t evals = 0;
             for (size_t t = 0; t < NT; ++t) {
                 const auto& te = tElems[t];
                 const TemplateLayout& tl = L[t];
                 const size_t cnt = te.size();
                 evals += cnt;
-                std::vector<uint32_t> nodes((size_t)tl.numNodes * cnt);
+                std::vector<uint16_t> slotTab((size_t)tl.numNodes * cnt);
                 for (size_t q = 0; q < cnt; ++q)
-                    for (int j = 0; j < tl.numNodes; ++j) nodes[(size_t)j * cnt + q] = en[enStart[te[q]] + (uint32_t)j];
-                const uint32_t nodesOff = appendWords(nodes.data(), nodes.size() * 4);
+                    for (int j = 0; j < tl.numNodes; ++j)
+                        slotTab[(size_t)j * cnt + q] = (uint16_t)localNode[en[enStart[te[q]] + (uint32_t)j]];
+                const uint32_t slotOff = appendWords(slotTab.data(), slotTab.size() * 2);
                 pool[H + HEADER + 3 * t] = (uint32_t)cnt;
-                pool[H + HEADER + 3 * t + 1] = nodesOff;
+                pool[H + HEADER + 3 * t + 1] = slotOff;
                 pool[H + HEADER + 3 * t + 2] = u32(out.consts.size(), "constants");
                 const size_t pb = out.consts.size();
                 out.consts.resize(pb + (size_t)tl.numParams * cnt);
@@ -425,8 +456,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                         if (tl.argParam[g] >= 0) out.consts[pb + (size_t)tl.argParam[g] * cnt + q] = a[g].value;
                 }
             }
-            size_t staged = 0;
-            for (size_t t = 0; t < NT; ++t) staged += tElems[t].size() * (size_t)(L[t].gatherNode.size() + L[t].numParams);
+            const size_t staged = fusedStagedDoubles(fp, &pool[H]);
             const size_t tileLds = 8 * (ldsBase[NT] + 1 + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * (size_t)pool[H + 11];
             out.cost.push_back((double)m + (double)ldsBase[NT]);
             out.values.push_back(ldsBase[NT]);
@@ -547,9 +577,9 @@ std::vector<float> tileStats(const FusedProgram& fp) {
         for (size_t q = 0; q < NT; ++q) {
             const size_t cnt = P[H + HEADER + 3 * q];
             elems += cnt;
-            staged += cnt * (fp.templates[q].argNode.size() + (size_t)fp.templates[q].numParams);
             vals += cnt * (size_t)fp.templates[q].valueStride;
         }
+        if (fp.kind == 0) staged = fusedStagedDoubles(fp, &P[H]);
         while (P[H + P[H + 4] + 2 * runs + 1] != m) ++runs;
         float* f = &out[t * TILE_STATS];
         f[0] = (float)m;
@@ -562,6 +592,16 @@ std::vector<float> tileStats(const FusedProgram& fp) {
         f[7] = (float)runs;
     }
     return out;
+}
+
+size_t fusedStagedDoubles(const FusedProgram& fp, const uint32_t* H) {
+    size_t params = 0, words = 0;
+    for (size_t t = 0; t < fp.templates.size(); ++t) {
+        const size_t cnt = H[FUSED_HEADER + 3 * t];
+        params += (size_t)fp.templates[t].numParams * cnt;
+        words += ((size_t)fp.templates[t].numNodes * cnt + 1) / 2;
+    }
+    return params + (size_t)fp.stagedDofs * H[13] + (words + 1) / 2;
 }
 
 void orderTiles(FusedProgram& fp, FusedSchedule schedule) {
@@ -592,10 +632,7 @@ void orderTiles(FusedProgram& fp, FusedSchedule schedule) {
     for (size_t t = 0; t < n; ++t) {
         const size_t H = fp.pool[1 + t];
         uint32_t* dst = &fp.pool[fp.headerTable + t * fp.headerStride];
-        for (size_t w = 0; w < words; ++w) {
-            const bool rel = (w >= 4 && w <= 7) || (w >= (size_t)FUSED_HEADER && (w - FUSED_HEADER) % 3 == 1);
-            dst[w] = rel ? (uint32_t)(H + fp.pool[H + w]) : fp.pool[H + w];
-        }
+        for (size_t w = 0; w < words; ++w) dst[w] = fusedHeaderOffset(w) ? (uint32_t)(H + fp.pool[H + w]) : fp.pool[H + w];
     }
 }
 
@@ -666,17 +703,22 @@ void validateFused(const FusedProgram& fp) {
         need(P[H + 10] == P[H + 7] + CW * nch - P[H + 4] && P[H + 4] <= P[H + 5] && P[H + 5] <= P[H + 6] &&
                  P[H + 6] <= P[H + 7],
              "recipe tables not contiguous");
+        // tile nodes: sorted, inside the instance, every staged dof inside the inputs
+        const size_t nn = P[H + 13], tn = H + P[H + 14];
+        need(nn <= 65536 && tn + nn <= P.size(), "tile node list outside the pool");
+        for (size_t i = 0; i < nn; ++i) need(P[tn + i] < N && (i == 0 || P[tn + i - 1] < P[tn + i]), "tile node list");
+        need((size_t)B + (size_t)fp.stagedDofs * N <= fp.numInputs, "staged dofs outside the inputs");
         for (size_t q = 0; q < NT; ++q) {
             const FusedTemplate& ft = fp.templates[q];
-            const size_t cnt = P[H + HEADER + 3 * q], nodes = H + P[H + HEADER + 3 * q + 1], prm = P[H + HEADER + 3 * q + 2];
-            need(nodes + (size_t)ft.numNodes * cnt <= P.size(), "node list outside the pool");
+            const size_t cnt = P[H + HEADER + 3 * q], tab = H + P[H + HEADER + 3 * q + 1], prm = P[H + HEADER + 3 * q + 2];
+            need(tab + ((size_t)ft.numNodes * cnt + 1) / 2 <= P.size(), "slot table outside the pool");
             need(prm + (size_t)ft.numParams * cnt <= fp.consts.size(), "parameters outside the constants");
-            for (size_t i = 0; i < (size_t)ft.numNodes * cnt; ++i) need(P[nodes + i] < N, "node outside its instance");
-            for (size_t g = 0; g < ft.argDof.size(); ++g)
-                need((size_t)B + (size_t)ft.argDof[g] * N + N <= fp.numInputs, "gather outside the inputs");
+            const uint16_t* st = reinterpret_cast<const uint16_t*>(&P[tab]);
+            for (size_t i = 0; i < (size_t)ft.numNodes * cnt; ++i) need(st[i] < nn, "slot outside the tile's nodes");
+            for (size_t g = 0; g < ft.argDof.size(); ++g) need(ft.argDof[g] < fp.stagedDofs, "argument dof not staged");
             lds += cnt * (size_t)ft.valueStride;
-            staged += cnt * (ft.argNode.size() + (size_t)ft.numParams);
         }
+        staged = fusedStagedDoubles(fp, &P[H]);
         need(lds <= 65536, "tile LDS values exceed 16-bit slots");
         need((size_t)P[H + 12] >= fp.numInputs && (size_t)P[H + 12] + lds <= fp.numInputs + fp.imageDoubles,
              "element image outside the value buffer");
@@ -729,10 +771,9 @@ void validateFused(const FusedProgram& fp) {
     need(fp.headerStride >= words && fp.headerTable + fp.numTiles * fp.headerStride <= P.size(), "header table outside the pool");
     for (size_t t = 0; t < fp.numTiles; ++t) {
         const size_t H = P[1 + t];
-        for (size_t w = 0; w < words; ++w) {
-            const bool rel = (w >= 4 && w <= 7) || (w >= (size_t)FUSED_HEADER && (w - FUSED_HEADER) % 3 == 1);
-            need(P[fp.headerTable + t * fp.headerStride + w] == (rel ? H + P[H + w] : P[H + w]), "header table entry");
-        }
+        for (size_t w = 0; w < words; ++w)
+            need(P[fp.headerTable + t * fp.headerStride + w] == (fusedHeaderOffset(w) ? H + P[H + w] : P[H + w]),
+                 "header table entry");
     }
 }
 

@@ -9,8 +9,9 @@
 //
 // Node layout: instance variables are numNodes x dofs, column-major (variable = node + dof *
 // numNodes), the layout of an n x 3 position matrix; an element's variables are dofs of its
-// nodes, so a tile gathers them as x[base + node + dof * numNodes] from a per-element node
-// list.
+// nodes.  A tile stages the coordinates of its nodes (every node of its elements, each once)
+// as x[base + node + dof * numNodes] from a node list, and its elements reach them through
+// u16 local node slots.
 //
 // Tile pool (u32 words, read by the generated kernel and checked by validateFused):
 //   pool[0]              number of tiles
@@ -31,12 +32,16 @@
 //     H[10] words from runs to the end of chunks (the recipe, contiguous)
 //     H[11] number of output constants
 //     H[12] the tile's element image in the value buffer (split form: first value, in doubles)
-//     H[13 + 3 t], H[14 + 3 t], H[15 + 3 t]: template t's element count, rel node list
-//       (numNodes_t columns of count), parameter block in c[] (numParams_t columns of count)
+//     H[13] nn, nodes of the tile's elements    H[14] rel tile node list (nn u32, sorted)
+//     H[15 + 3 t], H[16 + 3 t], H[17 + 3 t]: template t's element count, rel slot table
+//       (u16 local node slots, numNodes_t columns of count, packed two per word), parameter
+//       block in c[] (numParams_t columns of count)
 //   LDS of a tile, in doubles: element values (template t's outputs at V_t + i * stride_t
-//   + slot, V_t = sum over t' < t of count_t' * stride_t', stride = valueStride), then the staged element
-//   arguments and parameters (template t: argument k at A_t + k * count_t + i), then the
-//   recipe (u32 words copied from H[4]), then the output constants.
+//   + slot, V_t = sum over t' < t of count_t' * stride_t', stride = valueStride), the zero
+//   V[nvals], then the staged region: parameters (template t: parameter k at P_t + k *
+//   count_t + i), node coordinates (dof d of local node s at X + d * nn + s, stagedDofs
+//   dofs), slot tables (template t's words at W_t), then the recipe (u32 words copied from
+//   H[4]), then the output constants.
 #pragma once
 
 #include <cstdint>
@@ -110,6 +115,7 @@ struct FusedProgram {
     // (headerTable = 1 + numTiles, right after the launch order) with the relative offsets
     // (H[4..7], node lists) replaced by absolute pool indices
     size_t headerTable = 0, headerStride = 0;
+    int stagedDofs = 0;  // dofs staged per tile node: 1 + the largest dof an element reads
     size_t elementEvaluations = 0, numElements = 0, numContributions = 0;
     double recompute() const { return numElements ? (double)elementEvaluations / (double)numElements : 0.0; }
     std::string summary() const;
@@ -118,7 +124,13 @@ struct FusedProgram {
 // Per tile in launch order, TILE_STATS floats: owned outputs, element contributions, elements,
 // staged doubles, element LDS values, recipe words, output constants, runs.
 constexpr int TILE_STATS = 8;
-constexpr int FUSED_HEADER = 13;  // fixed header words of an element tile
+constexpr int FUSED_HEADER = 15;  // fixed header words of an element tile
+// Header words holding pool offsets (relative in a tile's header, absolute in the table).
+inline bool fusedHeaderOffset(size_t w) {
+    return (w >= 4 && w <= 7) || w == 14 || (w >= (size_t)FUSED_HEADER && (w - FUSED_HEADER) % 3 == 1);
+}
+// Doubles of a tile's staged region (parameters, node coordinates, slot tables) from its header.
+size_t fusedStagedDoubles(const FusedProgram& fp, const uint32_t* H);
 constexpr int FUSED_CHUNK_WORDS = 5;  // words per 64-output chunk record of an element tile
 std::vector<float> tileStats(const FusedProgram& fp);
 

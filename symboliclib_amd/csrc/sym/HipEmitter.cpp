@@ -967,42 +967,6 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
     std::vector<int> groupsOf(NT);
     for (size_t t = 0; t < NT; ++t) {
         const FusedTemplate& ft = fp.templates[t];
-        const size_t nargs = ft.argNode.size();
-        // A0: stage arguments and parameters of one element
-        os << "__device__ __forceinline__ void symlib_stage" << t
-           << "(const double* __restrict__ x, unsigned B, unsigned N, const INDEX* __restrict__ nd,\n"
-           << "    const double* __restrict__ pr, unsigned cnt, unsigned i, double* __restrict__ A) {\n";
-        for (int j = 0; j < ft.numNodes; ++j) os << "    const unsigned n" << j << " = nd[" << j << "u * cnt + i];\n";
-        for (size_t g = 0; g < nargs; ++g) {
-            os << "    A[" << g << "u * cnt + i] = x[B + n" << ft.argNode[g];
-            if (ft.argDof[g]) os << " + " << ft.argDof[g] << "u * N";
-            os << "];\n";
-        }
-        for (int q = 0; q < ft.numParams; ++q)
-            os << "    A[" << nargs + (size_t)q << "u * cnt + i] = pr[" << q << "u * cnt + i];\n";
-        os << "}\n\n";
-        if (opt.ldsDma) {
-            // one wave stages node slot j of 32 consecutive elements: its lanes load the node
-            // numbers (two lanes per element) and issue one direct load per gathered argument
-            // of that slot (low and high dwords of 32 doubles)
-            os << "__device__ __forceinline__ void symlib_stagew" << t
-               << "(const double* __restrict__ x, unsigned B, unsigned N, const INDEX* __restrict__ nd,\n"
-               << "    unsigned cnt, unsigned nb, unsigned v, double* A, unsigned lane) {\n"
-               << "    const unsigned j = v / nb, i0 = (v - j * nb) << 5, i = i0 + (lane >> 1);\n"
-               << "    if (i >= cnt) return;\n"
-               << "    const unsigned* s = (const unsigned*)(x + B + nd[j * cnt + i]) + (lane & 1u);\n";
-            for (int j = 0; j < ft.numNodes; ++j) {
-                os << "    " << (j ? "else " : "") << "if (j == " << j << "u) {\n";
-                for (size_t g = 0; g < nargs; ++g)
-                    if (ft.argNode[g] == j) {
-                        os << "        SYMLIB_LDS_DMA4(s";
-                        if (ft.argDof[g]) os << " + " << 2 * ft.argDof[g] << "u * N";
-                        os << ", A + " << g << "u * cnt + i0);\n";
-                    }
-                os << "    }\n";
-            }
-            os << "}\n\n";
-        }
         // A1: output groups, statements in a low-pressure order
         const std::vector<Stmt> S = flatten(ft.code, opt.shareReciprocals, opt.foldCoefficients);
         std::vector<size_t> outs;
@@ -1034,11 +998,24 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 if (keep[i])
                     for (int u : S[i].uses)
                         if (u >= ARG) leaves.insert(u);
+            // leaves: arguments through the element's local node slots (E, numNodes columns of
+            // cnt) into the tile's node coordinates (X, dof-major, nn per dof); parameters
+            // from Q (columns of cnt)
             os << "__device__ __forceinline__ void symlib_elem" << t << "_" << g
-               << "(const double* __restrict__ A, unsigned cnt, unsigned i, double* __restrict__ L) {\n";
+               << "(const double* __restrict__ X, unsigned nn, const unsigned short* __restrict__ E,\n"
+               << "    const double* __restrict__ Q, unsigned cnt, unsigned i, double* __restrict__ L) {\n"
+               << "    (void)X; (void)nn; (void)E; (void)Q;\n";
+            std::set<int> nodesUsed;
+            for (int u : leaves)
+                if (u < PARAM) nodesUsed.insert(ft.argNode[(size_t)(u - ARG)]);
+            for (int j : nodesUsed) os << "    const unsigned e" << j << " = E[" << j << "u * cnt + i];\n";
             for (int u : leaves) {
-                if (u < PARAM) os << "    const double a" << u - ARG << " = A[" << u - ARG << "u * cnt + i];\n";
-                else os << "    const double q" << u - PARAM << " = A[" << nargs + (size_t)(u - PARAM) << "u * cnt + i];\n";
+                if (u < PARAM) {
+                    const size_t a = (size_t)(u - ARG);
+                    os << "    const double a" << a << " = X[" << ft.argDof[a] << "u * nn + e" << ft.argNode[a] << "];\n";
+                } else {
+                    os << "    const double q" << u - PARAM << " = Q[" << u - PARAM << "u * cnt + i];\n";
+                }
             }
             KernelEmitter ke{noProgram, noKernel, false};
             ke.fusedT = &ft;
@@ -1069,77 +1046,78 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         if (variant != 2) stamp(os, 0);
         os << "        const INDEX* __restrict__ H = tiles + 1u + numTiles + it * " << fp.headerStride << "u;  // launch-order header\n";
         os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10], nconst = H[11];\n";
-        std::string vsum = "0u", ssum = "0u";
+        std::string vsum = "0u", psum = "0u", wsum = "0u";
+        std::vector<std::string> pbase(NT), wbase(NT);
         for (size_t t = 0; t < NT; ++t) {
             const size_t h = FUSED_HEADER + 3 * t;
             const FusedTemplate& ft = fp.templates[t];
-            os << "        const unsigned cnt" << t << " = H[" << h << "];\n";
-            vsum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.valueStride) + "u";
-            ssum += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.argNode.size() + (size_t)ft.numParams) + "u";
+            const std::string cnt = "cnt" + std::to_string(t);
+            os << "        const unsigned " << cnt << " = H[" << h << "];\n";
+            pbase[t] = psum;
+            wbase[t] = wsum;
+            vsum += " + " + cnt + " * " + std::to_string(ft.valueStride) + "u";
+            psum += " + " + cnt + " * " + std::to_string(ft.numParams) + "u";
+            wsum += " + ((" + cnt + " * " + std::to_string(ft.numNodes) + "u + 1u) >> 1)";
         }
-        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)lane; (void)wv;\n";
-        os << "        const unsigned nvals = " << vsum << ", nstaged = " << ssum << ";\n";
+        os << "        const unsigned nn = H[13];\n";
+        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)lane; (void)wv; (void)nn;\n";
+        // staged region: parameters, node coordinates (stagedDofs per node), slot tables
+        os << "        const unsigned np = " << psum << ", nsw = " << wsum << ";\n";
+        os << "        const unsigned nvals = " << vsum << ", nstaged = np + " << fp.stagedDofs << "u * nn + ((nsw + 1u) >> 1);\n";
         // V[nvals]: the tile's zero (odd contribution lists are padded with its slot)
         if (variant == 0) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
         if (variant == 1) os << "        double* __restrict__ V = y + H[12];\n        double* __restrict__ Ast = lds;\n";
         if (variant == 2) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
         os << "        INDEX* __restrict__ R = (INDEX*)(Ast + " << (variant == 2 ? "0u" : "nstaged") << ");\n";
         os << "        double* __restrict__ K = (double*)(R + ((words + 1u) & ~1u));\n";
-        os << "        (void)nvals; (void)nstaged; (void)R; (void)K; (void)V;\n";
+        os << "        double* __restrict__ Xs = Ast + np;\n"
+           << "        unsigned* __restrict__ Es = (unsigned*)(Xs + " << fp.stagedDofs << "u * nn);\n";
+        os << "        (void)nvals; (void)nstaged; (void)R; (void)K; (void)V; (void)Xs; (void)Es;\n";
         os << "        {\n";
         if (sums) os << "            if (threadIdx.x == 0u) V[nvals] = 0.0;\n";
         if (stage && opt.ldsDma) {
-            // A0 with direct loads: one loop over (template, node slot, 32 elements) units, one
-            // wave each, then the parameter columns of each template (contiguous in `c`)
-            std::string total = "0u";
-            for (size_t t = 0; t < NT; ++t) {
-                os << "            const unsigned nb" << t << " = (cnt" << t << " + 31u) >> 5;\n";
-                total += " + " + std::to_string(fp.templates[t].numNodes) + "u * nb" + std::to_string(t);
+            // A0 with direct loads: each wave takes 32 tile nodes at a time (two lanes per node:
+            // low and high dword) and issues one direct load per staged dof; the parameter
+            // columns (contiguous in c) and the slot tables (contiguous in the pool) are copied
+            os << "            const INDEX* __restrict__ tn = tiles + H[14];\n"
+               << "            for (unsigned u = wv; u < ((nn + 31u) >> 5); u += " << T / 64 << "u) {\n"
+               << "                const unsigned s = (u << 5) + (lane >> 1);\n"
+               << "                if (s < nn) {\n"
+               << "                    const unsigned* src = (const unsigned*)(x + B + tn[s]) + (lane & 1u);\n";
+            for (int d = 0; d < fp.stagedDofs; ++d) {
+                os << "                    SYMLIB_LDS_DMA4(src";
+                if (d) os << " + " << 2 * d << "u * N";
+                os << ", Xs + " << d << "u * nn + (u << 5));\n";
             }
-            os << "            for (unsigned u = wv; u < " << total << "; u += " << T / 64 << "u) {\n"
-               << "                unsigned v = u, Sb = 0;\n";
-            for (size_t t = 0; t < NT; ++t) {
-                const size_t h = FUSED_HEADER + 3 * t;
-                const FusedTemplate& ft = fp.templates[t];
-                const std::string units = std::to_string(ft.numNodes) + "u * nb" + std::to_string(t);
-                os << "                if (v < " << units << ") {\n"
-                   << "                    symlib_stagew" << t << "(x, B, N, tiles + H[" << h + 1 << "], cnt" << t << ", nb" << t
-                   << ", v, Ast + Sb, lane);\n"
-                   << "                    continue;\n"
-                   << "                }\n"
-                   << "                v -= " << units << ";\n"
-                   << "                Sb += cnt" << t << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
-            }
-            os << "            }\n";
-            std::string sb = "0u";
+            os << "                }\n"
+               << "            }\n";
             for (size_t t = 0; t < NT; ++t) {
                 const size_t h = FUSED_HEADER + 3 * t;
                 const FusedTemplate& ft = fp.templates[t];
                 if (ft.numParams > 0)
-                    os << "            symlib_lds_copy32((const unsigned*)(c + H[" << h + 2 << "]), (unsigned*)(Ast + " << sb
-                       << " + cnt" << t << " * " << ft.argNode.size() << "u), " << 2 * ft.numParams << "u * cnt" << t
-                       << ", wv, lane);\n";
-                sb += " + cnt" + std::to_string(t) + " * " + std::to_string(ft.argNode.size() + (size_t)ft.numParams) + "u";
+                    os << "            symlib_lds_copy32((const unsigned*)(c + H[" << h + 2 << "]), (unsigned*)(Ast + " << pbase[t]
+                       << "), " << 2 * ft.numParams << "u * cnt" << t << ", wv, lane);\n";
+                os << "            symlib_lds_copy32(tiles + H[" << h + 1 << "], Es + " << wbase[t] << ", (cnt" << t << " * "
+                   << ft.numNodes << "u + 1u) >> 1, wv, lane);\n";
             }
         } else if (stage) {
-            // A0: one loop over the elements of every template (an if-chain picks the template),
-            // so a tile's templates share one pass of the workgroup
-            std::string total = "0u";
-            for (size_t t = 0; t < NT; ++t) total += " + cnt" + std::to_string(t);
-            os << "            for (unsigned e = threadIdx.x; e < " << total << "; e += " << T << "u) {\n"
-               << "                unsigned i = e, Sb = 0;\n";
+            // A0 through registers: node coordinates, parameters and slot tables, one value per
+            // thread and step
+            os << "            const INDEX* __restrict__ tn = tiles + H[14];\n"
+               << "            for (unsigned s = threadIdx.x; s < nn; s += " << T << "u) {\n"
+               << "                const double* __restrict__ src = x + B + tn[s];\n";
+            for (int d = 0; d < fp.stagedDofs; ++d)
+                os << "                Xs[" << d << "u * nn + s] = src[" << d << "u * N];\n";
+            os << "            }\n";
             for (size_t t = 0; t < NT; ++t) {
                 const size_t h = FUSED_HEADER + 3 * t;
                 const FusedTemplate& ft = fp.templates[t];
-                os << "                if (i < cnt" << t << ") {\n"
-                   << "                    symlib_stage" << t << "(x, B, N, tiles + H[" << h + 1 << "], c + H[" << h + 2 << "], cnt" << t
-                   << ", i, Ast + Sb);\n"
-                   << "                    continue;\n"
-                   << "                }\n"
-                   << "                i -= cnt" << t << ";\n"
-                   << "                Sb += cnt" << t << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
+                if (ft.numParams > 0)
+                    os << "            for (unsigned q = threadIdx.x; q < " << ft.numParams << "u * cnt" << t << "; q += " << T
+                       << "u) Ast[" << pbase[t] << " + q] = c[H[" << h + 2 << "] + q];\n";
+                os << "            for (unsigned q = threadIdx.x; q < ((cnt" << t << " * " << ft.numNodes << "u + 1u) >> 1); q += " << T
+                   << "u) Es[" << wbase[t] << " + q] = tiles[H[" << h + 1 << "] + q];\n";
             }
-            os << "            }\n";
         }
         if (opt.ldsDma) {
             if (variant == 2) os << "            symlib_lds_copy32((const unsigned*)(x + H[12]), (unsigned*)V, 2u * nvals, wv, lane);\n";
@@ -1179,7 +1157,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 os << "            symlib_lds_copy32(tiles + H[4], R, words, wv, lane);\n"
                    << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + H[8]), (unsigned*)K, 2u * nconst, wv, lane);\n";
             os << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
-               << "                unsigned j = w, Lb = 0, Sb = 0;\n";
+               << "                unsigned j = w, Lb = 0;\n";
             for (size_t t = 0; t < NT; ++t) {
                 const FusedTemplate& ft = fp.templates[t];
                 const std::string ts = std::to_string(t), cnt = "cnt" + ts, pc = "pc" + ts;
@@ -1190,14 +1168,13 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                    << "                    if (i < " << cnt << ") {\n";
                 for (int g = 0; g < groupsOf[t]; ++g)
                     os << "                        " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
-                       << "symlib_elem" << t << "_" << g << "(Ast + Sb, " << cnt << ", i, V + Lb + i * " << ft.valueStride
-                       << "u);\n";
+                       << "symlib_elem" << t << "_" << g << "(Xs, nn, (const unsigned short*)(Es + " << wbase[t] << "), Ast + "
+                       << pbase[t] << ", " << cnt << ", i, V + Lb + i * " << ft.valueStride << "u);\n";
                 os << "                    }\n"
                    << "                    continue;\n"
                    << "                }\n"
                    << "                j -= " << groupsOf[t] << "u * " << pc << ";\n"
-                   << "                Lb += " << cnt << " * " << ft.valueStride << "u;\n"
-                   << "                Sb += " << cnt << " * " << ft.argNode.size() + (size_t)ft.numParams << "u;\n";
+                   << "                Lb += " << cnt << " * " << ft.valueStride << "u;\n";
             }
             os << "            }\n"
                << "        }\n";
