@@ -244,9 +244,18 @@ void ComputeUnitBase::retuneFused(const FusedOptions& fo) {
     std::vector<LaunchDesc> ls;
     FusedCodeStats st;
     std::string src = emitHipFused(fusedProgram, fo, ls, &st);
-    orderTiles(fusedProgram, fo.schedule);
+    // the launch order (and the element form's header table after it) changes only between
+    // order classes: build order (static, queue), largest first, interleaved (and strided)
+    auto orderClass = [](FusedSchedule s) {
+        return s == FusedSchedule::QueueLargest ? 1 : (s == FusedSchedule::Interleaved || s == FusedSchedule::Strided) ? 2 : 0;
+    };
+    const bool reorder = orderClass(fo.schedule) != orderClass(fusedOptions.schedule);
+    if (reorder) orderTiles(fusedProgram, fo.schedule);
     if (rt) {
-        check(symhip_update_tiles(rt, 1, fusedProgram.numTiles, fusedProgram.pool.data() + 1), "symhip_update_tiles");
+        if (reorder) {
+            const size_t words = fusedProgram.numTiles + (fusedProgram.kind == 0 ? fusedProgram.numTiles * fusedProgram.headerStride : 0);
+            check(symhip_update_tiles(rt, 1, words, fusedProgram.pool.data() + 1), "symhip_update_tiles");
+        }
         std::vector<symhip_launch> L;
         for (auto& l : ls) L.push_back({l.name.c_str(), l.grid, l.block, l.shared});
         check(symhip_set_launches(rt, (uint32_t)L.size(), L.data()), "symhip_set_launches");

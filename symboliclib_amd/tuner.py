@@ -1,10 +1,10 @@
 """Autotuner of the fused assembly form: tile shape, output groups, register target, block size
 and schedule, chosen by exhaustive timing or by the learned cost model (costmodel.py).
 
-Search space (SPACE): tile LDS budget (the tile shape: how many owner nodes a tile takes),
-output groups per element, minimum waves per SIMD, threads per block, and the schedule
-(one workgroup per tile; persistent strided; persistent work queue in build order or largest
-tiles first).  A tile budget needs its own build of the tiles; everything else is a retune of
+Search space (SPACE): tile LDS budget (the whole workgroup's LDS, which sets the tile shape:
+how many owner nodes a tile takes), output groups per element, minimum waves per SIMD, threads
+per block, and the schedule (one workgroup per tile; persistent work queue in build order or
+largest tiles first).  A tile budget needs its own build of the tiles; everything else is a retune of
 the same unit (new code, new tile order), compiled ahead in parallel.
 
 Features (FEATURES, 24 per program and configuration) come from the tiles (outputs,
@@ -28,10 +28,10 @@ from . import _native
 
 SPACE = {
     "lds": [40448, 53760, 65024, 81408],  # whole-workgroup LDS: 4, 3, 2, 2 workgroups per CU
-    "groups": [3, 5, 9],
-    "wpe": [0, 4, 5],
-    "tpb": [256, 512, 1024],
-    "schedule": ["static", "strided", "queue", "largest"],
+    "groups": [2, 3, 5],
+    "wpe": [0, 4],
+    "tpb": [256, 512, 768, 1024],
+    "schedule": ["static", "queue", "largest"],
 }
 
 FEATURES = [

@@ -10,8 +10,8 @@ from symboliclib_amd import workloads as W
 
 def test_search_spaces():
     cfgs = list(TU.configs())
-    assert len(cfgs) == 4 * 3 * 3 * 3 * 4 == 432
-    assert len({json.dumps(c, sort_keys=True) for c in cfgs}) == 432
+    assert len(cfgs) == 4 * 3 * 2 * 4 * 3 == 288
+    assert len({json.dumps(c, sort_keys=True) for c in cfgs}) == 288
     assert len(list(TU.configs(TU.PRODUCT_SPACE))) == 96
 
 
@@ -23,10 +23,10 @@ def test_features_of_cloth_and_product_tiles():
     stats = TU.code_stats(u, cfgs)
     tiles = u.tile_stats()
     F = np.stack([TU.features(stats[(c["groups"], c["wpe"])], tiles, c) for c in cfgs])
-    assert F.shape == (108, len(TU.FEATURES)) and np.isfinite(F).all()
+    assert F.shape == (72, len(TU.FEATURES)) and np.isfinite(F).all()
     # more output groups: fewer values live per work item, more statements per element
-    assert stats[(9, 0)]["max_live"] < stats[(3, 0)]["max_live"]
-    assert stats[(9, 0)]["statements"] > stats[(3, 0)]["statements"]
+    assert stats[(5, 0)]["max_live"] < stats[(2, 0)]["max_live"]
+    assert stats[(5, 0)]["statements"] > stats[(2, 0)]["statements"]
     # tiles cover every output exactly once
     assert int(tiles[:, 0].sum()) == u.fused_stats()["num_outputs"]
     u.close()
