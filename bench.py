@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TUNED = os.path.join(ROOT, "symboliclib_amd", "tuning", "c4_selected.json")
+TRAFFIC = os.path.join(ROOT, "profiles", "symlib_fused_bw_hessian_batch256_traffic.json")
 
 
 def log(msg):
@@ -334,6 +335,14 @@ def run_batched(args, dev, stream, sp):
     copy_gbs = copy_bandwidth(dev, stream)
     achieved = balg / (kernel_ms * 1e-3) / 1e9
     value = n_out / (ms_per_step * 1e-3)
+    # HBM bytes per launch measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed
+    # under profiles/ (used when they were taken on this exact program and configuration)
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as f:
+            tr = json.load(f)
+        if tr.get("tiling") == info["tiling"] and tr.get("options") == opt:
+            traffic = tr["fetch_bytes"] + tr["write_bytes"]
     line = {
         "metric": "output nnz/s FP64, batched+learned-tuned vs plain HIP; HBM GB/s vs peak",
         "value": value,
@@ -349,7 +358,7 @@ def run_batched(args, dev, stream, sp):
                    "options": opt, "tuned_from": os.path.relpath(TUNED, ROOT) if tuned else None,
                    "tiling": info["tiling"], "grid": batched.launch_grid(0), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "+".join(names),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "+".join(names),
                      "algorithmic_bytes": balg, "kernel_ms": kernel_ms,
                      "launch_ms": dict(zip(names, launch_ms))},
         "plain_per_instance": plain_line,
