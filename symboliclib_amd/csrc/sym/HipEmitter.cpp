@@ -1079,8 +1079,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             // A0 with direct loads: each wave takes 32 tile nodes at a time (two lanes per node:
             // low and high dword) and issues one direct load per staged dof; the parameter
             // columns (contiguous in c) and the slot tables (contiguous in the pool) are copied
-            os << "            const INDEX* __restrict__ tn = tiles + H[14];\n"
-               << "            for (unsigned u = wv; u < ((nn + 31u) >> 5); u += " << T / 64 << "u) {\n"
+            os << "            const INDEX* __restrict__ tn = tiles + H[14];\n";
+            os << "            for (unsigned u = wv; u < ((nn + 31u) >> 5); u += " << T / 64 << "u) {\n"
                << "                const unsigned s = (u << 5) + (lane >> 1);\n"
                << "                if (s < nn) {\n"
                << "                    const unsigned* src = (const unsigned*)(x + B + tn[s]) + (lane & 1u);\n";
@@ -1181,6 +1181,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             if (opt.ldsDma && sums) os << "        __builtin_amdgcn_s_waitcnt(0);\n";
             if (sums) os << "        __syncthreads();\n";
             stamp(os, 2);
+
         }
         if (sums) {
             os << "        {\n"

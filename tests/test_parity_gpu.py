@@ -156,6 +156,8 @@ FUSED_CASES = [
     (33, "queue", 16384, 3, 0, True),
     (64, "strided", 49152, 5, 4, True),
     (256, "queue", 0, 5, 0, True),
+    (33, "strided", 0, 2, 0, False),
+    (256, "strided", 81408, 3, 0, False),
 ]
 
 
@@ -175,6 +177,19 @@ def test_bw_hessian_fused(require_device, side, schedule, lds, groups, wpe, spli
         u.execute(W.to_colmajor(X2))
         _check(f"Baraff-Witkin {side}x{side} fused second positions", u,
                O.bw_hessian(rest, F, X2, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results())
+
+
+@pytest.mark.parametrize("tpb", [64, 128])
+def test_bw_hessian_fused_strided_small_blocks(require_device, tpb):
+    # tiles with more nodes than threads: the prefetched node numbers cover the first tpb
+    # nodes, the rest are read in the staging loop
+    rest, F, X = W.c3_cloth(side=48)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, schedule="strided",
+                         tile_lds_bytes=81408, output_groups=2, threads_per_block=tpb) as u:
+        assert max(u.tile_stats()[:, 3]) > 0
+        u.compile().execute(W.to_colmajor(X))
+        _check(f"Baraff-Witkin 48x48 fused strided tpb={tpb}", u,
+               O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results())
 
 
 @pytest.mark.parametrize("schedule,split", [("static", False), ("queue", False), ("largest", False),
