@@ -212,13 +212,16 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         }
         std::vector<uint32_t> outCount((size_t)N, 0);
         std::vector<double> nodeBytes((size_t)N, 0.0);
+        // Constants of an instance with only nodal blocks (M (x) I_dofs) repeat in every dof
+        // column block of a tile (same values, same order): a tile then stores one period.
+        const double constPeriod = (I.cRow.empty() && D > 1) ? (double)D : 1.0;
         for (int b = 0; b < N; ++b) {
             double bytes = 8.0 * D;
             for (int d = 0; d < D; ++d) {
                 const int c = b + d * N;
                 outCount[b] += (uint32_t)(colptr[c + 1] - colptr[c]);
                 for (int q = colptr[c]; q < colptr[c + 1]; ++q)
-                    bytes += 2.0 * (P.count[q] + (P.count[q] & 1u)) + (P.hasConst[q] ? 8.0 : 0.0);
+                    bytes += 2.0 * (P.count[q] + (P.count[q] & 1u)) + (P.hasConst[q] ? 8.0 / constPeriod : 0.0);
             }
             nodeBytes[b] = bytes + outCount[b] * (1.0 + 4.0 * FUSED_CHUNK_WORDS / 64.0);
         }
@@ -422,15 +425,36 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             pool[H + 5] = countsOff;
             pool[H + 6] = slotsOff;
             pool[H + 7] = chunksOff;
+            // periodic constants: the outputs split into D equal dof blocks (columns b + d N
+            // for d = 0..D-1) and every block holds the same constant values in the same
+            // order; then one block's constants are stored and H[11] carries the period
+            uint32_t period = 0;
+            size_t stored = cvals.size();
+            if (D > 1 && D < 16 && m % (size_t)D == 0 && cvals.size() % (size_t)D == 0 && !cvals.empty()) {
+                const size_t per = cvals.size() / (size_t)D, blk = m / (size_t)D;
+                bool ok = true;
+                size_t k = 0;
+                for (int d = 0; d < D && ok; ++d) {
+                    size_t inBlock = 0;
+                    for (size_t o = (size_t)d * blk; o < (size_t)(d + 1) * blk; ++o) inBlock += counts[o] >> 7;
+                    ok = inBlock == per;
+                    for (size_t j = 0; j < per && ok; ++j, ++k) ok = cvals[k] == cvals[j];
+                }
+                if (ok) {
+                    period = (uint32_t)D;
+                    stored = per;
+                }
+            }
             if (anyConst) {
                 pool[H + 8] = u32(out.consts.size(), "constants");
-                out.consts.insert(out.consts.end(), cvals.begin(), cvals.end());
+                out.consts.insert(out.consts.end(), cvals.begin(), cvals.begin() + (long)stored);
             } else {
                 pool[H + 8] = NONE;
             }
             pool[H + 9] = 0;  // result base, set at merge
             pool[H + 10] = chunksOff + (uint32_t)(CW * nch) - runsOff;
-            pool[H + 11] = (uint32_t)cvals.size() * (anyConst ? 1u : 0u);
+            if (stored >= (1u << 28)) throw std::length_error("fused form: too many output constants in a tile");
+            pool[H + 11] = anyConst ? (uint32_t)stored | (period << 28) : 0u;
             pool[H + 12] = 0;  // element image, set at merge
             pool[H + 13] = (uint32_t)tileNodeList.size();
             pool[H + 14] = appendWords(tileNodeList.data(), tileNodeList.size() * 4);
@@ -457,7 +481,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 }
             }
             const size_t staged = fusedStagedDoubles(fp, &pool[H]);
-            const size_t tileLds = 8 * (ldsBase[NT] + 1 + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * (size_t)pool[H + 11];
+            const size_t tileLds = 8 * (ldsBase[NT] + 1 + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * stored;
             out.cost.push_back((double)m + (double)ldsBase[NT]);
             out.values.push_back(ldsBase[NT]);
             out.stagedLds = std::max(out.stagedLds, 8 * staged);
@@ -588,7 +612,7 @@ std::vector<float> tileStats(const FusedProgram& fp) {
         f[3] = (float)staged;
         f[4] = (float)vals;
         f[5] = (float)P[H + 10];
-        f[6] = (float)P[H + 11];
+        f[6] = (float)(P[H + 11] & 0x0fffffffu);
         f[7] = (float)runs;
     }
     return out;
@@ -730,9 +754,21 @@ void validateFused(const FusedProgram& fp) {
             S += cnts[o] & 0x7fu;
             numConst += cnts[o] >> 7;
         }
-        if (numConst) need(cOff != 0xffffffffu && (size_t)cOff + numConst <= fp.consts.size(), "output constants outside the constants");
-        need(P[H + 11] == numConst, "output constant count");
-        need(8 * (lds + 1 + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * numConst <= fp.ldsBytes,
+        // stored constants: all of them, or one period of D equal dof blocks
+        const size_t stored = P[H + 11] & 0x0fffffffu, period = P[H + 11] >> 28;
+        if (period) {
+            need(period > 1 && m % period == 0 && stored * period == numConst, "periodic output constants");
+            const size_t blk = m / period;
+            for (size_t d = 0; d < period; ++d) {
+                size_t inBlock = 0;
+                for (size_t o = d * blk; o < (d + 1) * blk; ++o) inBlock += cnts[o] >> 7;
+                need(inBlock == stored, "periodic output constants per dof block");
+            }
+        } else {
+            need(stored == numConst, "output constant count");
+        }
+        if (numConst) need(cOff != 0xffffffffu && (size_t)cOff + stored <= fp.consts.size(), "output constants outside the constants");
+        need(8 * (lds + 1 + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * stored <= fp.ldsBytes,
              "tile LDS exceeds the launch's LDS");
         need(slots + (S + 1) / 2 <= P.size(), "slot list outside the pool");
         const uint16_t* sl = reinterpret_cast<const uint16_t*>(&P[slots]);

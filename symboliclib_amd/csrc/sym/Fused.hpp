@@ -30,7 +30,9 @@
 //          0xffffffff
 //     H[9] result base of the instance
 //     H[10] words from runs to the end of chunks (the recipe, contiguous)
-//     H[11] number of output constants
+//     H[11] stored output constants (bits 0-27) and their period (bits 28-31): 0, or D when
+//          the tile's outputs form D equal dof blocks (columns b + d n) holding the same
+//          constants in the same order, of which one block's are stored
 //     H[12] the tile's element image in the value buffer (split form: first value, in doubles)
 //     H[13] nn, nodes of the tile's elements    H[14] rel tile node list (nn u32, sorted)
 //     H[15 + 3 t], H[16 + 3 t], H[17 + 3 t]: template t's element count, rel slot table

@@ -1045,7 +1045,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
         if (variant != 2) stamp(os, 0);
         os << "        const INDEX* __restrict__ H = tiles + 1u + numTiles + it * " << fp.headerStride << "u;  // launch-order header\n";
-        os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10], nconst = H[11];\n";
+        os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10];\n"
+           << "        const unsigned nconst = H[11] & 0x0fffffffu, kper = H[11] >> 28;  // stored constants, period\n";
         std::string vsum = "0u", psum = "0u", wsum = "0u";
         std::vector<std::string> pbase(NT), wbase(NT);
         for (size_t t = 0; t < NT; ++t) {
@@ -1060,7 +1061,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             wsum += " + ((" + cnt + " * " + std::to_string(ft.numNodes) + "u + 1u) >> 1)";
         }
         os << "        const unsigned nn = H[13];\n";
-        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)lane; (void)wv; (void)nn;\n";
+        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)kper; (void)lane; (void)wv; (void)nn;\n";
         // staged region: parameters, node coordinates (stagedDofs per node), slot tables
         os << "        const unsigned np = " << psum << ", nsw = " << wsum << ";\n";
         os << "        const unsigned nvals = " << vsum << ", nstaged = np + " << fp.stagedDofs << "u * nn + ((nsw + 1u) >> 1);\n";
@@ -1192,6 +1193,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
                << "            double* __restrict__ res = out + H[9];\n"
                << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
+               // periodic constants: output block j of kper holds the stored block's constants again
+               << "            const unsigned kblk = kper ? m / kper : 0u;\n"
                << "            const unsigned* __restrict__ S2 = (const unsigned*)S;  // slot pairs\n"
                << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
                << "                const unsigned oi = ch * 64u + lane;\n"
@@ -1205,8 +1208,10 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
                << "                const unsigned r = cr[1] + (unsigned)__popcll((((unsigned long long)cr[4] << 32) | cr[3]) & below);\n"
                << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
+               << "                unsigned kidx = cr[2] + (ex >> 16);\n"
+               << "                for (unsigned j = 1; j < kper; ++j) kidx -= oi >= j * kblk ? nconst : 0u;\n"
                << "                double kc = 0.0;\n"
-               << "                if (hc) kc = K[cr[2] + (ex >> 16)];\n"
+               << "                if (hc) kc = K[kidx];\n"
                << "                double s = 0.0;\n"
                << "                for (unsigned q = 0; q < k; q += 2u) {\n"
                << "                    const unsigned w = S2[(base + q) >> 1];\n"
