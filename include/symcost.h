@@ -6,7 +6,7 @@
  *
  * No counterpart in the reference: SymbolicLib picks no launch parameters (its HIP path always
  * launches ceil(n / ThreadsPerBlock) blocks per group, src/ComputeUnit.cpp:440-447).  The
- * autotuner that uses this model keeps exhaustive search as its fallback and as the judge of
+ * autotuner that uses this model keeps exhaustive search as its fallback and as the measure of
  * the model's regret.
  *
  * Features are standardised with the mean and deviation of the training set, the target with

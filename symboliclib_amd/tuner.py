@@ -12,7 +12,7 @@ contributions, elements, staged inputs, LDS, runs per tile), the emitted code (F
 per element, values a work item keeps live) and the launch (groups, waves per SIMD, block size,
 schedule, estimated residency).  The model predicts log(milliseconds); the tuner ranks all
 configurations by prediction and times only the best few.  Exhaustive search stays the
-fallback and the judge of the model's regret.
+fallback and the measure of the model's regret.
 """
 import ctypes
 import itertools
