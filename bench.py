@@ -137,8 +137,8 @@ def run_single(args, dev, stream, sp):
     import oracle as O
 
     if args.form == "fused":
-        if args.workload != "c3":
-            raise SystemExit("bench.py: the fused form applies to the cloth Hessian (c3, c4)")
+        if args.workload == "c2":
+            raise SystemExit("bench.py: the fused forms apply to the cloth Hessian (c3, c4) and A^T A + A (c1)")
         opt = fused_options(args)
     else:
         opt = {"tiled": args.form == "tiled", "tile_lds_bytes": args.tile_lds or 0,
