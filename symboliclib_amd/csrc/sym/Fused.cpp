@@ -66,6 +66,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
     PhaseTimer timer;
     const int T = fo.threadsPerBlock;
     if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("fused form: threads per block must be a multiple of 64 up to 1024");
+    if (fo.ldsBytes > FUSED_MAX_LDS_BYTES) throw std::invalid_argument("fused form: more LDS per workgroup than a CU has (160 KiB)");
     FusedProgram fp;
     std::vector<size_t> inOff;
     inputLayout(io, inOff);

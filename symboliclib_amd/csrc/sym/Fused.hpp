@@ -134,6 +134,7 @@ inline bool fusedHeaderOffset(size_t w) {
 // Doubles of a tile's staged region (parameters, node coordinates, slot tables) from its header.
 size_t fusedStagedDoubles(const FusedProgram& fp, const uint32_t* H);
 constexpr int FUSED_CHUNK_WORDS = 5;  // words per 64-output chunk record of an element tile
+constexpr size_t FUSED_MAX_LDS_BYTES = 160 * 1024;  // LDS of one gfx950 CU
 std::vector<float> tileStats(const FusedProgram& fp);
 
 // Write the tile table pool[1..numTiles] in the order of `schedule`.

@@ -26,6 +26,7 @@ constexpr int HEADER = 14;
 
 FusedProgram buildProductTiles(int n, const int* colptr, const int* rowidx, const FusedOptions& fo,
                                std::vector<int>& outer, std::vector<int>& inner) {
+    if (fo.ldsBytes > FUSED_MAX_LDS_BYTES) throw std::invalid_argument("product tiles: more LDS per workgroup than a CU has (160 KiB)");
     PhaseTimer timer;
     const int T = fo.threadsPerBlock;
     if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("product tiles: threads per block must be a multiple of 64 up to 1024");

@@ -225,6 +225,11 @@ def test_fused_form_errors():
     rest, F, X = W.c3_cloth(side=4)
     with pytest.raises(_native.SymlibError, match="more LDS"):
         Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=256)
+    # budgets beyond one CU's 160 KiB of LDS could never launch
+    with pytest.raises(_native.SymlibError, match="160 KiB"):
+        Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=200000)
+    with pytest.raises(_native.SymlibError, match="160 KiB"):
+        Unit.ata_plus_a(cp, ri, fused=True, tile_lds_bytes=200000)
 
 
 @pytest.mark.parametrize("subdiv", [0, 2, 4])
