@@ -205,6 +205,19 @@ def test_bw_hessian_batch_fused(require_device, schedule, split):
             _check(f"fused batch {schedule} split={split} cloth {k} ({sides[k]}x{sides[k]})", u, ref, u.get_results(k), k)
 
 
+@pytest.mark.parametrize("schedule", ["static", "queue"])
+def test_bw_hessian_fused_tiny_cloths(require_device, schedule):
+    # one-tile instances: two triangles (a single interior edge) and a 3x3 sheet, batched
+    sides = [2, 3, 2]
+    cloths = [W.cloth(s, seed=400 + k) for k, s in enumerate(sides)]
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True,
+                               schedule=schedule, output_groups=2) as u:
+        u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
+        for k, (rest, F, X) in enumerate(cloths):
+            _check(f"tiny fused {schedule} cloth {k} ({sides[k]}x{sides[k]})", u,
+                   O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results(k), k)
+
+
 def test_fused_retune_between_one_and_two_launches(require_device):
     sides = [20, 7, 33]
     cloths = [W.cloth(s, seed=250 + s) for s in sides]
