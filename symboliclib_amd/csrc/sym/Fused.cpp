@@ -49,6 +49,7 @@ struct InstanceTiles {
     std::vector<double> cost;    // per tile: outputs + element values
     std::vector<size_t> values;  // per tile: element values (its image in the split form)
     size_t evaluations = 0, contributions = 0, lds = 0, elementLds = 0, stagedLds = 0, outputs = 0;
+    int maxPairs = 0;  // the most slot pairs of an output
 };
 
 }  // namespace
@@ -364,6 +365,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                         if (cnt > 126) throw std::invalid_argument("fused form: an output has more than 126 element contributions");
                         // lists padded to even length with the zero slot (read in pairs)
                         counts.push_back((uint8_t)((cnt + (cnt & 1u)) | (P.hasConst[q] ? 0x80u : 0u)));
+                        out.maxPairs = std::max(out.maxPairs, (int)((cnt + 1) / 2));
                         for (uint32_t r = 0; r < cnt; ++r) {
                             const uint64_t s = P.contrib[P.first[q] + r];
                             const size_t i = (size_t)(s >> 16) - e0;
@@ -527,6 +529,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         fp.numContributions += tiles[k].contributions;
         fp.ldsBytes = std::max(fp.ldsBytes, tiles[k].lds);
         fp.elementLdsBytes = std::max(fp.elementLdsBytes, tiles[k].elementLds);
+        fp.maxPairs = std::max(fp.maxPairs, tiles[k].maxPairs);
     }
     u32(pb, "tile pool");
     u32(cb, "constants");
@@ -752,9 +755,14 @@ void validateFused(const FusedProgram& fp) {
         size_t S = 0, numConst = 0;
         for (size_t o = 0; o < m; ++o) {
             need((cnts[o] & 1u) == 0, "contribution list of odd length");
+            need((int)(cnts[o] & 0x7fu) <= 2 * fp.maxPairs, "contribution list longer than the program's longest");
             S += cnts[o] & 0x7fu;
             numConst += cnts[o] >> 7;
         }
+        // unrolled sums read maxPairs slot words from a lane's first one; the lane past the last
+        // output starts at the end of the slot table, so the chunk records must follow it
+        if (fp.maxPairs <= FUSED_UNROLLED_PAIRS && m)
+            need(chunks == slots + S / 2 && CW * nch >= (size_t)fp.maxPairs, "slot words past the table's end");
         // stored constants: all of them, or one period of D equal dof blocks
         const size_t stored = P[H + 11] & 0x0fffffffu, period = P[H + 11] >> 28;
         if (period) {

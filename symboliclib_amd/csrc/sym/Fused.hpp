@@ -118,6 +118,7 @@ struct FusedProgram {
     // (H[4..7], node lists) replaced by absolute pool indices
     size_t headerTable = 0, headerStride = 0;
     int stagedDofs = 0;  // dofs staged per tile node: 1 + the largest dof an element reads
+    int maxPairs = 0;    // element tiles: the most slot pairs (even-padded contributions / 2) of an output
     size_t elementEvaluations = 0, numElements = 0, numContributions = 0;
     double recompute() const { return numElements ? (double)elementEvaluations / (double)numElements : 0.0; }
     std::string summary() const;
@@ -134,6 +135,9 @@ inline bool fusedHeaderOffset(size_t w) {
 // Doubles of a tile's staged region (parameters, node coordinates, slot tables) from its header.
 size_t fusedStagedDoubles(const FusedProgram& fp, const uint32_t* H);
 constexpr int FUSED_CHUNK_WORDS = 5;  // words per 64-output chunk record of an element tile
+// Element tiles whose outputs have at most this many slot pairs sum with every pair's reads
+// issued at once (pairs past a lane's list replaced by the zero slot); longer lists loop.
+constexpr int FUSED_UNROLLED_PAIRS = 6;
 constexpr size_t FUSED_MAX_LDS_BYTES = 160 * 1024;  // LDS of one gfx950 CU
 std::vector<float> tileStats(const FusedProgram& fp);
 

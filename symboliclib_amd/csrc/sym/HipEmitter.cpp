@@ -1196,31 +1196,84 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                // periodic constants: output block j of kper holds the stored block's constants again
                << "            const unsigned kblk = kper ? m / kper : 0u;\n"
                << "            const unsigned* __restrict__ S2 = (const unsigned*)S;  // slot pairs\n"
-               << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
-               << "                const unsigned oi = ch * 64u + lane;\n"
-               << "                const bool act = oi < m;\n"
-               // the chunk record and the lane's count byte in one round of LDS reads
-               << "                const INDEX* __restrict__ cr = CH + " << FUSED_CHUNK_WORDS << "u * ch;\n"
-               << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
-               << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
-               << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
-               << "                const unsigned base = cr[0] + (ex & 0xffffu);  // even: lists have even length\n"
-               // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
-               << "                const unsigned r = cr[1] + (unsigned)__popcll((((unsigned long long)cr[4] << 32) | cr[3]) & below);\n"
-               << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
-               << "                unsigned kidx = cr[2] + (ex >> 16);\n"
-               << "                for (unsigned j = 1; j < kper; ++j) kidx -= oi >= j * kblk ? nconst : 0u;\n"
-               << "                double kc = 0.0;\n"
-               << "                if (hc) kc = K[kidx];\n"
-               << "                double s = 0.0;\n"
-               << "                for (unsigned q = 0; q < k; q += 2u) {\n"
-               << "                    const unsigned w = S2[(base + q) >> 1];\n"
-               << "                    s += V[w & 0xffffu];\n"
-               << "                    s += V[w >> 16];\n"
-               << "                }\n"
-               << "                if (act) res[pos] = hc ? s + kc : s;\n"
-               << "            }\n"
-               << "        }\n";
+               ;
+            if (fp.maxPairs <= FUSED_UNROLLED_PAIRS) {
+                // SUM_CHUNKS chunks per wave and iteration, straight-line: every LDS read of the
+                // chunks is issued before the first add.  A lane reads the program's longest list
+                // length in pairs; words past its list (inside the slot table or the chunk records
+                // after it) become the zero slot's pair, so the sums keep the list order and the
+                // extra adds are + 0.0 (a sum that starts at +0.0 is never -0.0).  A chunk past
+                // the last one reads the first chunk's record and stores nothing.
+                constexpr int SUM_CHUNKS = 2;
+                int maxPeriod = 0;
+                for (uint32_t h : fp.tileHeader) maxPeriod = std::max(maxPeriod, (int)(fp.pool[h + 11] >> 28));
+                os << "            const unsigned zp = nvals | (nvals << 16);\n"
+                   << "            for (unsigned ch0 = wv; ch0 < nch; ch0 += " << SUM_CHUNKS * (T / 64) << "u) {\n";
+                for (int u = 0; u < SUM_CHUNKS; ++u) {
+                    const std::string x = "_" + std::to_string(u);
+                    if (u == 0)
+                        os << "                const unsigned ch" << x << " = ch0;\n"
+                           << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * ch0;\n";
+                    else
+                        os << "                const unsigned ch" << x << " = ch0 + " << u * (T / 64) << "u;\n"
+                           << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * (ch" << x
+                           << " < nch ? ch" << x << " : ch0);\n";
+                    os << "                const unsigned oi" << x << " = ch" << x << " * 64u + lane;\n"
+                       << "                const bool act" << x << " = oi" << x << " < m;\n"
+                       << "                const unsigned b8" << x << " = act" << x << " ? (unsigned)C8[oi" << x << "] : 0u;\n"
+                       << "                const unsigned k" << x << " = b8" << x << " & 0x7fu, hc" << x << " = b8" << x << " >> 7;\n"
+                       << "                const unsigned ex" << x << " = symlib_excl_scan64(k" << x << " | (hc" << x << " << 16), lane);\n"
+                       << "                const unsigned b2" << x << " = (cr" << x << "[0] + (ex" << x << " & 0xffffu)) >> 1, np2" << x
+                       << " = k" << x << " >> 1;\n"
+                       << "                const unsigned r" << x << " = cr" << x << "[1] + (unsigned)__popcll((((unsigned long long)cr" << x
+                       << "[4] << 32) | cr" << x << "[3]) & below);\n"
+                       << "                const unsigned pos" << x << " = Rn[2u * r" << x << "] + (oi" << x << " - Rn[2u * r" << x << " + 1u]);\n"
+                       << "                unsigned kidx" << x << " = cr" << x << "[2] + (ex" << x << " >> 16);\n";
+                    if (maxPeriod > 1)
+                        os << "                for (unsigned j = 1; j < " << maxPeriod << "u; ++j) kidx" << x << " -= j < kper && oi" << x
+                           << " >= j * kblk ? nconst : 0u;\n";
+                    // no constant: the zero slot
+                    os << "                const double kc" << x << " = *(hc" << x << " ? K + kidx" << x << " : V + nvals);\n";
+                    for (int q = 0; q < fp.maxPairs; ++q)
+                        os << "                const unsigned a" << q << x << " = S2[b2" << x << " + " << q << "u], w" << q << x << " = " << q
+                           << "u < np2" << x << " ? a" << q << x << " : zp;\n";
+                }
+                for (int u = 0; u < SUM_CHUNKS; ++u) {
+                    const std::string x = "_" + std::to_string(u);
+                    os << "                double s" << x << " = 0.0;\n";
+                    for (int q = 0; q < fp.maxPairs; ++q)
+                        os << "                s" << x << " += V[w" << q << x << " & 0xffffu];\n"
+                           << "                s" << x << " += V[w" << q << x << " >> 16];\n";
+                    os << "                if (act" << x << ") res[pos" << x << "] = s" << x << " + kc" << x << ";\n";
+                }
+                os << "            }\n";
+            } else {
+                os << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
+                   << "                const unsigned oi = ch * 64u + lane;\n"
+                   << "                const bool act = oi < m;\n"
+                   // the chunk record and the lane's count byte in one round of LDS reads
+                   << "                const INDEX* __restrict__ cr = CH + " << FUSED_CHUNK_WORDS << "u * ch;\n"
+                   << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
+                   << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
+                   << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
+                   << "                const unsigned base = cr[0] + (ex & 0xffffu);  // even: lists have even length\n"
+                   // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
+                   << "                const unsigned r = cr[1] + (unsigned)__popcll((((unsigned long long)cr[4] << 32) | cr[3]) & below);\n"
+                   << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
+                   << "                unsigned kidx = cr[2] + (ex >> 16);\n"
+                   << "                for (unsigned j = 1; j < kper; ++j) kidx -= oi >= j * kblk ? nconst : 0u;\n"
+                   << "                double kc = 0.0;\n"
+                   << "                if (hc) kc = K[kidx];\n"
+                   << "                double s = 0.0;\n"
+                   << "                for (unsigned q = 0; q < k; q += 2u) {\n"
+                   << "                    const unsigned w = S2[(base + q) >> 1];\n"
+                   << "                    s += V[w & 0xffffu];\n"
+                   << "                    s += V[w >> 16];\n"
+                   << "                }\n"
+                   << "                if (act) res[pos] = hc ? s + kc : s;\n"
+                   << "            }\n";
+            }
+            os << "        }\n";
         }
         os << "        __syncthreads();\n";
         if (variant != 1) stamp(os, 3);
