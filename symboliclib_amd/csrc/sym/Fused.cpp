@@ -513,7 +513,13 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
     u32(ib, "element images");
     fp.imageDoubles = ib - fp.numInputs;
     // pool: tile count, launch order, launch-order header table, then the tiles
-    fp.headerStride = (FUSED_HEADER + 3 * NT + 3) & ~size_t(3);
+    // (the node list stride rounded up to 64 words keeps the stride, a literal of the kernel
+    // source, the same across most programs)
+    size_t maxNodes = 0;
+    for (size_t k = 0; k < K; ++k)
+        for (size_t h : tiles[k].header) maxNodes = std::max<size_t>(maxNodes, tiles[k].pool[h + 13]);
+    fp.headerNodes = (FUSED_HEADER + 3 * NT + 3) & ~size_t(3);
+    fp.headerStride = fp.headerNodes + ((maxNodes + 63) & ~size_t(63));
     fp.headerTable = 1 + nTiles;
     pb = 1 + nTiles + nTiles * fp.headerStride;
     for (size_t k = 0; k < K; ++k) {
@@ -658,9 +664,13 @@ void orderTiles(FusedProgram& fp, FusedSchedule schedule) {
     const size_t NT = fp.templates.size(), words = FUSED_HEADER + 3 * NT;
     if (fp.headerTable != 1 + n || fp.headerStride < words) throw std::logic_error("orderTiles: no header table in the pool");
     for (size_t t = 0; t < n; ++t) {
-        const size_t H = fp.pool[1 + t];
-        uint32_t* dst = &fp.pool[fp.headerTable + t * fp.headerStride];
+        const size_t H = fp.pool[1 + t], E = fp.headerTable + t * fp.headerStride;
+        uint32_t* dst = &fp.pool[E];
         for (size_t w = 0; w < words; ++w) dst[w] = fusedHeaderOffset(w) ? (uint32_t)(H + fp.pool[H + w]) : fp.pool[H + w];
+        const size_t nn = fp.pool[H + 13], tn = H + fp.pool[H + 14];
+        if (fp.headerNodes < words || fp.headerNodes + nn > fp.headerStride) throw std::logic_error("orderTiles: node list exceeds the table stride");
+        dst[14] = (uint32_t)(E + fp.headerNodes);
+        for (size_t i = 0; i < fp.headerStride - fp.headerNodes; ++i) dst[fp.headerNodes + i] = i < nn ? fp.pool[tn + i] : 0u;
     }
 }
 
@@ -814,11 +824,15 @@ void validateFused(const FusedProgram& fp) {
     // the launch-order header table
     const size_t words = FUSED_HEADER + 3 * NT;
     need(fp.headerStride >= words && fp.headerTable + fp.numTiles * fp.headerStride <= P.size(), "header table outside the pool");
+    need(fp.headerNodes >= words && fp.headerNodes <= fp.headerStride, "header table node lists");
     for (size_t t = 0; t < fp.numTiles; ++t) {
-        const size_t H = P[1 + t];
+        const size_t H = P[1 + t], E = fp.headerTable + t * fp.headerStride;
         for (size_t w = 0; w < words; ++w)
-            need(P[fp.headerTable + t * fp.headerStride + w] == (fusedHeaderOffset(w) ? H + P[H + w] : P[H + w]),
-                 "header table entry");
+            if (w != 14) need(P[E + w] == (fusedHeaderOffset(w) ? H + P[H + w] : P[H + w]), "header table entry");
+        // the node list, copied after the entry's header words
+        const size_t nn = P[H + 13], tn = H + P[H + 14];
+        need(P[E + 14] == E + fp.headerNodes && fp.headerNodes + nn <= fp.headerStride, "header table node list");
+        for (size_t i = 0; i < nn; ++i) need(P[E + fp.headerNodes + i] == P[tn + i], "header table node list copy");
     }
 }
 

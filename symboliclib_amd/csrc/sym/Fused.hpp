@@ -115,8 +115,10 @@ struct FusedProgram {
     size_t imageDoubles = 0;    // split form: element images of all tiles (after the inputs)
     // element tiles: copies of the headers in launch order at pool[headerTable + t * headerStride]
     // (headerTable = 1 + numTiles, right after the launch order) with the relative offsets
-    // (H[4..7], node lists) replaced by absolute pool indices
-    size_t headerTable = 0, headerStride = 0;
+    // (H[4..7], node lists) replaced by absolute pool indices, each followed at headerNodes by a
+    // copy of the tile's node list (H[14] points there), so a workgroup loads its header and its
+    // node numbers in one round of loads
+    size_t headerTable = 0, headerStride = 0, headerNodes = 0;
     int stagedDofs = 0;  // dofs staged per tile node: 1 + the largest dof an element reads
     int maxPairs = 0;    // element tiles: the most slot pairs (even-padded contributions / 2) of an output
     size_t elementEvaluations = 0, numElements = 0, numContributions = 0;

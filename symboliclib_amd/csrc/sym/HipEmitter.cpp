@@ -1045,7 +1045,22 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
         if (variant != 2) stamp(os, 0);
         os << "        const INDEX* __restrict__ H = tiles + 1u + numTiles + it * " << fp.headerStride << "u;  // launch-order header\n";
+        // A0's node numbers sit in the header's table entry: every 32-node group a wave can take
+        // is loaded up to the entry's capacity (zeros past the list) first, so these loads and
+        // the header's are one round trip
+        const size_t nodeCap = fp.headerStride - fp.headerNodes;
+        const size_t nodeRounds = (nodeCap + 32 * (size_t)(T / 64) - 1) / (32 * (size_t)(T / 64));
+        const bool earlyNodes = stage && opt.ldsDma && nodeRounds <= 4;
+        if (earlyNodes)
+            for (size_t k = 0; k < nodeRounds; ++k)
+                os << "        const unsigned u" << k << " = wv + " << k * (size_t)(T / 64) << "u, s" << k << " = (u" << k
+                   << " << 5) + (lane >> 1);\n"
+                   << "        const unsigned node" << k << " = s" << k << " < " << nodeCap << "u ? H[" << fp.headerNodes << "u + s" << k
+                   << "] : 0u;\n";
+        // recipe offsets too: read after a direct load to LDS, a header word becomes a vector load
+        // whose wait also waits for every load issued before it
         os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10];\n"
+           << "        const unsigned hrec = H[4], hcnt = H[5], hslt = H[6], hchk = H[7], hcst = H[8], hres = H[9];\n"
            << "        const unsigned nconst = H[11] & 0x0fffffffu, kper = H[11] >> 28;  // stored constants, period\n";
         std::string vsum = "0u", psum = "0u", wsum = "0u";
         std::vector<std::string> pbase(NT), wbase(NT);
@@ -1053,7 +1068,11 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             const size_t h = FUSED_HEADER + 3 * t;
             const FusedTemplate& ft = fp.templates[t];
             const std::string cnt = "cnt" + std::to_string(t);
-            os << "        const unsigned " << cnt << " = H[" << h << "];\n";
+            // the template's slot table and parameter block offsets are read here, before any
+            // direct load to LDS is issued: read after one, a header word becomes a vector load
+            // whose wait also waits for every load issued before it
+            os << "        const unsigned " << cnt << " = H[" << h << "], tab" << t << " = H[" << h + 1 << "], prm" << t << " = H["
+               << h + 2 << "];\n";
             pbase[t] = psum;
             wbase[t] = wsum;
             vsum += " + " + cnt + " * " + std::to_string(ft.valueStride) + "u";
@@ -1061,7 +1080,9 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             wsum += " + ((" + cnt + " * " + std::to_string(ft.numNodes) + "u + 1u) >> 1)";
         }
         os << "        const unsigned nn = H[13];\n";
-        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)kper; (void)lane; (void)wv; (void)nn;\n";
+        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)kper; (void)lane; (void)wv; (void)nn;\n"
+           << "        (void)hrec; (void)hcnt; (void)hslt; (void)hchk; (void)hcst; (void)hres;\n";
+        for (size_t t = 0; t < NT; ++t) os << "        (void)tab" << t << "; (void)prm" << t << ";\n";
         // staged region: parameters, node coordinates (stagedDofs per node), slot tables
         os << "        const unsigned np = " << psum << ", nsw = " << wsum << ";\n";
         os << "        const unsigned nvals = " << vsum << ", nstaged = np + " << fp.stagedDofs << "u * nn + ((nsw + 1u) >> 1);\n";
@@ -1080,31 +1101,44 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             // A0 with direct loads: each wave takes 32 tile nodes at a time (two lanes per node:
             // low and high dword) and issues one direct load per staged dof; the parameter
             // columns (contiguous in c) and the slot tables (contiguous in the pool) are copied
-            os << "            const INDEX* __restrict__ tn = tiles + H[14];\n";
-            os << "            for (unsigned u = wv; u < ((nn + 31u) >> 5); u += " << T / 64 << "u) {\n"
-               << "                const unsigned s = (u << 5) + (lane >> 1);\n"
-               << "                if (s < nn) {\n"
-               << "                    const unsigned* src = (const unsigned*)(x + B + tn[s]) + (lane & 1u);\n";
-            for (int d = 0; d < fp.stagedDofs; ++d) {
-                os << "                    SYMLIB_LDS_DMA4(src";
-                if (d) os << " + " << 2 * d << "u * N";
-                os << ", Xs + " << d << "u * nn + (u << 5));\n";
+            if (earlyNodes) {
+                for (size_t k = 0; k < nodeRounds; ++k) {
+                    const std::string ks = std::to_string(k);
+                    os << "            if (s" << ks << " < nn) {\n"
+                       << "                const unsigned* src = (const unsigned*)(x + B + node" << ks << ") + (lane & 1u);\n";
+                    for (int d = 0; d < fp.stagedDofs; ++d) {
+                        os << "                SYMLIB_LDS_DMA4(src";
+                        if (d) os << " + " << 2 * d << "u * N";
+                        os << ", Xs + " << d << "u * nn + (u" << ks << " << 5));\n";
+                    }
+                    os << "            }\n";
+                }
+            } else {
+                os << "            const INDEX* __restrict__ tn = H + " << fp.headerNodes << "u;\n"
+                   << "            for (unsigned u = wv; u < ((nn + 31u) >> 5); u += " << T / 64 << "u) {\n"
+                   << "                const unsigned s = (u << 5) + (lane >> 1);\n"
+                   << "                if (s < nn) {\n"
+                   << "                    const unsigned* src = (const unsigned*)(x + B + tn[s]) + (lane & 1u);\n";
+                for (int d = 0; d < fp.stagedDofs; ++d) {
+                    os << "                    SYMLIB_LDS_DMA4(src";
+                    if (d) os << " + " << 2 * d << "u * N";
+                    os << ", Xs + " << d << "u * nn + (u << 5));\n";
+                }
+                os << "                }\n"
+                   << "            }\n";
             }
-            os << "                }\n"
-               << "            }\n";
             for (size_t t = 0; t < NT; ++t) {
-                const size_t h = FUSED_HEADER + 3 * t;
                 const FusedTemplate& ft = fp.templates[t];
                 if (ft.numParams > 0)
-                    os << "            symlib_lds_copy32((const unsigned*)(c + H[" << h + 2 << "]), (unsigned*)(Ast + " << pbase[t]
+                    os << "            symlib_lds_copy32((const unsigned*)(c + prm" << t << "), (unsigned*)(Ast + " << pbase[t]
                        << "), " << 2 * ft.numParams << "u * cnt" << t << ", wv, lane);\n";
-                os << "            symlib_lds_copy32(tiles + H[" << h + 1 << "], Es + " << wbase[t] << ", (cnt" << t << " * "
+                os << "            symlib_lds_copy32(tiles + tab" << t << ", Es + " << wbase[t] << ", (cnt" << t << " * "
                    << ft.numNodes << "u + 1u) >> 1, wv, lane);\n";
             }
         } else if (stage) {
             // A0 through registers: node coordinates, parameters and slot tables, one value per
             // thread and step
-            os << "            const INDEX* __restrict__ tn = tiles + H[14];\n"
+            os << "            const INDEX* __restrict__ tn = H + " << fp.headerNodes << "u;\n"
                << "            for (unsigned s = threadIdx.x; s < nn; s += " << T << "u) {\n"
                << "                const double* __restrict__ src = x + B + tn[s];\n";
             for (int d = 0; d < fp.stagedDofs; ++d)
@@ -1125,18 +1159,22 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             // with element evaluation in the launch, the recipe and constants are copied during
             // it instead (A1), off the staging phase's critical path
             if (sums && !eval)
-                os << "            symlib_lds_copy32(tiles + H[4], R, words, wv, lane);\n"
-                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + H[8]), (unsigned*)K, 2u * nconst, wv, lane);\n";
+                os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n"
+                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
             os << "            __builtin_amdgcn_s_waitcnt(0);\n";
+            // the node numbers stay live to here, so no register that held one is rewritten
+            // while its load may be in flight (which would wait for every load issued so far)
+            if (earlyNodes)
+                for (size_t k = 0; k < nodeRounds; ++k) os << "            __asm__ volatile(\"\" :: \"v\"(node" << k << "));\n";
         } else {
             if (variant == 2)
                 os << "            const double* __restrict__ img = x + H[12];\n"
                    << "            for (unsigned q = threadIdx.x; q < nvals; q += " << T << "u) V[q] = img[q];\n";
             if (sums)
-                os << "            const INDEX* __restrict__ src = tiles + H[4];\n"
+                os << "            const INDEX* __restrict__ src = tiles + hrec;\n"
                    << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
                    << "            if (nconst) {\n"
-                   << "                const double* __restrict__ cs = c + H[8];\n"
+                   << "                const double* __restrict__ cs = c + hcst;\n"
                    << "                for (unsigned q = threadIdx.x; q < nconst; q += " << T << "u) K[q] = cs[q];\n"
                    << "            }\n";
         }
@@ -1155,8 +1193,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             }
             os << "        {\n";
             if (opt.ldsDma && sums)
-                os << "            symlib_lds_copy32(tiles + H[4], R, words, wv, lane);\n"
-                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + H[8]), (unsigned*)K, 2u * nconst, wv, lane);\n";
+                os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n"
+                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
             os << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
                << "                unsigned j = w, Lb = 0;\n";
             for (size_t t = 0; t < NT; ++t) {
@@ -1186,12 +1224,12 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         }
         if (sums) {
             os << "        {\n"
-               << "            const unsigned r0 = H[4];\n"
+               << "            const unsigned r0 = hrec;\n"
                << "            const INDEX* __restrict__ Rn = R;\n"
-               << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (H[5] - r0));\n"
-               << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (H[6] - r0));\n"
-               << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
-               << "            double* __restrict__ res = out + H[9];\n"
+               << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (hcnt - r0));\n"
+               << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (hslt - r0));\n"
+               << "            const INDEX* __restrict__ CH = R + (hchk - r0);\n"
+               << "            double* __restrict__ res = out + hres;\n"
                << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
                // periodic constants: output block j of kper holds the stored block's constants again
                << "            const unsigned kblk = kper ? m / kper : 0u;\n"
