@@ -824,8 +824,8 @@ void emitWaveScan(std::ostream& os) {
 
 // Direct global-to-LDS copy helpers for a workgroup of T threads: symlib_lds_copy32 moves n
 // dwords (64 per wave instruction); symlib_lds_gather64 moves doubles x[idx[i]], i < n, to
-// dst[i] (32 per wave instruction, four index loads in flight per wave).  Neither waits for
-// the data; the caller waits (s_waitcnt 0) before its barrier.
+// dst[i] (32 per wave instruction, four index loads in flight per wave, then four direct
+// loads).  Neither waits for the data; the caller waits (s_waitcnt 0) before its barrier.
 void emitLdsDmaHelpers(std::ostream& os, int T) {
     os << "#define SYMLIB_LDS_DMA4(src, dst) __builtin_amdgcn_global_load_lds((const void*)(src), "
           "(__attribute__((address_space(3))) void*)(dst), 4, 0, 0)\n\n";
@@ -843,6 +843,9 @@ void emitLdsDmaHelpers(std::ostream& os, int T) {
        << "            const unsigned i = b + 32u * u + (lane >> 1);\n"
        << "            a[u] = i < n ? idx[i] : 0u;\n"
        << "        }\n"
+       // one wait for the four indices here: waited inside the lane-masked blocks below, each
+       // would also wait for the direct loads issued before it
+       << "        __asm__ volatile(\"\" :: \"v\"(a[0]), \"v\"(a[1]), \"v\"(a[2]), \"v\"(a[3]));\n"
        << "#pragma unroll\n"
        << "        for (unsigned u = 0; u < 4u; ++u) {\n"
        << "            const unsigned i = b + 32u * u + (lane >> 1);\n"
@@ -881,28 +884,31 @@ std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std
     os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
     stamp(0);
     os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
-    os << "        const unsigned m = H[2], nch = H[3], words = H[10], nst = H[12];\n";
+    // every header word first: read after a direct load to LDS, a header word becomes a vector
+    // load whose wait also waits for every load issued before it
+    os << "        const unsigned m = H[2], nch = H[3], words = H[10], nst = H[12];\n"
+       << "        const unsigned hrec = H[4], hcnt = H[5], hslt = H[6], hchk = H[7], hres = H[9], hpos = H[13];\n";
     os << "        INDEX* __restrict__ R = (INDEX*)(lds + nst);\n";
     if (opt.ldsDma)
-        os << "        symlib_lds_gather64(x, H + H[13], lds, nst, wv, lane);\n"
-           << "        symlib_lds_copy32(H + H[4], R, words, wv, lane);\n"
+        os << "        symlib_lds_gather64(x, H + hpos, lds, nst, wv, lane);\n"
+           << "        symlib_lds_copy32(H + hrec, R, words, wv, lane);\n"
            << "        __builtin_amdgcn_s_waitcnt(0);\n";
     else
         os << "        {\n"
-           << "            const INDEX* __restrict__ PS = H + H[13];\n"
+           << "            const INDEX* __restrict__ PS = H + hpos;\n"
            << "            for (unsigned i = threadIdx.x; i < nst; i += " << T << "u) lds[i] = x[PS[i]];\n"
-           << "            const INDEX* __restrict__ src = H + H[4];\n"
+           << "            const INDEX* __restrict__ src = H + hrec;\n"
            << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
            << "        }\n";
     os << "        __syncthreads();\n";
     stamp(1);
     stamp(2);
     os << "        {\n"
-       << "            const unsigned r0 = H[4];\n"
-       << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (H[5] - r0));\n"
-       << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (H[6] - r0));\n"
-       << "            const INDEX* __restrict__ CH = R + (H[7] - r0);\n"
-       << "            double* __restrict__ res = out + H[9] + R[0];\n"
+       << "            const unsigned r0 = hrec;\n"
+       << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (hcnt - r0));\n"
+       << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (hslt - r0));\n"
+       << "            const INDEX* __restrict__ CH = R + (hchk - r0);\n"
+       << "            double* __restrict__ res = out + hres + R[0];\n"
        << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
        << "                const unsigned oi = ch * 64u + lane;\n"
        << "                const bool act = oi < m;\n"
