@@ -242,7 +242,11 @@ def regret(chosen_ms, records):
 
 
 def write_db(path, program, records):
-    with open(path, "a") as f:
+    """Append records to a timing database (gzip-compressed when the path ends in .gz)."""
+    import gzip
+
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "at") as f:
         for r in records:
             f.write(json.dumps(dict(r, program=program)) + "\n")
 
@@ -301,6 +305,33 @@ def matrix_tuner(cp, ri, a, space=PRODUCT_SPACE, log=None):
     return Tuner(build, [a], space, log)
 
 
+def evaluate(model, recs):
+    """Per program of `recs` (timing records of every configuration): rank the configurations by
+    the model's prediction and take the fastest recorded time among the best k (k = 1, 3); the
+    regret is how much slower that is than the fastest configuration.  Means and maxima per kind."""
+    key = lambda r: (r["program"]["kind"], r["program"]["seed"])  # noqa: E731
+    out = {"heldout": []}
+    for kind, sd in sorted({key(r) for r in recs}):
+        rs = [r for r in recs if key(r) == (kind, sd)]
+        Xh, yh = db_arrays(rs)
+        pred = model.predict(Xh)
+        row = {"kind": kind, "seed": sd, "configurations": len(rs), "best_ms": min(r["ms"] for r in rs),
+               "median_config_ms": float(np.median([r["ms"] for r in rs])),
+               "rank_corr": float(np.corrcoef(pred, yh)[0, 1])}
+        for k in (1, 3):
+            top = np.argsort(pred)[:k]
+            chosen = min(rs[int(i)]["ms"] for i in top)
+            row[f"regret_top{k}"] = regret(chosen, rs)
+        out["heldout"].append(row)
+    for kind in ("cloth_batch", "ata_plus_a"):
+        rows = [h for h in out["heldout"] if h["kind"] == kind]
+        if rows:
+            out[f"{kind}_regret_top1_mean"] = float(np.mean([h["regret_top1"] for h in rows]))
+            out[f"{kind}_regret_top3_mean"] = float(np.mean([h["regret_top3"] for h in rows]))
+            out[f"{kind}_regret_top3_max"] = float(np.max([h["regret_top3"] for h in rows]))
+    return out
+
+
 def _main():
     import argparse
     import sys
@@ -323,6 +354,9 @@ def _main():
     t.add_argument("--epochs", type=int, default=300)
     t.add_argument("--holdout", type=int, default=4, help="cloth programs kept out of training (the last seeds)")
     t.add_argument("--holdout-patterns", type=int, default=16, help="A^T A + A patterns kept out (the last seeds)")
+    e = sub.add_parser("evaluate", help="regret of a trained model on the programs of a timing database")
+    e.add_argument("--model", required=True)
+    e.add_argument("--db", required=True, nargs="+")
     s = sub.add_parser("select", help="tune the batched cloth workload: model choice vs exhaustive search")
     s.add_argument("--model", required=True)
     s.add_argument("--out", required=True)
@@ -374,27 +408,15 @@ def _main():
         log(f"[train] {len(tr)} samples from {len(progs)} programs, {args.epochs} epochs in {secs:.1f}s on the GPU, "
             f"loss {loss:.4f} (standardised)")
         m.save(args.out)
-        # held-out programs: rank every configuration by prediction, time the best k
         out = {"train_samples": len(tr), "train_programs": len(progs), "epochs": args.epochs,
-               "train_seconds": secs, "loss": loss, "heldout": []}
-        for kind, sd in sorted(held):
-            rs = [r for r in recs if key(r) == (kind, sd)]
-            Xh, yh = db_arrays(rs)
-            pred = m.predict(Xh)
-            row = {"kind": kind, "seed": sd, "configurations": len(rs), "best_ms": min(r["ms"] for r in rs),
-                   "median_config_ms": float(np.median([r["ms"] for r in rs])),
-                   "rank_corr": float(np.corrcoef(pred, yh)[0, 1])}
-            for k in (1, 3):
-                top = np.argsort(pred)[:k]
-                chosen = min(rs[int(i)]["ms"] for i in top)
-                row[f"regret_top{k}"] = regret(chosen, rs)
-            out["heldout"].append(row)
-        for kind in ("cloth_batch", "ata_plus_a"):
-            rows = [h for h in out["heldout"] if h["kind"] == kind]
-            if rows:
-                out[f"{kind}_regret_top1_mean"] = float(np.mean([h["regret_top1"] for h in rows]))
-                out[f"{kind}_regret_top3_mean"] = float(np.mean([h["regret_top3"] for h in rows]))
-                out[f"{kind}_regret_top3_max"] = float(np.max([h["regret_top3"] for h in rows]))
+               "train_seconds": secs, "loss": loss}
+        out.update(evaluate(m, [r for r in recs if key(r) in held]))
+        print(json.dumps(out))
+    elif args.cmd == "evaluate":
+        from .costmodel import CostModel
+
+        out = {"model": args.model, "db": args.db}
+        out.update(evaluate(CostModel.load(args.model), [r for path in args.db for r in read_db(path)]))
         print(json.dumps(out))
     elif args.cmd == "select":
         from . import workloads as W
