@@ -263,7 +263,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             std::fill(heads.begin(), heads.end(), 0);
             tNodes.clear();
             for (auto& te : tElems) te.clear();
-            size_t elemBytes = 0, outs = 0;
+            size_t elemBytes = 0, outs = 0, valueBytes = 0;
             double recipeBytes = 0.0;
             tileNodeList.clear();
             push((uint32_t)cursor, cost((uint32_t)cursor, tileId));
@@ -295,13 +295,14 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                     continue;
                 }
                 // LDS the node adds: its new elements, the nodes they bring in, its outputs
-                size_t newBytes = 0;
+                size_t newBytes = 0, newValueBytes = 0;
                 newNodes.clear();
                 ++step;
                 for (uint32_t q = incStart[v]; q < incStart[v + 1]; ++q) {
                     const uint32_t i = inc[q];
                     if (stamp[i] == tileId) continue;
                     newBytes += L[elemT[e0 + i]].elementBytes();
+                    newValueBytes += 8 * (size_t)L[elemT[e0 + i]].valueStride();
                     for (uint32_t j = enStart[i]; j < enStart[i + 1]; ++j) {
                         const uint32_t n = en[j];
                         if (nodeStamp[n] == tileId || nodeStep[n] == step) continue;
@@ -310,13 +311,15 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                     }
                 }
                 const double need = (double)(elemBytes + newBytes + nodeLds * (tileNodeList.size() + newNodes.size()));
-                const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget;
+                // element values and the zero after them within 64 KiB: slots are u16 byte offsets
+                const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget && valueBytes + newValueBytes + 8 <= 65536;
                 if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs)) break;
                 if (!fits) throw std::invalid_argument("fused form: one node and its elements need more LDS than a tile holds");
                 assigned[v] = 1;
                 tNodes.push_back(v);
                 outs += outCount[v];
                 elemBytes += newBytes;
+                valueBytes += newValueBytes;
                 recipeBytes += nodeBytes[v];
                 for (uint32_t n : newNodes) {
                     nodeStamp[n] = tileId;
@@ -343,8 +346,9 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 for (size_t q = 0; q < tElems[t].size(); ++q) local[tElems[t][q]] = (uint32_t)q;
                 ldsBase[t + 1] = ldsBase[t] + tElems[t].size() * (size_t)L[t].valueStride();
             }
-            // slot ldsBase[NT] is the tile's zero (padding of odd contribution lists)
-            if (ldsBase[NT] >= 65536) throw std::logic_error("fused form: tile LDS index exceeds 16 bits");
+            // slot ldsBase[NT] is the tile's zero (padding of odd contribution lists); slots are
+            // stored as LDS byte offsets (8 x index), so the values and the zero fit in 64 KiB
+            if (8 * ldsBase[NT] >= 65536) throw std::logic_error("fused form: tile values exceed 64 KiB of byte offsets");
             std::vector<uint32_t> runs;  // (position, first output)
             std::vector<uint8_t> counts;
             std::vector<uint16_t> slotList;
@@ -372,9 +376,9 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                             const size_t slot = (size_t)(s & 0xffff);
                             if (stamp[i] != tileId) throw std::logic_error("fused form: contribution from an element outside the tile");
                             const size_t t = (size_t)elemT[e0 + i];
-                            slotList.push_back((uint16_t)(ldsBase[t] + (size_t)local[i] * (size_t)L[t].valueStride() + slot));
+                            slotList.push_back((uint16_t)(8 * (ldsBase[t] + (size_t)local[i] * (size_t)L[t].valueStride() + slot)));
                         }
-                        if (cnt & 1u) slotList.push_back((uint16_t)ldsBase[NT]);
+                        if (cnt & 1u) slotList.push_back((uint16_t)(8 * ldsBase[NT]));
                         if (P.hasConst[q]) {
                             cvals.push_back(P.cval[q]);
                             anyConst = true;
@@ -757,7 +761,7 @@ void validateFused(const FusedProgram& fp) {
             lds += cnt * (size_t)ft.valueStride;
         }
         staged = fusedStagedDoubles(fp, &P[H]);
-        need(lds <= 65536, "tile LDS values exceed 16-bit slots");
+        need(8 * (lds + 1) <= 65536, "tile values exceed 16-bit byte offsets");
         need((size_t)P[H + 12] >= fp.numInputs && (size_t)P[H + 12] + lds <= fp.numInputs + fp.imageDoubles,
              "element image outside the value buffer");
         need(8 * staged <= fp.stagedLdsBytes, "staged values exceed the element launch's LDS");
@@ -791,7 +795,8 @@ void validateFused(const FusedProgram& fp) {
              "tile LDS exceeds the launch's LDS");
         need(slots + (S + 1) / 2 <= P.size(), "slot list outside the pool");
         const uint16_t* sl = reinterpret_cast<const uint16_t*>(&P[slots]);
-        for (size_t s = 0; s < S; ++s) need(sl[s] <= lds, "contribution outside the tile's LDS");  // lds: the zero slot
+        // byte offsets of values; lds: the zero slot
+        for (size_t s = 0; s < S; ++s) need(sl[s] % 8 == 0 && sl[s] / 8 <= lds, "contribution outside the tile's LDS");
         totalContrib += S;
         // runs: first outputs increasing from 0 to the sentinel m; positions inside the results
         size_t r = 0;

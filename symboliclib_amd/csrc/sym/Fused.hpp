@@ -22,7 +22,8 @@
 //     H[4] rel runs: (result position, first output) pairs, one sentinel (first = m)
 //     H[5] rel counts: u8 per output, element contributions (bits 0-6) and whether the
 //          output adds a constant (bit 7)
-//     H[6] rel slots: u16 LDS indices, contributions of every output in order
+//     H[6] rel slots: u16 LDS byte offsets of element values (8 x value index), contributions
+//          of every output in order
 //     H[7] rel chunks: (first slot, run of the first output, first constant, run-start mask
 //          low and high: bit j set when output 64 ch + j, j >= 1, starts a run) per 64-output
 //          chunk, FUSED_CHUNK_WORDS words each

@@ -880,7 +880,8 @@ std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std
     if (counter) os << "    __shared__ unsigned s_next;\n";
     if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 4);\n";
     os << "    const unsigned numTiles = tiles[0];\n";
-    os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
+    // the wave index is uniform: loops over waves' shares run on scalar registers
+    os << "    const unsigned lane = threadIdx.x & 63u, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
     os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
     stamp(0);
     os << "        const INDEX* __restrict__ H = tiles + tiles[1u + it];\n";
@@ -1047,7 +1048,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         if (counter) os << "    __shared__ unsigned s_next;\n";
         if (opt.profile) os << "    unsigned long long* __restrict__ prof = (unsigned long long*)(work + 4);\n";
         os << "    const unsigned numTiles = tiles[0];\n";
-        os << "    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;\n";
+        // the wave index is uniform: loops over waves' shares run on scalar registers
+    os << "    const unsigned lane = threadIdx.x & 63u, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
         os << "    for (unsigned it = blockIdx.x; it < numTiles;) {\n";
         if (variant != 2) stamp(os, 0);
         os << "        const INDEX* __restrict__ H = tiles + 1u + numTiles + it * " << fp.headerStride << "u;  // launch-order header\n";
@@ -1253,7 +1255,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 constexpr int SUM_CHUNKS = 1;
                 int maxPeriod = 0;
                 for (uint32_t h : fp.tileHeader) maxPeriod = std::max(maxPeriod, (int)(fp.pool[h + 11] >> 28));
-                os << "            const unsigned zp = nvals | (nvals << 16);\n"
+                os << "            const unsigned zp = (nvals << 3) | (nvals << 19);  // the zero slot's byte offset, twice\n"
                    << "            for (unsigned ch0 = wv; ch0 < nch; ch0 += " << SUM_CHUNKS * (T / 64) << "u) {\n";
                 for (int u = 0; u < SUM_CHUNKS; ++u) {
                     const std::string x = "_" + std::to_string(u);
@@ -1275,9 +1277,18 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                        << "[4] << 32) | cr" << x << "[3]) & below);\n"
                        << "                const unsigned pos" << x << " = Rn[2u * r" << x << "] + (oi" << x << " - Rn[2u * r" << x << " + 1u]);\n"
                        << "                unsigned kidx" << x << " = cr" << x << "[2] + (ex" << x << " >> 16);\n";
-                    if (maxPeriod > 1)
-                        os << "                for (unsigned j = 1; j < " << maxPeriod << "u; ++j) kidx" << x << " -= j < kper && oi" << x
-                           << " >= j * kblk ? nconst : 0u;\n";
+                    if (maxPeriod > 1) {
+                        // dof blocks that start at or before the chunk: one scalar correction;
+                        // a block starting inside the chunk: a lane compare (chunk indices are
+                        // uniform, the wave index being a scalar)
+                        os << "                {\n"
+                           << "                    unsigned j = 1, past = 0;\n"
+                           << "                    for (; j < kper && j * kblk <= ch" << x << " * 64u; ++j) past += nconst;\n"
+                           << "                    kidx" << x << " -= past;\n"
+                           << "                    for (; j < kper && j * kblk < ch" << x << " * 64u + 64u; ++j) kidx" << x << " -= oi" << x
+                           << " >= j * kblk ? nconst : 0u;\n"
+                           << "                }\n";
+                    }
                     // no constant: the zero slot
                     os << "                const double kc" << x << " = *(hc" << x << " ? K + kidx" << x << " : V + nvals);\n";
                     for (int q = 0; q < fp.maxPairs; ++q)
@@ -1288,8 +1299,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                     const std::string x = "_" + std::to_string(u);
                     os << "                double s" << x << " = 0.0;\n";
                     for (int q = 0; q < fp.maxPairs; ++q)
-                        os << "                s" << x << " += V[w" << q << x << " & 0xffffu];\n"
-                           << "                s" << x << " += V[w" << q << x << " >> 16];\n";
+                        os << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " & 0xffffu));\n"
+                           << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " >> 16));\n";
                     os << "                if (act" << x << ") res[pos" << x << "] = s" << x << " + kc" << x << ";\n";
                 }
                 os << "            }\n";
@@ -1313,8 +1324,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                    << "                double s = 0.0;\n"
                    << "                for (unsigned q = 0; q < k; q += 2u) {\n"
                    << "                    const unsigned w = S2[(base + q) >> 1];\n"
-                   << "                    s += V[w & 0xffffu];\n"
-                   << "                    s += V[w >> 16];\n"
+                   << "                    s += *(const double*)((const char*)V + (w & 0xffffu));\n"
+                   << "                    s += *(const double*)((const char*)V + (w >> 16));\n"
                    << "                }\n"
                    << "                if (act) res[pos] = hc ? s + kc : s;\n"
                    << "            }\n";
