@@ -434,7 +434,11 @@ def _main():
             t0 = time.perf_counter()
             recs = tuner.exhaustive(timer, record_features=False)
             best = min(recs, key=lambda r: r["ms"])
-            res.update(exhaustive_best=best["config"], exhaustive_ms=best["ms"], regret=regret(ms, recs),
+            # regret within the exhaustive pass (the chosen configuration's time there), and
+            # across passes (its time in the model's own trials), which includes run-to-run noise
+            same = next((r for r in recs if r["config"] == cfg), None)
+            res.update(exhaustive_best=best["config"], exhaustive_ms=best["ms"],
+                       regret=regret(same["ms"] if same else ms, recs), regret_across_passes=regret(ms, recs),
                        exhaustive_seconds=time.perf_counter() - t0, exhaustive_trials=len(recs))
             log(f"[select] exhaustive best {best['config']}: {best['ms']:.4f} ms; regret {res['regret']:.3f}")
         with open(args.out, "w") as f:
