@@ -99,6 +99,26 @@ def cloth(side, seed, fold=0.2, noise=1e-3):
     return rest, F, X
 
 
+def wheel(n, seed, fold=0.1, noise=1e-3):
+    """(rest, F, X): a flat disc of two rings of n vertices (radii 0.5 and 1) around a center vertex
+    shared by n triangles, so the center's Hessian entries sum n element contributions;
+    X = rest + fold x^2 in z + N(0, noise^2)."""
+    t = 2.0 * np.pi * np.arange(n) / n
+    ring = lambda r: np.stack([r * np.cos(t), r * np.sin(t), np.zeros(n)], 1)  # noqa: E731
+    rest = np.concatenate([np.zeros((1, 3)), ring(0.5), ring(1.0)])
+    F = np.empty((3 * n, 3), np.int32)
+    for i in range(n):
+        a, b = 1 + i, 1 + (i + 1) % n
+        F[3 * i] = (0, a, b)
+        F[3 * i + 1] = (a, a + n, b + n)
+        F[3 * i + 2] = (a, b + n, b)
+    rng = np.random.default_rng(seed)
+    X = rest.copy()
+    X[:, 2] += fold * rest[:, 0] ** 2
+    X += rng.normal(0.0, noise, X.shape)
+    return rest, F, X
+
+
 def random_values(n, seed, lo=-1.0, hi=1.0):
     return np.random.default_rng(seed).uniform(lo, hi, n)
 

@@ -232,6 +232,32 @@ def test_fused_form_errors():
         Unit.ata_plus_a(cp, ri, fused=True, tile_lds_bytes=200000)
 
 
+def test_fused_sums_unrolled_or_looped():
+    # contribution lists of at most 6 slot pairs: every lane reads the longest length at once;
+    # the center of an 18-triangle fan sums 18 contributions (9 pairs), so lists loop per lane
+    rest, F, _ = W.c3_cloth(side=16)
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
+        src = u.source()
+        assert "np2" in src and "q < k; q += 2u" not in src
+    rest, F, _ = W.wheel(18, seed=5)
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
+        src = u.source()
+        assert "q < k; q += 2u" in src and "np2" not in src
+        cp, ri = O.bw_pattern(len(rest), F)
+        outer, inner = u.output_pattern()
+        assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
+
+
+def test_fused_values_within_byte_offsets():
+    # slots are u16 LDS byte offsets: with a whole CU's LDS, tiles stop growing when their element
+    # values and the zero slot would pass 64 KiB
+    rest, F, _ = W.c3_cloth(side=64)
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=160 * 1024) as u:
+        values = u.tile_stats()[:, 4]
+        assert 8 * (values.max() + 1) <= 65536
+        assert values.max() > 8000  # the cap, not the LDS budget, bounds the largest tiles
+
+
 @pytest.mark.parametrize("subdiv", [0, 2, 4])
 def test_product_tiles_pattern(subdiv):
     n, cp, ri, a = W.c1_matrix(subdiv=subdiv)

@@ -181,8 +181,9 @@ def test_bw_hessian_fused(require_device, side, schedule, lds, groups, wpe, spli
 
 @pytest.mark.parametrize("tpb", [64, 128])
 def test_bw_hessian_fused_strided_small_blocks(require_device, tpb):
-    # tiles with more nodes than threads: the prefetched node numbers cover the first tpb
-    # nodes, the rest are read in the staging loop
+    # blocks of one or two waves: the node numbers of a tile take several rounds of 32 nodes
+    # per wave (loaded ahead of the header, or in the staging loop when that needs more than
+    # four rounds)
     rest, F, X = W.c3_cloth(side=48)
     with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, schedule="strided",
                          tile_lds_bytes=81408, output_groups=2, threads_per_block=tpb) as u:
@@ -215,6 +216,26 @@ def test_bw_hessian_fused_tiny_cloths(require_device, schedule):
         u.compile().execute(*[W.to_colmajor(X) for _, _, X in cloths])
         for k, (rest, F, X) in enumerate(cloths):
             _check(f"tiny fused {schedule} cloth {k} ({sides[k]}x{sides[k]})", u,
+                   O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results(k), k)
+
+
+@pytest.mark.parametrize("schedule", ["static", "queue"])
+def test_bw_hessian_fused_long_contribution_lists(require_device, schedule):
+    # the center of an 18-triangle fan sums 18 element contributions (9 slot pairs, more than the
+    # unrolled sums take): the per-lane loop, alone and batched with a cloth
+    rest, F, X = W.wheel(18, seed=5)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, schedule=schedule,
+                         output_groups=2) as u:
+        assert "q < k; q += 2u" in u.source()
+        u.compile().execute(W.to_colmajor(X))
+        _check(f"fan of 18 fused {schedule}", u, O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND),
+               u.get_results())
+    meshes = [W.wheel(18, seed=6), W.cloth(9, seed=8), W.wheel(7, seed=9)]
+    with Unit.bw_hessian_batch([(r, F) for r, F, _ in meshes], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True,
+                               schedule=schedule, output_groups=2) as u:
+        u.compile().execute(*[W.to_colmajor(X) for _, _, X in meshes])
+        for k, (rest, F, X) in enumerate(meshes):
+            _check(f"fans and a cloth fused {schedule}, mesh {k}", u,
                    O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results(k), k)
 
 
