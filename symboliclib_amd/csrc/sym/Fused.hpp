@@ -139,8 +139,10 @@ inline bool fusedHeaderOffset(size_t w) {
 size_t fusedStagedDoubles(const FusedProgram& fp, const uint32_t* H);
 constexpr int FUSED_CHUNK_WORDS = 5;  // words per 64-output chunk record of an element tile
 // Element tiles whose outputs have at most this many slot pairs sum with every pair's reads
-// issued at once (pairs past a lane's list replaced by the zero slot); longer lists loop.
-constexpr int FUSED_UNROLLED_PAIRS = 6;
+// issued at once (pairs past a lane's list replaced by the zero slot); longer lists loop.  A
+// lane past a tile's last output reads that many words from the end of the slot table, inside
+// the chunk records that follow it (at least FUSED_CHUNK_WORDS words).
+constexpr int FUSED_UNROLLED_PAIRS = FUSED_CHUNK_WORDS;
 constexpr size_t FUSED_MAX_LDS_BYTES = 160 * 1024;  // LDS of one gfx950 CU
 std::vector<float> tileStats(const FusedProgram& fp);
 

@@ -233,7 +233,7 @@ def test_fused_form_errors():
 
 
 def test_fused_sums_unrolled_or_looped():
-    # contribution lists of at most 6 slot pairs: every lane reads the longest length at once;
+    # contribution lists of at most 5 slot pairs: every lane reads the longest length at once;
     # the center of an 18-triangle fan sums 18 contributions (9 pairs), so lists loop per lane
     rest, F, _ = W.c3_cloth(side=16)
     with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
@@ -246,6 +246,11 @@ def test_fused_sums_unrolled_or_looped():
         cp, ri = O.bw_pattern(len(rest), F)
         outer, inner = u.output_pattern()
         assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
+    # the bound: 10 contributions (5 pairs) unrolled, 12 (6 pairs) looped
+    for n, looped in [(10, False), (12, True)]:
+        rest, F, _ = W.wheel(n, seed=5)
+        with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
+            assert ("q < k; q += 2u" in u.source()) == looped
 
 
 def test_fused_values_within_byte_offsets():

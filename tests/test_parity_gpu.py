@@ -219,16 +219,17 @@ def test_bw_hessian_fused_tiny_cloths(require_device, schedule):
                    O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND), u.get_results(k), k)
 
 
-@pytest.mark.parametrize("schedule", ["static", "queue"])
-def test_bw_hessian_fused_long_contribution_lists(require_device, schedule):
-    # the center of an 18-triangle fan sums 18 element contributions (9 slot pairs, more than the
-    # unrolled sums take): the per-lane loop, alone and batched with a cloth
-    rest, F, X = W.wheel(18, seed=5)
+@pytest.mark.parametrize("n,schedule", [(10, "static"), (18, "static"), (18, "queue")])
+def test_bw_hessian_fused_long_contribution_lists(require_device, n, schedule):
+    # the center of an n-triangle fan sums n element contributions: 5 slot pairs (n = 10, the
+    # most the unrolled sums take) or 9 (n = 18, the per-lane loop); then fans batched with a
+    # cloth (the loop)
+    rest, F, X = W.wheel(n, seed=5)
     with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, schedule=schedule,
                          output_groups=2) as u:
-        assert "q < k; q += 2u" in u.source()
+        assert ("q < k; q += 2u" in u.source()) == (n > 10)
         u.compile().execute(W.to_colmajor(X))
-        _check(f"fan of 18 fused {schedule}", u, O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND),
+        _check(f"fan of {n} fused {schedule}", u, O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND),
                u.get_results())
     meshes = [W.wheel(18, seed=6), W.cloth(9, seed=8), W.wheel(7, seed=9)]
     with Unit.bw_hessian_batch([(r, F) for r, F, _ in meshes], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True,
