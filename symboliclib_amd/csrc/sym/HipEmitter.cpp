@@ -1247,11 +1247,12 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 // SUM_CHUNKS chunks per wave and iteration, straight-line: every LDS read of the
                 // chunks is issued before the first add (one chunk: the 256-cloth batch ran 1.5 %
                 // faster than with two, whose extra waves' worth of work unbalances the last
-                // round of chunks more than it hides latency).  A lane reads the program's longest list
-                // length in pairs; words past its list (inside the slot table or the chunk records
-                // after it) become the zero slot's pair, so the sums keep the list order and the
-                // extra adds are + 0.0 (a sum that starts at +0.0 is never -0.0).  A chunk past
-                // the last one reads the first chunk's record and stores nothing.
+                // round of chunks more than it hides latency).  A lane reads the program's
+                // longest list length in pairs; words past its list (inside the slot table or the
+                // chunk records after it) become the zero slot's pair, so the sums keep the list
+                // order and the extra adds are + 0.0 (a sum that starts at +0.0 is never -0.0).
+                // A chunk past the last one reads the first chunk's record and stores nothing.
+                // Slots are byte offsets: an address is one mask or shift.
                 constexpr int SUM_CHUNKS = 1;
                 int maxPeriod = 0;
                 for (uint32_t h : fp.tileHeader) maxPeriod = std::max(maxPeriod, (int)(fp.pool[h + 11] >> 28));
