@@ -829,10 +829,15 @@ void emitWaveScan(std::ostream& os) {
 void emitLdsDmaHelpers(std::ostream& os, int T) {
     os << "#define SYMLIB_LDS_DMA4(src, dst) __builtin_amdgcn_global_load_lds((const void*)(src), "
           "(__attribute__((address_space(3))) void*)(dst), 4, 0, 0)\n\n";
+    // whole 64-word blocks unmasked, then at most one partial block; the source of a block is a
+    // uniform base plus the lane's 32-bit byte offset, so the loop spends no vector instructions
+    // on addresses (wv is uniform)
     os << "__device__ __forceinline__ void symlib_lds_copy32(const unsigned* __restrict__ src, unsigned* dst, unsigned n,\n"
        << "    unsigned wv, unsigned lane) {\n"
-       << "    for (unsigned b = wv * 64u; b < n; b += " << T << "u)\n"
-       << "        if (b + lane < n) SYMLIB_LDS_DMA4(src + b + lane, dst + b);\n"
+       << "    const unsigned off = lane << 2;\n"
+       << "    unsigned b = wv * 64u;\n"
+       << "    for (; b + 64u <= n; b += " << T << "u) SYMLIB_LDS_DMA4((const char*)(src + b) + off, dst + b);\n"
+       << "    if (b < n && b + lane < n) SYMLIB_LDS_DMA4((const char*)(src + b) + off, dst + b);\n"
        << "}\n\n";
     os << "__device__ __forceinline__ void symlib_lds_gather64(const double* __restrict__ x, const INDEX* __restrict__ idx,\n"
        << "    double* dst, unsigned n, unsigned wv, unsigned lane) {\n"
