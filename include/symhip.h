@@ -104,7 +104,8 @@ int symhip_create(symhip_t* out, const symhip_program_desc* desc);
 /*
  * Compile `source` with hipRTC for the current device's architecture (gfx950 when no device
  * is present) and load it.  `options` is a space-separated list of extra compiler options
- * or NULL.  Code objects are cached on disk by a hash of source, options and architecture.
+ * or NULL.  Code objects are cached on disk by a hash of the source, the full option list
+ * (fixed and extra options, architecture) and the hipRTC version.
  */
 int symhip_compile(symhip_t h, const char* source, const char* options);
 
@@ -117,8 +118,15 @@ int symhip_compile_to_file(const char* source, const char* options, const char* 
  * source and options only loads it.  Thread-safe: several sources may be compiled at once. */
 int symhip_cache_compile(const char* source, const char* options, const char* arch);
 
-/* Upload input container i from host memory (reference: setArg).  n must equal its length. */
+/* Upload input container i from host memory (reference: setArg, src/StringResources.h:273-275).
+ * n must equal its length.  Blocking, like the reference's hipMemcpy: it first waits for all
+ * work on the device, so a launch of this program still running on any stream (a non-blocking
+ * one included) never sees its inputs change. */
 int symhip_set_arg(symhip_t h, int i, const double* host, size_t n);
+/* Same, ordered on `stream` (hipStream_t, NULL = default) instead of waiting for the device:
+ * the copy runs after the work already queued on that stream and before what follows it.
+ * The caller keeps `host` unchanged until the stream has passed the copy. */
+int symhip_set_arg_async(symhip_t h, int i, const double* host, size_t n, void* stream);
 /* Same from device memory, asynchronously on `stream` (hipStream_t, NULL = default). */
 int symhip_set_arg_device(symhip_t h, int i, const double* dev, size_t n, void* stream);
 
@@ -135,7 +143,7 @@ uint32_t symhip_num_launches(symhip_t h);
 uint32_t symhip_launch_grid(symhip_t h, uint32_t idx);
 
 /* Re-tuning a compiled program without rebuilding it: overwrite `n` words of the tile pool
- * from word `offset`, replace launch `idx` (takes effect at the next symhip_compile), resize
+ * from word `offset` (after waiting for the device, so no running launch reads them), replace launch `idx` (takes effect at the next symhip_compile), resize
  * the work words (zeroed), read work words back (after synchronising). */
 int symhip_update_tiles(symhip_t h, uint64_t offset, uint64_t n, const uint32_t* data);
 int symhip_set_launch(symhip_t h, uint32_t idx, const symhip_launch* launch);

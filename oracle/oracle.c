@@ -21,7 +21,10 @@
  *
  * Parity pins (tests/test_oracle.py): Tutorial known answers, exact rational A^T A + A on the
  * reference's data/sphere.mtx, scipy structural patterns, closed-form cotangent weights on a
- * regular grid, zero row sums, finite differences of the energy gradient, Hessian symmetry.
+ * regular grid, zero row sums, finite differences of the energy gradient, Hessian symmetry,
+ * hand-computed bending stiffness Q on two quads, the contribution scale against finite
+ * differences of single-face gradients, and the FP64 values against this same file compiled
+ * in 80-bit extended precision (liboracle_ld.so) within 2e-15 of the term scale.
  */
 #include <math.h>
 #include <stdint.h>
@@ -32,34 +35,42 @@
 #include <omp.h>
 #endif
 
+/* Scale kinds of the cloth and cotangent values (the A^T A + A terms are the products
+ * themselves, so both kinds agree there):
+ *   SCALE_TERMS   every term inside each per-element contribution: the closed form evaluated
+ *                 with the absolute value of every factor and additive term (the running
+ *                 rounding-error scale of evaluating that closed form);
+ *   SCALE_CONTRIB |contribution| summed over the per-element contributions of the output (the
+ *                 per-face Hessian entries and bending constants, the corner weights). */
+enum { SCALE_TERMS = 0, SCALE_CONTRIB = 1 };
+
 /* ---------------------------------------------------------------- assembly --- */
 
-/* Triplet list -> CSC pattern.  Writes out_cp (ncols + 1) and, when out_ri != NULL, out_ri
- * (nnz) and slot[t] = position of triplet t.  Returns nnz. */
+/* Triplet list -> CSC pattern of a square matrix (rows and columns < ncols).  Writes out_cp
+ * (ncols + 1) and, when out_ri != NULL, out_ri (nnz) and slot[t] = position of triplet t.
+ * Returns nnz.  Two stable counting sorts (by row, then by column) put the triplets in
+ * (column, row, triplet) order. */
 static long assemble_pattern(long nt, const int* tr, const int* tc, int ncols, int* out_cp, int* out_ri, long* slot) {
     long* cnt = (long*)calloc((size_t)ncols + 1, sizeof(long));
+    long* byrow = (long*)malloc(sizeof(long) * (size_t)(nt > 0 ? nt : 1));
+    long* order = (long*)malloc(sizeof(long) * (size_t)(nt > 0 ? nt : 1));
+    for (long t = 0; t < nt; ++t) cnt[tr[t] + 1]++;
+    for (int j = 0; j < ncols; ++j) cnt[j + 1] += cnt[j];
+    for (long t = 0; t < nt; ++t) byrow[cnt[tr[t]]++] = t;
+    memset(cnt, 0, sizeof(long) * ((size_t)ncols + 1));
     for (long t = 0; t < nt; ++t) cnt[tc[t] + 1]++;
     for (int j = 0; j < ncols; ++j) cnt[j + 1] += cnt[j];
-    long* order = (long*)malloc(sizeof(long) * (size_t)(nt > 0 ? nt : 1));
     long* fill = (long*)malloc(sizeof(long) * ((size_t)ncols + 1));
     memcpy(fill, cnt, sizeof(long) * ((size_t)ncols + 1));
-    for (long t = 0; t < nt; ++t) order[fill[tc[t]]++] = t;
-    /* sort each column's triplets by row (insertion sort on short runs, else qsort-free merge) */
+    for (long a = 0; a < nt; ++a) {
+        const long t = byrow[a];
+        order[fill[tc[t]]++] = t;
+    }
     long nnz = 0;
     for (int j = 0; j < ncols; ++j) {
-        long b = cnt[j], e = cnt[j + 1];
-        for (long a = b + 1; a < e; ++a) {
-            long v = order[a];
-            long k = a - 1;
-            while (k >= b && (tr[order[k]] > tr[v] || (tr[order[k]] == tr[v] && order[k] > v))) {
-                order[k + 1] = order[k];
-                --k;
-            }
-            order[k + 1] = v;
-        }
         out_cp[j] = (int)nnz;
         int last = -1;
-        for (long a = b; a < e; ++a) {
+        for (long a = cnt[j]; a < cnt[j + 1]; ++a) {
             const long t = order[a];
             if (tr[t] != last) {
                 if (out_ri) out_ri[nnz] = tr[t];
@@ -71,6 +82,7 @@ static long assemble_pattern(long nt, const int* tr, const int* tc, int ncols, i
     }
     out_cp[ncols] = (int)nnz;
     free(cnt);
+    free(byrow);
     free(order);
     free(fill);
     return nnz;
@@ -88,6 +100,36 @@ static void scatter_sum(long nt, const long* slot, const double* tv, long nnz, d
 
 /* ---------------------------------------------------------------- A^T A + A --- */
 
+static int cmp_int(const void* a, const void* b) {
+    const int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+/* Rows i of column j of the structural A^T A + A: every i with A(k, i) and A(k, j) both stored
+ * for some k, plus the rows of A's column j; unsorted, written to rows (mark: n ints, the
+ * caller's, holding values != j on entry).  Returns their count. */
+static int ata_column_rows(int j, const int* cp, const int* ri, const int* rp, const int* ci, int* mark, int* rows) {
+    int m = 0;
+    for (int p = cp[j]; p < cp[j + 1]; ++p) {
+        const int k = ri[p];
+        for (int q = rp[k]; q < rp[k + 1]; ++q) {
+            const int i = ci[q];
+            if (mark[i] != j) {
+                mark[i] = j;
+                if (rows) rows[m] = i;
+                ++m;
+            }
+        }
+    }
+    for (int p = cp[j]; p < cp[j + 1]; ++p)
+        if (mark[ri[p]] != j) {
+            mark[ri[p]] = j;
+            if (rows) rows[m] = ri[p];
+            ++m;
+        }
+    return m;
+}
+
 long oracle_ata_plus_a_pattern(int n, const int* cp, const int* ri, int* out_cp, int* out_ri) {
     /* CSR of A = rows' column lists */
     int* rp = (int*)calloc((size_t)n + 1, sizeof(int));
@@ -99,47 +141,36 @@ long oracle_ata_plus_a_pattern(int n, const int* cp, const int* ri, int* out_cp,
     for (int j = 0; j < n; ++j)
         for (int p = cp[j]; p < cp[j + 1]; ++p) ci[fill[ri[p]]++] = j;
 
-    int* mark = (int*)malloc(sizeof(int) * (size_t)n);
-    int* rows = (int*)malloc(sizeof(int) * (size_t)n);
-    for (int i = 0; i < n; ++i) mark[i] = -1;
-    long nnz = 0;
-    for (int j = 0; j < n; ++j) {
-        int m = 0;
-        /* (A^T A)(i, j) != 0 structurally iff rows k with A(k,i), A(k,j) both stored */
-        for (int p = cp[j]; p < cp[j + 1]; ++p) {
-            const int k = ri[p];
-            for (int q = rp[k]; q < rp[k + 1]; ++q) {
-                const int i = ci[q];
-                if (mark[i] != j) {
-                    mark[i] = j;
-                    rows[m++] = i;
-                }
-            }
-        }
-        for (int p = cp[j]; p < cp[j + 1]; ++p)
-            if (mark[ri[p]] != j) {
-                mark[ri[p]] = j;
-                rows[m++] = ri[p];
-            }
-        /* sort rows */
-        for (int a = 1; a < m; ++a) {
-            int v = rows[a], b = a - 1;
-            while (b >= 0 && rows[b] > v) {
-                rows[b + 1] = rows[b];
-                --b;
-            }
-            rows[b + 1] = v;
-        }
-        out_cp[j] = (int)nnz;
-        if (out_ri) memcpy(out_ri + nnz, rows, sizeof(int) * (size_t)m);
-        nnz += m;
+    long* count = (long*)calloc((size_t)n + 1, sizeof(long));
+#pragma omp parallel
+    {
+        int* mark = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+        for (int i = 0; i < n; ++i) mark[i] = -1;
+#pragma omp for schedule(dynamic, 256)
+        for (int j = 0; j < n; ++j) count[j + 1] = ata_column_rows(j, cp, ri, rp, ci, mark, NULL);
+        free(mark);
     }
-    out_cp[n] = (int)nnz;
+    for (int j = 0; j < n; ++j) count[j + 1] += count[j];
+    for (int j = 0; j <= n; ++j) out_cp[j] = (int)count[j];
+    if (out_ri) {
+#pragma omp parallel
+        {
+            int* mark = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+            for (int i = 0; i < n; ++i) mark[i] = -1;
+#pragma omp for schedule(dynamic, 256)
+            for (int j = 0; j < n; ++j) {
+                int* rows = out_ri + count[j];
+                const int m = ata_column_rows(j, cp, ri, rp, ci, mark, rows);
+                qsort(rows, (size_t)m, sizeof(int), cmp_int);
+            }
+            free(mark);
+        }
+    }
+    const long nnz = count[n];
     free(rp);
     free(ci);
     free(fill);
-    free(mark);
-    free(rows);
+    free(count);
     return nnz;
 }
 
@@ -245,8 +276,8 @@ long oracle_cotan_pattern(int nv, int nf, const int* F, int* out_cp, int* out_ri
 }
 
 /* corner weights w = (e0 . e1) / |e0 x e1| (BuildCotan/main.cpp:49-70); tm (optional) gets the
- * magnitude sum |e0_k e1_k| / |e0 x e1| of every weight */
-static void cotan_triplet_values(int nv, int nf, const int* F, const double* V, double* tv, double* tm) {
+ * magnitude of every weight: SCALE_TERMS |e0_k e1_k| summed over k / |e0 x e1|, SCALE_CONTRIB |w| */
+static void cotan_triplet_values(int nv, int nf, const int* F, const double* V, double* tv, double* tm, int kind) {
 #pragma omp parallel for schedule(static)
     for (long f = 0; f < nf; ++f) {
         const int* fv = F + 3 * f;
@@ -263,7 +294,8 @@ static void cotan_triplet_values(int nv, int nf, const int* F, const double* V, 
             tv[t + 2] = w;
             tv[t + 3] = w;
             if (tm) {
-                const double m = (fabs(d0.x * d1.x) + fabs(d0.y * d1.y) + fabs(d0.z * d1.z)) / a;
+                const double m = kind == SCALE_CONTRIB ? fabs(w)
+                                                       : (fabs(d0.x * d1.x) + fabs(d0.y * d1.y) + fabs(d0.z * d1.z)) / a;
                 tm[t] = tm[t + 1] = tm[t + 2] = tm[t + 3] = m;
             }
         }
@@ -280,10 +312,12 @@ typedef struct {
     double* tm; /* optional magnitude of each triplet value (NULL: |tv|) */
 } assembly;
 
-static void assembly_init(assembly* A, long nt, const int* tr, const int* tc, int ncols, int* cp) {
+static void assembly_init(assembly* A, long nt, const int* tr, const int* tc, int ncols, int* cp, int** ri) {
     A->nt = nt;
     A->slot = (long*)malloc(sizeof(long) * (size_t)(nt > 0 ? nt : 1));
-    A->nnz = assemble_pattern(nt, tr, tc, ncols, cp, NULL, A->slot);
+    int* rows = NULL;
+    if (ri) *ri = rows = (int*)malloc(sizeof(int) * (size_t)(nt > 0 ? nt : 1)); /* nnz <= nt */
+    A->nnz = assemble_pattern(nt, tr, tc, ncols, cp, rows, A->slot);
     A->segStart = (long*)calloc((size_t)A->nnz + 1, sizeof(long));
     A->perm = (long*)malloc(sizeof(long) * (size_t)(nt > 0 ? nt : 1));
     A->tv = (double*)malloc(sizeof(double) * (size_t)(nt > 0 ? nt : 1));
@@ -319,16 +353,16 @@ static void assembly_free(assembly* A) {
     free(A->tm);
 }
 
-void oracle_cotan_values(int nv, int nf, const int* F, const double* V, double* vals, double* scale) {
+void oracle_cotan_values(int nv, int nf, const int* F, const double* V, double* vals, double* scale, int kind) {
     const long nt = 12L * nf;
     int* tr = (int*)malloc(sizeof(int) * (size_t)nt);
     int* tc = (int*)malloc(sizeof(int) * (size_t)nt);
     int* cp = (int*)malloc(sizeof(int) * ((size_t)nv + 1));
     cotan_triplets_rc(nf, F, tr, tc);
     assembly A;
-    assembly_init(&A, nt, tr, tc, nv, cp);
+    assembly_init(&A, nt, tr, tc, nv, cp, NULL);
     if (scale) A.tm = (double*)malloc(sizeof(double) * (size_t)nt);
-    cotan_triplet_values(nv, nf, F, V, A.tv, A.tm);
+    cotan_triplet_values(nv, nf, F, V, A.tv, A.tm, kind);
     assembly_sum(&A, vals, scale);
     assembly_free(&A);
     free(tr);
@@ -373,6 +407,12 @@ static void bw_face_K9(int nv, const int* fv, const double* R, const double* X, 
     B[1][1] = Vi[1];
     B[2][0] = Vi[2];
     B[2][1] = Vi[3];
+    /* magnitudes of B: the first row is a sum of two inverse-frame entries, which cancel to
+     * exactly zero on some rest triangles (a right isosceles triangle whose first edge is a
+     * leg); its rounding then scales with |Vi|, not with |B| */
+    const double Bm[3][2] = {{fabs(Vi[0]) + fabs(Vi[2]), fabs(Vi[1]) + fabs(Vi[3])},
+                             {fabs(Vi[0]), fabs(Vi[1])},
+                             {fabs(Vi[2]), fabs(Vi[3])}};
 
     const v3 x0 = vget(X, nv, fv[0]);
     const v3 D0 = vsub(vget(X, nv, fv[1]), x0), D1 = vsub(vget(X, nv, fv[2]), x0);
@@ -431,15 +471,15 @@ static void bw_face_K9(int nv, const int* fv, const double* R, const double* X, 
                         const double IJxm = a * (id + Jn0m[r] * Jn0m[s]) / n0;
                         const double IJym = a * (id + Jn1m[r] * Jn1m[s]) / n1;
                         const double C0m = a * (n0m + 1.0), C1m = a * (n1m + 1.0);
-                        const double stm = a * a * fabs(B[i][0] * B[j][0]) * Jn0m[r] * Jn0m[s] +
-                                           a * a * fabs(B[i][1] * B[j][1]) * Jn1m[r] * Jn1m[s] +
-                                           fabs(B[i][0] * B[j][0]) * IJxm * C0m + fabs(B[i][1] * B[j][1]) * IJym * C1m;
-                        const double Cdvm_ri = a * (fabs(B[i][0]) * J1m[r] + fabs(B[i][1]) * J0m[r]);
-                        const double Cdvm_sj = a * (fabs(B[j][0]) * J1m[s] + fabs(B[j][1]) * J0m[s]);
+                        const double stm = a * a * Bm[i][0] * Bm[j][0] * Jn0m[r] * Jn0m[s] +
+                                           a * a * Bm[i][1] * Bm[j][1] * Jn1m[r] * Jn1m[s] +
+                                           Bm[i][0] * Bm[j][0] * IJxm * C0m + Bm[i][1] * Bm[j][1] * IJym * C1m;
+                        const double Cdvm_ri = a * (Bm[i][0] * J1m[r] + Bm[i][1] * J0m[r]);
+                        const double Cdvm_sj = a * (Bm[j][0] * J1m[s] + Bm[j][1] * J0m[s]);
                         double shm = Cdvm_ri * Cdvm_sj;
                         if (r == s) {
                             const double Csm = a * (J0m[0] * J1m[0] + J0m[1] * J1m[1] + J0m[2] * J1m[2]);
-                            shm += a * (fabs(B[i][0] * B[j][1]) + fabs(B[i][1] * B[j][0])) * Csm;
+                            shm += a * (Bm[i][0] * Bm[j][1] + Bm[i][1] * Bm[j][0]) * Csm;
                         }
                         M9[(3 * i + r) * 9 + 3 * j + s] = fabs(ks) * stm + fabs(ksh) * shm;
                     }
@@ -598,6 +638,7 @@ typedef struct {
     double ks, ksh;
     assembly A;
     int* cp;
+    int* ri; /* rows of the pattern (nnz) */
 } bw_plan;
 
 void* oracle_bw_plan_create(int nv, int nf, const int* F, const double* R, double ks, double ksh, double kb) {
@@ -621,7 +662,7 @@ void* oracle_bw_plan_create(int nv, int nf, const int* F, const double* R, doubl
     int* tc = (int*)malloc(sizeof(int) * (size_t)nt);
     bw_triplets_rc(nv, nf, F, P->ne, P->E4, tr, tc);
     P->cp = (int*)malloc(sizeof(int) * (3 * (size_t)nv + 1));
-    assembly_init(&P->A, nt, tr, tc, 3 * nv, P->cp);
+    assembly_init(&P->A, nt, tr, tc, 3 * nv, P->cp, &P->ri);
     free(tr);
     free(tc);
     /* bending values are constant */
@@ -638,12 +679,27 @@ void* oracle_bw_plan_create(int nv, int nf, const int* F, const double* R, doubl
 
 long oracle_bw_plan_nnz(void* plan) { return ((bw_plan*)plan)->A.nnz; }
 
-void oracle_bw_plan_values(void* plan, const double* X, double* vals, double* scale) {
+/* The plan's CSC pattern (cp: 3 nv + 1, ri: nnz) and, per output, whether a bending constant
+ * is among its contributions (bend: nnz bytes). */
+void oracle_bw_plan_pattern(void* plan, int* cp, int* ri, unsigned char* bend) {
+    const bw_plan* P = (const bw_plan*)plan;
+    memcpy(cp, P->cp, sizeof(int) * (3 * (size_t)P->nv + 1));
+    memcpy(ri, P->ri, sizeof(int) * (size_t)P->A.nnz);
+    const long faceTriplets = 81L * P->nf;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < P->A.nnz; ++s) {
+        unsigned char b = 0;
+        for (long q = P->A.segStart[s]; q < P->A.segStart[s + 1]; ++q) b |= P->A.perm[q] >= faceTriplets;
+        bend[s] = b;
+    }
+}
+
+void oracle_bw_plan_values(void* plan, const double* X, double* vals, double* scale, int kind) {
     bw_plan* P = (bw_plan*)plan;
 #pragma omp parallel for schedule(static)
     for (long f = 0; f < P->nf; ++f) {
         double K9[81], M9[81];
-        bw_face_K9(P->nv, P->F + 3 * f, P->R, X, P->ks, P->ksh, K9, scale ? M9 : NULL);
+        bw_face_K9(P->nv, P->F + 3 * f, P->R, X, P->ks, P->ksh, K9, scale && kind == SCALE_TERMS ? M9 : NULL);
         double* tv = P->A.tv + 81 * f;
         double* tm = P->A.tm + 81 * f;
         /* triplet (i, j, p, q) -> 2 K9(3i + p, 3j + q) */
@@ -652,10 +708,25 @@ void oracle_bw_plan_values(void* plan, const double* X, double* vals, double* sc
                 for (int p = 0; p < 3; ++p)
                     for (int q = 0; q < 3; ++q) {
                         tv[27 * i + 9 * j + 3 * p + q] = 2.0 * K9[(3 * i + p) * 9 + 3 * j + q];
-                        if (scale) tm[27 * i + 9 * j + 3 * p + q] = 2.0 * M9[(3 * i + p) * 9 + 3 * j + q];
+                        if (scale)
+                            tm[27 * i + 9 * j + 3 * p + q] = kind == SCALE_TERMS ? 2.0 * M9[(3 * i + p) * 9 + 3 * j + q]
+                                                                                  : fabs(tv[27 * i + 9 * j + 3 * p + q]);
                     }
     }
     assembly_sum(&P->A, vals, scale);
+}
+
+/* Values with both scales in one pass: st (SCALE_TERMS) and sk (SCALE_CONTRIB). */
+void oracle_bw_plan_check(void* plan, const double* X, double* vals, double* st, double* sk) {
+    bw_plan* P = (bw_plan*)plan;
+    oracle_bw_plan_values(plan, X, vals, st, SCALE_TERMS);
+    const assembly* A = &P->A;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < A->nnz; ++s) {
+        double m = 0.0;
+        for (long q = A->segStart[s]; q < A->segStart[s + 1]; ++q) m += fabs(A->tv[A->perm[q]]);
+        sk[s] = m;
+    }
 }
 
 void oracle_bw_plan_free(void* plan) {
@@ -665,6 +736,7 @@ void oracle_bw_plan_free(void* plan) {
     free(P->E4);
     free(P->Q);
     free(P->cp);
+    free(P->ri);
     free(P);
 }
 
@@ -686,16 +758,16 @@ void* oracle_cotan_plan_create(int nv, int nf, const int* F) {
     int* tc = (int*)malloc(sizeof(int) * (size_t)nt);
     cotan_triplets_rc(nf, F, tr, tc);
     P->cp = (int*)malloc(sizeof(int) * ((size_t)nv + 1));
-    assembly_init(&P->A, nt, tr, tc, nv, P->cp);
+    assembly_init(&P->A, nt, tr, tc, nv, P->cp, NULL);
     free(tr);
     free(tc);
     return P;
 }
 
-void oracle_cotan_plan_values(void* plan, const double* V, double* vals, double* scale) {
+void oracle_cotan_plan_values(void* plan, const double* V, double* vals, double* scale, int kind) {
     cotan_plan* P = (cotan_plan*)plan;
     if (scale && !P->A.tm) P->A.tm = (double*)malloc(sizeof(double) * (size_t)P->A.nt);
-    cotan_triplet_values(P->nv, P->nf, P->F, V, P->A.tv, scale ? P->A.tm : NULL);
+    cotan_triplet_values(P->nv, P->nf, P->F, V, P->A.tv, scale ? P->A.tm : NULL, kind);
     assembly_sum(&P->A, vals, scale);
 }
 

@@ -4,8 +4,12 @@ ctypes wrapper over oracle/liboracle.so (oracle/oracle.c).  Only tests/, the smo
 __graft_entry__.py and the CPU-baseline leg of bench.py import this module; the product
 package symboliclib_amd never does.
 
-Every function returns CSC arrays (indptr, indices) plus values and per-value scales
-scale = sum of |terms|, the denominator of the relative error used by the parity tests.
+Every function returns CSC arrays (indptr, indices) plus values and per-value scales, the
+denominator of the relative error used by the parity tests.  Two scale kinds (oracle.c):
+  "terms"   the magnitudes of every term inside each per-element contribution (the closed form
+            evaluated with |.| of every factor and additive term): the rounding-error scale;
+  "contrib" |contribution| summed over the per-element contributions of each output.
+For A^T A + A both are the sum of |A(k,i) A(k,j)| and |A(i,j)|.
 Reference files restated by the C code are cited in oracle.c.
 """
 import ctypes
@@ -32,7 +36,8 @@ def lib():
         L.oracle_ata_plus_a_values.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p, ctypes.c_void_p]
         L.oracle_cotan_pattern.restype = ctypes.c_long
         L.oracle_cotan_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, ctypes.c_void_p]
-        L.oracle_cotan_values.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _f64p, _f64p, ctypes.c_void_p]
+        L.oracle_cotan_values.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _f64p, _f64p, ctypes.c_void_p,
+                                          ctypes.c_int]
         L.oracle_bw_pattern.restype = ctypes.c_long
         L.oracle_bw_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, ctypes.c_void_p]
         L.oracle_bw_energy.restype = ctypes.c_double
@@ -47,11 +52,13 @@ def lib():
                                             ctypes.c_double]
         L.oracle_bw_plan_nnz.restype = ctypes.c_long
         L.oracle_bw_plan_nnz.argtypes = [ctypes.c_void_p]
-        L.oracle_bw_plan_values.argtypes = [ctypes.c_void_p, _f64p, _f64p, ctypes.c_void_p]
+        L.oracle_bw_plan_values.argtypes = [ctypes.c_void_p, _f64p, _f64p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_bw_plan_free.argtypes = [ctypes.c_void_p]
+        L.oracle_bw_plan_pattern.argtypes = [ctypes.c_void_p, _i32p, _i32p, ctypes.c_void_p]
+        L.oracle_bw_plan_check.argtypes = [ctypes.c_void_p, _f64p, _f64p, _f64p, _f64p]
         L.oracle_cotan_plan_create.restype = ctypes.c_void_p
         L.oracle_cotan_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, _i32p]
-        L.oracle_cotan_plan_values.argtypes = [ctypes.c_void_p, _f64p, _f64p, ctypes.c_void_p]
+        L.oracle_cotan_plan_values.argtypes = [ctypes.c_void_p, _f64p, _f64p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_cotan_plan_free.argtypes = [ctypes.c_void_p]
         L.oracle_num_threads.restype = ctypes.c_int
         _LIB = L
@@ -64,6 +71,9 @@ def _i32(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+SCALE_KINDS = {"terms": 0, "contrib": 1}
 
 
 def _colmajor(X):
@@ -100,7 +110,7 @@ def cotan_pattern(nv, F):
     return cp, ri
 
 
-def cotan(V, F):
+def cotan(V, F, scale="terms"):
     """Cotangent Laplacian of mesh (V: n x 3, F: m x 3)."""
     L = lib()
     nv = len(V)
@@ -108,7 +118,8 @@ def cotan(V, F):
     vals = np.zeros(len(ri))
     sc = np.zeros(len(ri))
     Ff = _i32(F).reshape(-1)
-    L.oracle_cotan_values(nv, len(Ff) // 3, Ff, _colmajor(V), vals, sc.ctypes.data_as(ctypes.c_void_p))
+    L.oracle_cotan_values(nv, len(Ff) // 3, Ff, _colmajor(V), vals, sc.ctypes.data_as(ctypes.c_void_p),
+                          SCALE_KINDS[scale])
     return cp, ri, vals, sc
 
 
@@ -133,12 +144,26 @@ class BWPlan:
         self._p = lib().oracle_bw_plan_create(self.nv, self.nf, self._F, self._R, k_stretch, k_shear, k_bend)
         self.nnz = lib().oracle_bw_plan_nnz(self._p)
 
-    def values(self, X, with_scale=True):
+    def values(self, X, with_scale=True, scale="terms"):
         vals = np.zeros(self.nnz)
         sc = np.zeros(self.nnz) if with_scale else None
         lib().oracle_bw_plan_values(self._p, _colmajor(X), vals,
-                                    sc.ctypes.data_as(ctypes.c_void_p) if with_scale else None)
+                                    sc.ctypes.data_as(ctypes.c_void_p) if with_scale else None, SCALE_KINDS[scale])
         return vals, sc
+
+    def pattern(self):
+        """(indptr, indices, bend): bend[s] is True when output s has a bending constant."""
+        cp = np.zeros(3 * self.nv + 1, np.int32)
+        ri = np.zeros(self.nnz, np.int32)
+        bend = np.zeros(self.nnz, np.uint8)
+        lib().oracle_bw_plan_pattern(self._p, cp, ri, bend.ctypes.data_as(ctypes.c_void_p))
+        return cp, ri, bend.astype(bool)
+
+    def check(self, X):
+        """(values, scale "terms", scale "contrib")."""
+        vals, st, sk = np.zeros(self.nnz), np.zeros(self.nnz), np.zeros(self.nnz)
+        lib().oracle_bw_plan_check(self._p, _colmajor(X), vals, st, sk)
+        return vals, st, sk
 
     def close(self):
         if self._p:
@@ -155,11 +180,12 @@ class CotanPlan:
         self.nv = nv
         self._p = lib().oracle_cotan_plan_create(nv, len(self._F) // 3, self._F)
 
-    def values(self, V, nnz, with_scale=False):
+    def values(self, V, nnz, with_scale=False, scale="terms"):
         vals = np.zeros(nnz)
         sc = np.zeros(nnz) if with_scale else None
         lib().oracle_cotan_plan_values(self._p, _colmajor(V), vals,
-                                       sc.ctypes.data_as(ctypes.c_void_p) if with_scale else None)
+                                       sc.ctypes.data_as(ctypes.c_void_p) if with_scale else None,
+                                       SCALE_KINDS[scale])
         return vals, sc
 
     def close(self):
@@ -171,14 +197,24 @@ class CotanPlan:
         self.close()
 
 
-def bw_hessian(rest, F, X, k_stretch, k_shear, k_bend):
+def bw_hessian(rest, F, X, k_stretch, k_shear, k_bend, scale="terms"):
     """Full symmetric Hessian of the stretch + shear energy plus k_bend * Q per interior edge."""
     cp, ri = bw_pattern(len(rest), F, with_bend=k_bend != 0.0)
     plan = BWPlan(rest, F, k_stretch, k_shear, k_bend)
-    vals, sc = plan.values(X)
+    vals, sc = plan.values(X, scale=scale)
     plan.close()
     assert len(vals) == len(ri)
     return cp, ri, vals, sc
+
+
+def bw_hessian_check(rest, F, X, k_stretch, k_shear, k_bend):
+    """Everything a cloth parity check needs: (indptr, indices, values, scale "terms", scale
+    "contrib", bend) where bend[s] is True when output s carries a bending constant."""
+    plan = BWPlan(rest, F, k_stretch, k_shear, k_bend)
+    cp, ri, bend = plan.pattern()
+    vals, st, sk = plan.check(X)
+    plan.close()
+    return cp, ri, vals, st, sk, bend
 
 
 def bw_energy(rest, F, X, k_stretch, k_shear):
@@ -206,6 +242,57 @@ def bend_Q(rest, quad):
     Q = np.zeros(16)
     lib().oracle_bend_Q(len(rest), _i32(quad), _colmajor(rest), Q)
     return Q.reshape(4, 4)
+
+
+_LIB_LD = None
+_ldp = np.ctypeslib.ndpointer(dtype=np.longdouble, flags="C_CONTIGUOUS")
+
+
+def lib_extended():
+    """The same closed forms in 80-bit x87 extended precision (oracle/liboracle_ld.so)."""
+    global _LIB_LD
+    if _LIB_LD is None:
+        path = os.path.join(_HERE, "liboracle_ld.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle_ld.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        LD = ctypes.c_longdouble
+        L.oracle_bw_plan_create.restype = ctypes.c_void_p
+        L.oracle_bw_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _ldp, LD, LD, LD]
+        L.oracle_bw_plan_nnz.restype = ctypes.c_long
+        L.oracle_bw_plan_nnz.argtypes = [ctypes.c_void_p]
+        L.oracle_bw_plan_check.argtypes = [ctypes.c_void_p, _ldp, _ldp, _ldp, _ldp]
+        L.oracle_bw_plan_free.argtypes = [ctypes.c_void_p]
+        L.oracle_cotan_values.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _ldp, _ldp, ctypes.c_void_p, ctypes.c_int]
+        _LIB_LD = L
+    return _LIB_LD
+
+
+def bw_hessian_extended(rest, F, X, k_stretch, k_shear, k_bend):
+    """Values of bw_hessian evaluated in 80-bit extended precision (long double array, same
+    pattern and order as bw_hessian)."""
+    L = lib_extended()
+    Ff = _i32(F).reshape(-1)
+    R = np.ascontiguousarray(np.asarray(rest, np.float64).T.reshape(-1), dtype=np.longdouble)
+    Xl = np.ascontiguousarray(np.asarray(X, np.float64).T.reshape(-1), dtype=np.longdouble)
+    p = L.oracle_bw_plan_create(len(rest), len(Ff) // 3, Ff, R, k_stretch, k_shear, k_bend)
+    n = L.oracle_bw_plan_nnz(p)
+    vals, st, sk = (np.zeros(n, np.longdouble) for _ in range(3))
+    L.oracle_bw_plan_check(p, Xl, vals, st, sk)
+    L.oracle_bw_plan_free(p)
+    return vals
+
+
+def cotan_extended(V, F):
+    """Values of cotan evaluated in 80-bit extended precision (long double array)."""
+    L = lib_extended()
+    nv = len(V)
+    nnz = cotan_pattern(nv, F)[0][-1]
+    Ff = _i32(F).reshape(-1)
+    vals = np.zeros(nnz, np.longdouble)
+    Vl = np.ascontiguousarray(np.asarray(V, np.float64).T.reshape(-1), dtype=np.longdouble)
+    L.oracle_cotan_values(nv, len(Ff) // 3, Ff, Vl, vals, None, 0)
+    return vals
 
 
 def rel_errors(values, ref_values, scale):

@@ -76,6 +76,7 @@ SIGNATURES = {
     "symhip_launch_info": (_I, [_VP, _U32, _IP, _IP, _IP]),
     "symhip_wall_clock_khz": (_I, [_IP]),
     "symhip_set_arg": (_I, [_VP, _I, _DP, _SZ]),
+    "symhip_set_arg_async": (_I, [_VP, _I, _DP, _SZ, _VP]),
     "symhip_set_arg_device": (_I, [_VP, _I, _VP, _SZ, _VP]),
     "symhip_run": (_I, [_VP, _VP]),
     "symhip_run_graph": (_I, [_VP, _VP]),

@@ -81,15 +81,21 @@ std::string cacheDir() {
     return ".codecache";
 }
 
+// Every option hipRTC compiles a program with: the fixed ones, then the caller's extras.
+std::vector<std::string> compileOptions(const std::vector<std::string>& extra, const std::string& arch) {
+    std::vector<std::string> opts = {"--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                     "-Xclang", "-packed-fp32-ops"};
+    opts.insert(opts.end(), extra.begin(), extra.end());
+    return opts;
+}
+
 int compileSource(const std::string& src, const std::vector<std::string>& extra, const std::string& arch,
                   std::vector<char>& code) {
     hiprtcProgram prog;
     hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "symlib_program.hip", 0, nullptr, nullptr);
     if (r != HIPRTC_SUCCESS) return fail(SYMHIP_E_COMPILE, std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r));
 
-    std::vector<std::string> opts = {"--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature",
-                                     "-Xclang", "-packed-fp32-ops"};
-    opts.insert(opts.end(), extra.begin(), extra.end());
+    const std::vector<std::string> opts = compileOptions(extra, arch);
     std::vector<const char*> copts;
     for (auto& o : opts) copts.push_back(o.c_str());
 
@@ -110,8 +116,14 @@ int compileSource(const std::string& src, const std::vector<std::string>& extra,
     return SYMHIP_SUCCESS;
 }
 
+// Cache key of a code object: hash of the source, the full option list it is compiled with
+// and the hipRTC version, so that a changed flag or compiler never reuses an old object.
 std::string cacheKey(const std::string& src, const char* options, const std::string& arch) {
-    std::string all = src + "\n//opts:" + (options ? options : "") + "\n//arch:" + arch + "\n//v1";
+    int major = 0, minor = 0;
+    if (hiprtcVersion(&major, &minor) != HIPRTC_SUCCESS) major = minor = -1;
+    std::string all = src + "\n//opts:";
+    for (const auto& o : compileOptions(splitOptions(options), arch)) all += o + " ";
+    all += "\n//hiprtc:" + std::to_string(major) + "." + std::to_string(minor);
     char buf[32];
     std::snprintf(buf, sizeof(buf), "%016llx", (unsigned long long)Sym::CRC64_hash(all.data(), all.size()));
     return std::string(buf) + "_" + std::to_string(all.size());
@@ -326,7 +338,20 @@ int symhip_set_arg(symhip_t h, int i, const double* host, size_t n) {
     if (n != h->argLength[i]) return fail(SYMHIP_E_INVALID, "symhip_set_arg: length does not match the input container");
     if (!n) return SYMHIP_SUCCESS;
     if (!host) return fail(SYMHIP_E_INVALID, "symhip_set_arg: null data");
+    // launches of this program may still run on any stream (a non-blocking one included) and
+    // read the inputs: wait for them before the copy overwrites the inputs
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize(setArg)");
     HIP_TRY(hipMemcpy(h->x + h->argStart[i], host, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy(setArg)");
+    return SYMHIP_SUCCESS;
+}
+
+int symhip_set_arg_async(symhip_t h, int i, const double* host, size_t n, void* stream) {
+    if (!h || i < 0 || (size_t)i >= h->argStart.size()) return fail(SYMHIP_E_INVALID, "symhip_set_arg_async: bad argument index");
+    if (n != h->argLength[i]) return fail(SYMHIP_E_INVALID, "symhip_set_arg_async: length does not match the input container");
+    if (!n) return SYMHIP_SUCCESS;
+    if (!host) return fail(SYMHIP_E_INVALID, "symhip_set_arg_async: null data");
+    HIP_TRY(hipMemcpyAsync(h->x + h->argStart[i], host, sizeof(double) * n, hipMemcpyHostToDevice, (hipStream_t)stream),
+            "hipMemcpyAsync(setArg)");
     return SYMHIP_SUCCESS;
 }
 
@@ -402,7 +427,10 @@ int symhip_run(symhip_t h, void* stream) {
 
 int symhip_update_tiles(symhip_t h, uint64_t offset, uint64_t n, const uint32_t* data) {
     if (!h || (n && !data) || offset + n > h->numTilesData) return fail(SYMHIP_E_INVALID, "symhip_update_tiles: range outside the tile pool");
-    if (n) HIP_TRY(hipMemcpy(h->tiles + offset, data, sizeof(uint32_t) * n, hipMemcpyHostToDevice), "hipMemcpy(tiles)");
+    if (!n) return SYMHIP_SUCCESS;
+    // a launch still running on any stream may be reading the pool
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize(update_tiles)");
+    HIP_TRY(hipMemcpy(h->tiles + offset, data, sizeof(uint32_t) * n, hipMemcpyHostToDevice), "hipMemcpy(tiles)");
     return SYMHIP_SUCCESS;
 }
 

@@ -10,6 +10,11 @@
 //   user outputs      out[o[base + i]] = ...    (final position, no gather pass)
 //   intermediates     y[base + i] = ...         (read back by a later wave through p)
 // All arithmetic is FP64.
+//
+// Every kernel takes x and y, which are the same value buffer (the runtime passes it twice).
+// y, written, is not __restrict__; x keeps it because no launch reads through x what it
+// writes through y: a wave reads only intermediates of earlier waves, and the split fused
+// form's element launch writes the element images that only its sum launch reads.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -710,7 +715,7 @@ std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<Lau
     for (int w = 0; w < prog.numWaves; ++w) {
         const std::string name = "symlib_wave" + std::to_string(w);
         os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name
-           << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+           << "(const double* __restrict__ x, double* y, const double* __restrict__ c,\n"
            << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
            << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
         os << "    const unsigned b = blockIdx.x;\n";
@@ -748,7 +753,7 @@ std::string emitHipTiled(const Program& prog, const TiledProgram& tp, const Emit
     os << "__device__ __forceinline__ double gather(const double* __restrict__ x, const double* lds, INDEX v, int off) {\n"
        << "    return (v & 0x80000000u) ? lds[v & 0x7fffffffu] : x[(int)v + off];\n}\n\n";
     os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") symlib_tiled"
-       << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+       << "(const double* __restrict__ x, double* y, const double* __restrict__ c,\n"
        << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
        << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
     os << "    extern __shared__ double lds[];\n";
@@ -878,7 +883,7 @@ std::string emitHipProducts(const FusedProgram& fp, const FusedOptions& opt, std
     os << "extern \"C\" __global__ void __launch_bounds__(" << T;
     if (opt.wavesPerEU > 0) os << ", " << opt.wavesPerEU;
     os << ") symlib_fused"
-       << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+       << "(const double* __restrict__ x, double* y, const double* __restrict__ c,\n"
        << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
        << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
     os << "    extern __shared__ double lds[];\n";
@@ -1046,7 +1051,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         os << "extern \"C\" __global__ void __launch_bounds__(" << T;
         if (opt.wavesPerEU > 0) os << ", " << opt.wavesPerEU;
         os << ") " << name
-           << "(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ c,\n"
+           << "(const double* __restrict__ x, double* y, const double* __restrict__ c,\n"
            << "    const INDEX* __restrict__ p, const INDEX* __restrict__ o, double* __restrict__ out,\n"
            << "    const INDEX* __restrict__ tiles, unsigned* __restrict__ work) {\n";
         os << "    extern __shared__ double lds[];\n";

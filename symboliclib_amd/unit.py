@@ -112,6 +112,37 @@ class Unit:
         _native.check(_native.lib().symlib_unit_execute(self._h, len(arrs), ptrs, lens), "symlib_unit_execute")
         return self
 
+    def execute_async(self, *inputs, stream=None):
+        """execute() ordered on `stream` (symhip_set_arg_async): the arrays must stay unchanged
+        until the stream has passed the copies."""
+        rt = self.runtime()
+        if not rt:
+            raise _native.SymlibError("execute_async: compile() first")
+        arrs = [np.ascontiguousarray(x, np.float64).reshape(-1) for x in inputs]
+        for i, a in enumerate(arrs):
+            _native.check(_native.lib().symhip_set_arg_async(rt, i, a.ctypes.data_as(_native._DP), a.size, stream),
+                          "symhip_set_arg_async")
+        self._keep_async = arrs
+        return self
+
+    def execute_device(self, *addresses, stream=None):
+        """Inputs already in device memory: input container i is copied from device address
+        addresses[i] (its length in doubles) on `stream`."""
+        rt = self.runtime()
+        if not rt:
+            raise _native.SymlibError("execute_device: compile() first")
+        for i, a in enumerate(addresses):
+            _native.check(_native.lib().symhip_set_arg_device(rt, i, a, self.input_lengths[i], stream),
+                          "symhip_set_arg_device")
+        return self
+
+    def get_results_device(self, k, address, stream=None):
+        """Copy output container k to device address `address` on `stream` (no synchronisation)."""
+        n = self.output_shape(k)[2]
+        _native.check(_native.lib().symhip_get_result_device(self.runtime(), k, address, n, stream),
+                      "symhip_get_result_device")
+        return self
+
     def run(self, stream=None):
         _native.check(_native.lib().symlib_unit_run(self._h, stream), "symlib_unit_run")
         return self

@@ -8,8 +8,17 @@ sign, a transposed index, a wrong vertex order) fails it:
     symmetry, invariance to a rigid motion;
   * Baraff-Witkin: Hessian = finite difference of the gradient, gradient = finite difference
     of the energy, symmetry, translation invariance;
-  * bending: Q e = 0 for constants and for coordinates of a flat quad (discrete quadratic
-    bending vanishes on planar deformations).
+  * bending: known answers of Q on two quads computed by hand (every cotangent 1, K0 =
+    (2, 2, -2, -2), A0 = A1 = 1, Q = -1.5 K0 K0^T; and an asymmetric quad), invariance to
+    rotating the second triangle about the hinge, 1 / L^2 scaling, Q e = 0 for constants and
+    for coordinates of a flat quad (discrete quadratic bending vanishes on planar deformations);
+  * error scales: the "contrib" scale of the cloth equals the sum of |entries| of per-face
+    Hessians obtained independently by finite differences of single-face gradients; the
+    "terms" scale is at least that and at least |value|; and the FP64 oracle agrees with the
+    same closed forms evaluated in 80-bit extended precision to within 2e-15 of the "terms"
+    scale on cloths of the batched workload, among them the ones whose rest triangles make
+    the inverse-frame sums cancel to zero (so the scale is a real bound of the oracle's own
+    rounding, not a loose one).
 """
 import os
 from fractions import Fraction
@@ -160,6 +169,114 @@ def test_bend_Q_annihilates_flat_modes():
         pts = rest[q]
         for v in (np.ones(4), pts[:, 0], pts[:, 1]):
             assert np.abs(Q @ v).max() <= 1e-10 * max(1.0, np.abs(Q).max())
+
+
+def test_bend_Q_known_answer_square_quad():
+    # v0 = (0,0,0), v1 = (1,1,0) the hinge; v2 = (1,0,0), v3 = (0,1,0) opposite.  Every angle
+    # opposite the hinge sides is 45 degrees: c01 = c02 = c03 = c04 = 1, K0 = (2, 2, -2, -2),
+    # A0 = |e0 x e1| = 1, A1 = |e0 x e2| = 1, coef = -3 / 2, Q = -1.5 K0 K0^T: entries -6 on the
+    # (v0, v1) and (v2, v3) blocks, +6 across.
+    R = np.array([[0, 0, 0], [1, 1, 0], [1, 0, 0], [0, 1, 0]], float)
+    K0 = np.array([2.0, 2.0, -2.0, -2.0])
+    assert np.array_equal(O.bend_Q(R, [0, 1, 2, 3]), -1.5 * np.outer(K0, K0))
+
+
+def test_bend_Q_known_answer_asymmetric_quad():
+    # v0 = (0,0,0), v1 = (2,0,0), v2 = (0,1,0), v3 = (1,-2,0):
+    # e0 = (2,0,0), e1 = (0,1,0), e2 = (1,-2,0), e3 = (-2,1,0), e4 = (-1,-2,0)
+    # c01 = 0/2 = 0, c02 = 2/4 = 0.5, c03 = cot(-e0, e3) = 4/2 = 2, c04 = cot(-e0, e4) = 2/4 = 0.5
+    # K0 = (c03 + c04, c01 + c02, -c01 - c03, -c02 - c04) = (2.5, 0.5, -2, -1)
+    # A0 = 2, A1 = 4, coef = -3 / 6 = -0.5
+    R = np.array([[0, 0, 0], [2, 0, 0], [0, 1, 0], [1, -2, 0]], float)
+    expect = np.array([[-3.125, -0.625, 2.5, 1.25],
+                       [-0.625, -0.125, 0.5, 0.25],
+                       [2.5, 0.5, -2.0, -1.0],
+                       [1.25, 0.25, -1.0, -0.5]])
+    assert np.array_equal(O.bend_Q(R, [0, 1, 2, 3]), expect)
+
+
+def test_bend_Q_hinge_rotation_and_scale():
+    rng = np.random.default_rng(12)
+    R = np.array([[0, 0, 0], [1.3, 0.2, 0], [0.4, 0.9, 0], [0.8, -1.1, 0]], float)
+    Q = O.bend_Q(R, [0, 1, 2, 3])
+    # rotating v3 about the hinge v0-v1 keeps both triangles' angles and areas: Q unchanged
+    axis = (R[1] - R[0]) / np.linalg.norm(R[1] - R[0])
+    for th in (0.3, 1.9, np.pi):
+        c, s_ = np.cos(th), np.sin(th)
+        Kx = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+        Rot = np.eye(3) + s_ * Kx + (1 - c) * Kx @ Kx
+        R2 = R.copy()
+        R2[3] = R[0] + Rot @ (R[3] - R[0])
+        assert np.allclose(O.bend_Q(R2, [0, 1, 2, 3]), Q, rtol=1e-12, atol=1e-12 * np.abs(Q).max())
+    # scaling the quad by L scales Q by 1 / L^2 (cotangents are scale-free, areas scale by L^2)
+    assert np.allclose(O.bend_Q(2.5 * R, [0, 1, 2, 3]), Q / 6.25, rtol=1e-13)
+    # a rigid motion of the whole quad leaves Q unchanged
+    A, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    assert np.allclose(O.bend_Q(R @ A.T + 3.0, [0, 1, 2, 3]), Q, rtol=1e-11, atol=1e-11 * np.abs(Q).max())
+
+
+def _fd_face_hessians(rest, F, X, ks, ksh, eps=1e-6):
+    """Per-face 9 x 9 Hessians (index 3 i + d) by central differences of single-face gradients."""
+    n = len(rest)
+    out = []
+    for f in F:
+        Ff = np.asarray(f, np.int32).reshape(1, 3)
+        idx = [int(f[i]) + d * n for i in range(3) for d in range(3)]
+        H = np.zeros((9, 9))
+        for a, (i, d) in enumerate((i, d) for i in range(3) for d in range(3)):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[f[i], d] += eps
+            Xm[f[i], d] -= eps
+            H[:, a] = ((O.bw_gradient(rest, Ff, Xp, ks, ksh) - O.bw_gradient(rest, Ff, Xm, ks, ksh)) / (2 * eps))[idx]
+        out.append((idx, H))
+    return out
+
+
+def test_bw_scales_against_finite_difference_contributions():
+    rest, F, X = _bw_small(side=6)
+    ks, ksh, kb = W.K_STRETCH, W.K_SHEAR, W.K_BEND
+    n = len(rest)
+    cp, ri, vals, st, sk, bend = O.bw_hessian_check(rest, F, X, ks, ksh, kb)
+    col = np.repeat(np.arange(3 * n), np.diff(cp))
+    pos = {(int(r), int(c)): k for k, (r, c) in enumerate(zip(ri, col))}
+    brute = np.zeros(len(vals))
+    for idx, H in _fd_face_hessians(rest, F, X, ks, ksh):
+        for a in range(9):
+            for b in range(9):
+                brute[pos[(idx[a], idx[b])]] += abs(H[a, b])
+    # bending constants, brute force from Q of every interior edge
+    for q in O.interior_edges(F):
+        Q = kb * O.bend_Q(rest, q)
+        for d in range(3):
+            for a in range(4):
+                for b in range(4):
+                    brute[pos[(int(q[a]) + d * n, int(q[b]) + d * n)]] += abs(Q[a, b])
+    assert np.allclose(sk, brute, rtol=1e-5, atol=1e-7 * brute.max())
+    assert bend.any() and not bend.all()
+    assert np.all(st >= sk * (1 - 1e-12)) and np.all(sk >= np.abs(vals) * (1 - 1e-12))
+
+
+@pytest.mark.parametrize("k", [150, 5, 176, 0])
+def test_bw_oracle_rounding_within_term_scale(k):
+    # cloths of the 256-cloth batch: sides 71, 70 (the largest FP64 deviations of the former
+    # scale, which missed the cancellation inside the inverse-frame sums), 33 and 227
+    rest, F, X = W.c4_cloths()[k]
+    cp, ri, v, st, sk, bend = O.bw_hessian_check(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+    vl = O.bw_hessian_extended(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
+    err = np.abs((v - vl).astype(np.float64)) / st
+    assert err.max() <= 2e-15, err.max()
+    # the contribution scale is no rounding bound: outputs whose exact value nearly cancels
+    # (inside a face, or in the inverse frame) deviate by many times |contribution| u
+    assert (np.abs((v - vl).astype(np.float64)) / sk).max() > 10 * err.max()
+    # the term scale is within a modest factor of the contribution scale on most outputs
+    assert 5.0 < np.median(st / sk) < 20.0
+
+
+def test_cotan_oracle_rounding_within_term_scale():
+    V, F = W.c2_mesh(side=64)
+    cp, ri, v, st = O.cotan(V, F)
+    vl = O.cotan_extended(V, F)
+    assert (np.abs((v - vl).astype(np.float64)) / st).max() <= 2e-15
 
 
 def test_interior_edge_quads():
