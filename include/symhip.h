@@ -154,6 +154,10 @@ int symhip_read_work(symhip_t h, uint64_t offset, uint64_t n, uint32_t* out);
 /* Registers and residency of launch `idx` after compile: VGPRs per work-item, workgroups per
  * compute unit at its block and LDS size, compute units of the device. */
 int symhip_launch_info(symhip_t h, uint32_t idx, int* vgprs, int* blocks_per_cu, int* cus);
+/* STREAM-style copy of n doubles between 16-byte aligned device buffers with `blocks`
+ * workgroups of 256 work-items, enqueued on `stream` (the measured HBM ceiling; not part of
+ * the reference). */
+int symhip_stream_copy(double* dst, const double* src, size_t n, uint32_t blocks, void* stream);
 /* Rate of the device wall clock read by profiled kernels, in kHz. */
 int symhip_wall_clock_khz(int* khz);
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../sym/Hashing.hpp"
+#include "symhip_internal.hpp"
 
 namespace {
 
@@ -149,26 +150,6 @@ bool writeFileAtomic(const std::string& path, const std::vector<char>& data) {
 }
 
 }  // namespace
-
-struct symhip_program {
-    std::vector<uint64_t> argStart, argLength, outStart, outLength;
-    uint64_t numValues = 0, numResults = 0;
-    double* x = nullptr;
-    double* result = nullptr;
-    double* c = nullptr;
-    uint32_t* p = nullptr;
-    uint32_t* o = nullptr;
-    uint32_t* tiles = nullptr;
-    uint32_t* work = nullptr;
-    uint64_t numTilesData = 0, numWork = 0;
-    std::vector<symhip_launch> launches;
-    std::vector<uint32_t> grids;  // resolved workgroup counts
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
-    std::vector<std::string> names;
-    hipModule_t module = nullptr;
-    std::vector<hipFunction_t> functions;
-};
 
 extern "C" {
 
@@ -372,16 +353,8 @@ uint32_t symhip_launch_grid(symhip_t h, uint32_t idx) {
 }
 
 int symhip_launch_one(symhip_t h, uint32_t idx, void* stream) {
-    if (!h || idx >= h->launches.size()) return fail(SYMHIP_E_INVALID, "symhip_launch_one: bad launch index");
-    if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
-    const symhip_launch& L = h->launches[idx];
-    const uint32_t grid = h->grids[idx];
-    if (grid == 0) return SYMHIP_SUCCESS;
-    void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result, &h->tiles, &h->work};
-    HIP_TRY(hipModuleLaunchKernel(h->functions[idx], grid, 1, 1, L.block, 1, 1, L.shared, (hipStream_t)stream, args,
-                                  nullptr),
-            "hipModuleLaunchKernel");
-    return SYMHIP_SUCCESS;
+    if (!h) return fail(SYMHIP_E_INVALID, "symhip_launch_one: null handle");
+    return symhipLaunch(h, idx, stream, h->work);
 }
 
 int symhip_run_graph(symhip_t h, void* stream) {
@@ -552,3 +525,18 @@ const double* symhip_output_device_ptr(symhip_t h, int k) {
 double* symhip_values_device_ptr(symhip_t h) { return h ? h->x : nullptr; }
 
 }  // extern "C"
+
+int symhipLaunch(symhip_t h, uint32_t idx, void* stream, uint32_t* work) {
+    if (!h || idx >= h->launches.size()) return fail(SYMHIP_E_INVALID, "symhip_launch_one: bad launch index");
+    if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
+    const symhip_launch& L = h->launches[idx];
+    const uint32_t grid = h->grids[idx];
+    if (grid == 0) return SYMHIP_SUCCESS;
+    void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result, &h->tiles, &work};
+    HIP_TRY(hipModuleLaunchKernel(h->functions[idx], grid, 1, 1, L.block, 1, 1, L.shared, (hipStream_t)stream, args,
+                                  nullptr),
+            "hipModuleLaunchKernel");
+    return SYMHIP_SUCCESS;
+}
+
+int symhipFail(int code, const std::string& msg) { return fail(code, msg); }

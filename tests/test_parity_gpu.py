@@ -471,56 +471,53 @@ def test_c5_large_patterns_product_tiles(require_device, rank):
 
 def test_device_io_on_a_side_stream(require_device):
     # inputs from device memory, the launch and the result copy all on one non-blocking stream
-    import ctypes
-
-    import torch
+    from hip_runtime import DeviceArray, Stream
 
     rest, F, X = W.c3_cloth(side=40)
     X2 = W.cloth(40, seed=17)[2]
-    dev = torch.device("cuda:0")
-    side = torch.cuda.Stream(device=dev)
-    sp = ctypes.c_void_p(side.cuda_stream)
+    side = Stream()
     with Unit.bw_hessian_batch([(rest, F)], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True) as u:
         u.compile()
         n = u.output_shape(0)[2]
+        xin, out = DeviceArray(3 * len(rest)), DeviceArray(n)
         for pos in (X, X2):
-            xin = torch.from_numpy(W.to_colmajor(pos)).to(dev)
-            out = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
-            torch.cuda.synchronize()
-            u.execute_device(ctypes.c_void_p(xin.data_ptr()), stream=sp)
-            u.run(sp)
-            u.get_results_device(0, ctypes.c_void_p(out.data_ptr()), stream=sp)
+            xin.upload(W.to_colmajor(pos))
+            out.upload(np.full(n, np.nan))
+            u.execute_device(xin.ptr, stream=side.handle)
+            u.run(side.handle)
+            u.get_results_device(0, out.ptr, stream=side.handle)
             side.synchronize()
-            _check_cloth("device I/O on a side stream", u, rest, F, pos, W.K_BEND, out.cpu().numpy())
+            _check_cloth("device I/O on a side stream", u, rest, F, pos, W.K_BEND, out.download())
         # host inputs ordered on the stream, then a graph replay on it
-        u.execute_async(W.to_colmajor(X), stream=sp)
-        u.run_graph(sp)
+        u.execute_async(W.to_colmajor(X), stream=side.handle)
+        u.run_graph(side.handle)
         side.synchronize()
         _check_cloth("stream-ordered host inputs, graph replay", u, rest, F, X, W.K_BEND, u.get_results())
+        xin.free()
+        out.free()
+    side.destroy()
 
 
 def test_execute_after_run_on_a_nonblocking_stream(require_device):
-    # execute() copies host inputs while a launch of the program may still run on a
-    # non-blocking stream: the copy waits for it, and the next launch sees the new inputs
-    import ctypes
-
-    import torch
+    # execute() copies host inputs while launches of the program may still run on a
+    # non-blocking stream: the copy waits for them, and the next launch sees the new inputs
+    from hip_runtime import Stream
 
     rest, F, X = W.c3_cloth(side=128)
     X2 = W.cloth(128, seed=23)[2]
-    side = torch.cuda.Stream(device=torch.device("cuda:0"))
-    sp = ctypes.c_void_p(side.cuda_stream)
+    side = Stream()
     with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True) as u:
         u.compile().execute(W.to_colmajor(X))
         for _ in range(20):
-            u.run(sp)
+            u.run(side.handle)
         u.execute(W.to_colmajor(X2))
-        u.run(sp)
+        u.run(side.handle)
         _check_cloth("execute after queued launches", u, rest, F, X2, W.K_BEND, u.get_results())
         u.retune(schedule="strided")
         u.execute(W.to_colmajor(X))
         for _ in range(5):
-            u.run(sp)
+            u.run(side.handle)
         u.retune(schedule="static")
-        u.run(sp)
+        u.run(side.handle)
         _check_cloth("retune after queued launches", u, rest, F, X, W.K_BEND, u.get_results())
+    side.destroy()
