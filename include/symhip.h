@@ -136,6 +136,23 @@ int symhip_run(symhip_t h, void* stream);
  * launches are the graph's only nodes: no memset or copy nodes).  Later calls only launch
  * the graph on `stream`. */
 int symhip_run_graph(symhip_t h, void* stream);
+/* Work words of the static scratch array that symhip_run_noalloc uses (64 KiB per device). */
+#define SYMHIP_SCRATCH_WORDS 16384u
+
+/*
+ * symhip_run without any device allocation.  It has no counterpart in the reference: it is for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  The program's launches take their work words (the
+ * work-queue counters of persistent schedules, profiling stamps) from a static array of
+ * SYMHIP_SCRATCH_WORDS words per device, held until the process ends, instead of the handle's
+ * buffer; the call clears the words it uses first (a fill kernel on `stream`, no memset).
+ * Programs that need more work words are rejected with SYMHIP_E_INVALID.  Every call on one
+ * device shares that device's array: calls that overlap on one device (two streams, two
+ * threads) race.  The handle must be compiled; inputs and outputs are the handle's as for
+ * symhip_run.  Calls no hipMalloc, hipFree or allocating library routine.
+ */
+int symhip_run_noalloc(symhip_t h, void* stream);
+
 /* Enqueue launch `idx` only (for per-kernel timing). */
 int symhip_launch_one(symhip_t h, uint32_t idx, void* stream);
 uint32_t symhip_num_launches(symhip_t h);

@@ -81,6 +81,7 @@ SIGNATURES = {
     "symhip_set_arg_device": (_I, [_VP, _I, _VP, _SZ, _VP]),
     "symhip_run": (_I, [_VP, _VP]),
     "symhip_run_graph": (_I, [_VP, _VP]),
+    "symhip_run_noalloc": (_I, [_VP, _VP]),
     "symhip_launch_one": (_I, [_VP, _U32, _VP]),
     "symhip_num_launches": (_U32, [_VP]),
     "symhip_launch_grid": (_U32, [_VP, _U32]),

@@ -1,5 +1,6 @@
 // Internal to the runtime (not part of the C ABI): the program handle and the launch helper
-// shared by symhip.cpp and the device code compiled ahead of time.
+// shared by symhip.cpp and the device code compiled ahead of time (the STREAM copy and the
+// no-allocation entry point symhip_run_noalloc).
 #pragma once
 
 #include <hip/hip_runtime.h>
