@@ -45,7 +45,7 @@ typedef struct {
                                     tiles of owner vertices, element Hessians in LDS, one launch */
     int schedule;                /* fused form: 0 one workgroup per tile, 1 persistent work queue
                                     (default), 2 queue with largest tiles first, 3 queue with
-                                    instances interleaved */
+                                    instances interleaved, 4 persistent strided */
     int output_groups;           /* fused form: work items per element, 0 = default (5) */
     int waves_per_eu;            /* fused form: minimum waves per SIMD to compile for, 0 = none */
     int profile_tiles;           /* fused form: record wall-clock stamps per tile (symlib_unit_tile_profile) */

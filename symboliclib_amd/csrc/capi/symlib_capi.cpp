@@ -311,7 +311,7 @@ Sym::FusedOptions fusedOptionsFrom(const symlib_unit* u, const symlib_options* o
     if (opt->output_groups > 0) fo.outputGroups = opt->output_groups;
     if (opt->waves_per_eu < 0 || opt->waves_per_eu > 8) throw std::invalid_argument("waves per EU must be 0..8");
     fo.wavesPerEU = opt->waves_per_eu;
-    if (opt->schedule < 0 || opt->schedule > 4) throw std::invalid_argument("unknown schedule");
+    if (opt->schedule < 0 || opt->schedule > (int)Sym::FusedSchedule::Strided) throw std::invalid_argument("unknown schedule");
     fo.schedule = (Sym::FusedSchedule)opt->schedule;
     fo.profile = opt->profile_tiles != 0;
     fo.split = opt->split != 0 && u->unit->getFusedProgram().kind == 0;

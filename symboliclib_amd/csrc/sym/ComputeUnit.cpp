@@ -81,7 +81,7 @@ void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
         FusedOptions fo;
         fo.ldsBytes = device.fusedLdsBytes;
         fo.threadsPerBlock = device.threadsPerBlock();
-        if (device.schedule < 0 || device.schedule > 4) throw std::invalid_argument("ComputeUnit: unknown schedule");
+        if (device.schedule < 0 || device.schedule > (int)FusedSchedule::Strided) throw std::invalid_argument("ComputeUnit: unknown schedule");
         fo.schedule = (FusedSchedule)device.schedule;
         if (device.outputGroups < 1 || device.outputGroups > 64) throw std::invalid_argument("ComputeUnit: output groups must be 1..64");
         if (device.wavesPerEU < 0 || device.wavesPerEU > 8) throw std::invalid_argument("ComputeUnit: waves per EU must be 0..8");
@@ -112,7 +112,7 @@ void ComputeUnitBase::initFromProducts(const ProductSum& ps) {
     FusedOptions fo;
     fo.ldsBytes = device.fusedLdsBytes;
     fo.threadsPerBlock = device.threadsPerBlock();
-    if (device.schedule < 0 || device.schedule > 4) throw std::invalid_argument("ComputeUnit: unknown schedule");
+    if (device.schedule < 0 || device.schedule > (int)FusedSchedule::Strided) throw std::invalid_argument("ComputeUnit: unknown schedule");
     fo.schedule = (FusedSchedule)device.schedule;
     fo.wavesPerEU = device.wavesPerEU;
     fo.profile = device.profileTiles;

@@ -1043,6 +1043,234 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         }
     }
     emitWaveScan(os);
+    // Per-template offsets inside a tile's staged region (parameter blocks, slot tables) and the
+    // tile's value count, as expressions over its header words cnt<t> (emitTemplateWords).
+    std::string vsum = "0u", psum = "0u", wsum = "0u";
+    std::vector<std::string> pbase(NT), wbase(NT);
+    for (size_t t = 0; t < NT; ++t) {
+        const FusedTemplate& ft = fp.templates[t];
+        const std::string cnt = "cnt" + std::to_string(t);
+        pbase[t] = psum;
+        wbase[t] = wsum;
+        vsum += " + " + cnt + " * " + std::to_string(ft.valueStride) + "u";
+        psum += " + " + cnt + " * " + std::to_string(ft.numParams) + "u";
+        wsum += " + ((" + cnt + " * " + std::to_string(ft.numNodes) + "u + 1u) >> 1)";
+    }
+    // A0's node numbers sit in the header's table entry: every 32-node group a wave can take is
+    // loaded up to the entry's capacity (zeros past the list) with the header words, so these
+    // loads and the header's are one round trip
+    const size_t nodeCap = fp.headerStride - fp.headerNodes;
+    const size_t nodeRounds = (nodeCap + 32 * (size_t)(T / 64) - 1) / (32 * (size_t)(T / 64));
+    auto emitNodeNumbers = [&](const std::string& ind, const std::string& H, const std::string& var, bool lanes) {
+        for (size_t k = 0; k < nodeRounds; ++k) {
+            if (lanes)
+                os << ind << "const unsigned u" << k << " = wv + " << k * (size_t)(T / 64) << "u, s" << k << " = (u" << k
+                   << " << 5) + (lane >> 1);\n";
+            os << ind << "const unsigned " << var << k << " = s" << k << " < " << nodeCap << "u ? " << H << "["
+               << fp.headerNodes << "u + s" << k << "] : 0u;\n";
+        }
+    };
+    // Header words of the tile at H.  Every one is read before any direct load to LDS is issued:
+    // read after one, a header word becomes a vector load whose wait also waits for every load
+    // issued before it.
+    auto emitTileWords = [&](const std::string& ind) {
+        os << ind << "const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10];\n"
+           << ind << "const unsigned hrec = H[4], hcnt = H[5], hslt = H[6], hchk = H[7], hcst = H[8], hres = H[9];\n"
+           << ind << "const unsigned nconst = H[11] & 0x0fffffffu, kper = H[11] >> 28;  // stored constants, period\n";
+    };
+    auto emitTemplateWords = [&](const std::string& ind) {
+        for (size_t t = 0; t < NT; ++t) {
+            const size_t h = FUSED_HEADER + 3 * t;
+            os << ind << "const unsigned cnt" << t << " = H[" << h << "], tab" << t << " = H[" << h + 1 << "], prm" << t << " = H["
+               << h + 2 << "];\n";
+        }
+        os << ind << "const unsigned nn = H[13];\n";
+        os << ind << "(void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)kper; (void)lane; (void)wv; (void)nn;\n"
+           << ind << "(void)hrec; (void)hcnt; (void)hslt; (void)hchk; (void)hcst; (void)hres;\n";
+        for (size_t t = 0; t < NT; ++t) os << ind << "(void)tab" << t << "; (void)prm" << t << ";\n";
+        // staged region: parameters, node coordinates (stagedDofs per node), slot tables
+        os << ind << "const unsigned np = " << psum << ", nsw = " << wsum << ";\n";
+        os << ind << "const unsigned nvals = " << vsum << ", nstaged = np + " << fp.stagedDofs << "u * nn + ((nsw + 1u) >> 1);\n";
+    };
+    // A0 with direct loads into Ast / Xs / Es of the tile whose words are in scope
+    auto emitStageDma = [&](bool earlyNodes, const std::string& nodeVar) {
+        // A0 with direct loads: each wave takes 32 tile nodes at a time (two lanes per node:
+        // low and high dword) and issues one direct load per staged dof; the parameter
+        // columns (contiguous in c) and the slot tables (contiguous in the pool) are copied
+        if (earlyNodes) {
+            for (size_t k = 0; k < nodeRounds; ++k) {
+                const std::string ks = std::to_string(k);
+                os << "            if (s" << ks << " < nn) {\n"
+                   << "                const unsigned* src = (const unsigned*)(x + B + " << nodeVar << ks << ") + (lane & 1u);\n";
+                for (int d = 0; d < fp.stagedDofs; ++d) {
+                    os << "                SYMLIB_LDS_DMA4(src";
+                    if (d) os << " + " << 2 * d << "u * N";
+                    os << ", Xs + " << d << "u * nn + (u" << ks << " << 5));\n";
+                }
+                os << "            }\n";
+            }
+        } else {
+            os << "            const INDEX* __restrict__ tn = H + " << fp.headerNodes << "u;\n"
+               << "            for (unsigned u = wv; u < ((nn + 31u) >> 5); u += " << T / 64 << "u) {\n"
+               << "                const unsigned s = (u << 5) + (lane >> 1);\n"
+               << "                if (s < nn) {\n"
+               << "                    const unsigned* src = (const unsigned*)(x + B + tn[s]) + (lane & 1u);\n";
+            for (int d = 0; d < fp.stagedDofs; ++d) {
+                os << "                    SYMLIB_LDS_DMA4(src";
+                if (d) os << " + " << 2 * d << "u * N";
+                os << ", Xs + " << d << "u * nn + (u << 5));\n";
+            }
+            os << "                }\n"
+               << "            }\n";
+        }
+        for (size_t t = 0; t < NT; ++t) {
+            const FusedTemplate& ft = fp.templates[t];
+            if (ft.numParams > 0)
+                os << "            symlib_lds_copy32((const unsigned*)(c + prm" << t << "), (unsigned*)(Ast + " << pbase[t]
+                   << "), " << 2 * ft.numParams << "u * cnt" << t << ", wv, lane);\n";
+            os << "            symlib_lds_copy32(tiles + tab" << t << ", Es + " << wbase[t] << ", (cnt" << t << " * "
+               << ft.numNodes << "u + 1u) >> 1, wv, lane);\n";
+        }
+    };
+    // A1, element evaluation; copyRecipe: the recipe and output constants are copied to LDS by
+    // direct loads issued first, overlapping the evaluation
+    auto emitEval = [&](bool copyRecipe) {
+        std::string total = "0u";
+        for (size_t t = 0; t < NT; ++t) {
+            os << "        const unsigned pc" << t << " = (cnt" << t << " + 63u) & ~63u;\n";
+            total += " + " + std::to_string(groupsOf[t]) + "u * pc" + std::to_string(t);
+        }
+        os << "        {\n";
+        if (copyRecipe)
+            os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n"
+               << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
+        os << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
+           << "                unsigned j = w, Lb = 0;\n";
+        for (size_t t = 0; t < NT; ++t) {
+            const FusedTemplate& ft = fp.templates[t];
+            const std::string ts = std::to_string(t), cnt = "cnt" + ts, pc = "pc" + ts;
+            os << "                if (j < " << groupsOf[t] << "u * " << pc << ") {\n"
+               << "                    unsigned g = 0;\n";
+            for (int g = 1; g < groupsOf[t]; ++g) os << "                    g += j >= " << g << "u * " << pc << " ? 1u : 0u;\n";
+            os << "                    const unsigned i = j - g * " << pc << ";\n"
+               << "                    if (i < " << cnt << ") {\n";
+            for (int g = 0; g < groupsOf[t]; ++g)
+                os << "                        " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
+                   << "symlib_elem" << t << "_" << g << "(Xs, nn, (const unsigned short*)(Es + " << wbase[t] << "), Ast + "
+                   << pbase[t] << ", " << cnt << ", i, V + Lb + i * " << ft.valueStride << "u);\n";
+            os << "                    }\n"
+               << "                    continue;\n"
+               << "                }\n"
+               << "                j -= " << groupsOf[t] << "u * " << pc << ";\n"
+               << "                Lb += " << cnt << " * " << ft.valueStride << "u;\n";
+        }
+        os << "            }\n"
+           << "        }\n";
+    };
+    // B, sums
+    auto emitSums = [&]() {
+        os << "        {\n"
+           << "            const unsigned r0 = hrec;\n"
+           << "            const INDEX* __restrict__ Rn = R;\n"
+           << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (hcnt - r0));\n"
+           << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (hslt - r0));\n"
+           << "            const INDEX* __restrict__ CH = R + (hchk - r0);\n"
+           << "            double* __restrict__ res = out + hres;\n"
+           << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
+           // periodic constants: output block j of kper holds the stored block's constants again
+           << "            const unsigned kblk = kper ? m / kper : 0u;\n"
+           << "            const unsigned* __restrict__ S2 = (const unsigned*)S;  // slot pairs\n"
+           ;
+        if (fp.maxPairs <= FUSED_UNROLLED_PAIRS) {
+            // SUM_CHUNKS chunks per wave and iteration, straight-line: every LDS read of the
+            // chunks is issued before the first add (one chunk: the 256-cloth batch ran 1.5 %
+            // faster than with two, whose extra waves' worth of work unbalances the last
+            // round of chunks more than it hides latency).  A lane reads the program's
+            // longest list length in pairs; words past its list (inside the slot table or the
+            // chunk records after it) become the zero slot's pair, so the sums keep the list
+            // order and the extra adds are + 0.0 (a sum that starts at +0.0 is never -0.0).
+            // A chunk past the last one reads the first chunk's record and stores nothing.
+            // Slots are byte offsets: an address is one mask or shift.
+            constexpr int SUM_CHUNKS = 1;
+            int maxPeriod = 0;
+            for (uint32_t h : fp.tileHeader) maxPeriod = std::max(maxPeriod, (int)(fp.pool[h + 11] >> 28));
+            os << "            const unsigned zp = (nvals << 3) | (nvals << 19);  // the zero slot's byte offset, twice\n"
+               << "            for (unsigned ch0 = wv; ch0 < nch; ch0 += " << SUM_CHUNKS * (T / 64) << "u) {\n";
+            for (int u = 0; u < SUM_CHUNKS; ++u) {
+                const std::string x = "_" + std::to_string(u);
+                if (u == 0)
+                    os << "                const unsigned ch" << x << " = ch0;\n"
+                       << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * ch0;\n";
+                else
+                    os << "                const unsigned ch" << x << " = ch0 + " << u * (T / 64) << "u;\n"
+                       << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * (ch" << x
+                       << " < nch ? ch" << x << " : ch0);\n";
+                os << "                const unsigned oi" << x << " = ch" << x << " * 64u + lane;\n"
+                   << "                const bool act" << x << " = oi" << x << " < m;\n"
+                   << "                const unsigned b8" << x << " = act" << x << " ? (unsigned)C8[oi" << x << "] : 0u;\n"
+                   << "                const unsigned k" << x << " = b8" << x << " & 0x7fu, hc" << x << " = b8" << x << " >> 7;\n"
+                   << "                const unsigned ex" << x << " = symlib_excl_scan64(k" << x << " | (hc" << x << " << 16), lane);\n"
+                   << "                const unsigned b2" << x << " = (cr" << x << "[0] + (ex" << x << " & 0xffffu)) >> 1, np2" << x
+                   << " = k" << x << " >> 1;\n"
+                   << "                const unsigned r" << x << " = cr" << x << "[1] + (unsigned)__popcll((((unsigned long long)cr" << x
+                   << "[4] << 32) | cr" << x << "[3]) & below);\n"
+                   << "                const unsigned pos" << x << " = Rn[2u * r" << x << "] + (oi" << x << " - Rn[2u * r" << x << " + 1u]);\n"
+                   << "                unsigned kidx" << x << " = cr" << x << "[2] + (ex" << x << " >> 16);\n";
+                if (maxPeriod > 1) {
+                    // dof blocks that start at or before the chunk: one scalar correction;
+                    // a block starting inside the chunk: a lane compare (chunk indices are
+                    // uniform, the wave index being a scalar)
+                    os << "                {\n"
+                       << "                    unsigned j = 1, past = 0;\n"
+                       << "                    for (; j < kper && j * kblk <= ch" << x << " * 64u; ++j) past += nconst;\n"
+                       << "                    kidx" << x << " -= past;\n"
+                       << "                    for (; j < kper && j * kblk < ch" << x << " * 64u + 64u; ++j) kidx" << x << " -= oi" << x
+                       << " >= j * kblk ? nconst : 0u;\n"
+                       << "                }\n";
+                }
+                // no constant: the zero slot
+                os << "                const double kc" << x << " = *(hc" << x << " ? K + kidx" << x << " : V + nvals);\n";
+                for (int q = 0; q < fp.maxPairs; ++q)
+                    os << "                const unsigned a" << q << x << " = S2[b2" << x << " + " << q << "u], w" << q << x << " = " << q
+                       << "u < np2" << x << " ? a" << q << x << " : zp;\n";
+            }
+            for (int u = 0; u < SUM_CHUNKS; ++u) {
+                const std::string x = "_" + std::to_string(u);
+                os << "                double s" << x << " = 0.0;\n";
+                for (int q = 0; q < fp.maxPairs; ++q)
+                    os << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " & 0xffffu));\n"
+                       << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " >> 16));\n";
+                os << "                if (act" << x << ") res[pos" << x << "] = s" << x << " + kc" << x << ";\n";
+            }
+            os << "            }\n";
+        } else {
+            os << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
+               << "                const unsigned oi = ch * 64u + lane;\n"
+               << "                const bool act = oi < m;\n"
+               // the chunk record and the lane's count byte in one round of LDS reads
+               << "                const INDEX* __restrict__ cr = CH + " << FUSED_CHUNK_WORDS << "u * ch;\n"
+               << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
+               << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
+               << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
+               << "                const unsigned base = cr[0] + (ex & 0xffffu);  // even: lists have even length\n"
+               // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
+               << "                const unsigned r = cr[1] + (unsigned)__popcll((((unsigned long long)cr[4] << 32) | cr[3]) & below);\n"
+               << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
+               << "                unsigned kidx = cr[2] + (ex >> 16);\n"
+               << "                for (unsigned j = 1; j < kper; ++j) kidx -= oi >= j * kblk ? nconst : 0u;\n"
+               << "                double kc = 0.0;\n"
+               << "                if (hc) kc = K[kidx];\n"
+               << "                double s = 0.0;\n"
+               << "                for (unsigned q = 0; q < k; q += 2u) {\n"
+               << "                    const unsigned w = S2[(base + q) >> 1];\n"
+               << "                    s += *(const double*)((const char*)V + (w & 0xffffu));\n"
+               << "                    s += *(const double*)((const char*)V + (w >> 16));\n"
+               << "                }\n"
+               << "                if (act) res[pos] = hc ? s + kc : s;\n"
+               << "            }\n";
+        }
+        os << "        }\n";
+        };
     // Kernel variants: 0 fused (stage, evaluate into LDS, sum), 1 split elements (stage, evaluate
     // into the tile's image in HBM), 2 split sums (copy image, recipe and constants to LDS, sum).
     auto kernel = [&](const char* name, int variant) {
@@ -1066,44 +1294,10 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         // A0's node numbers sit in the header's table entry: every 32-node group a wave can take
         // is loaded up to the entry's capacity (zeros past the list) first, so these loads and
         // the header's are one round trip
-        const size_t nodeCap = fp.headerStride - fp.headerNodes;
-        const size_t nodeRounds = (nodeCap + 32 * (size_t)(T / 64) - 1) / (32 * (size_t)(T / 64));
         const bool earlyNodes = stage && opt.ldsDma && nodeRounds <= 4;
-        if (earlyNodes)
-            for (size_t k = 0; k < nodeRounds; ++k)
-                os << "        const unsigned u" << k << " = wv + " << k * (size_t)(T / 64) << "u, s" << k << " = (u" << k
-                   << " << 5) + (lane >> 1);\n"
-                   << "        const unsigned node" << k << " = s" << k << " < " << nodeCap << "u ? H[" << fp.headerNodes << "u + s" << k
-                   << "] : 0u;\n";
-        // recipe offsets too: read after a direct load to LDS, a header word becomes a vector load
-        // whose wait also waits for every load issued before it
-        os << "        const unsigned B = H[0], N = H[1], m = H[2], nch = H[3], words = H[10];\n"
-           << "        const unsigned hrec = H[4], hcnt = H[5], hslt = H[6], hchk = H[7], hcst = H[8], hres = H[9];\n"
-           << "        const unsigned nconst = H[11] & 0x0fffffffu, kper = H[11] >> 28;  // stored constants, period\n";
-        std::string vsum = "0u", psum = "0u", wsum = "0u";
-        std::vector<std::string> pbase(NT), wbase(NT);
-        for (size_t t = 0; t < NT; ++t) {
-            const size_t h = FUSED_HEADER + 3 * t;
-            const FusedTemplate& ft = fp.templates[t];
-            const std::string cnt = "cnt" + std::to_string(t);
-            // the template's slot table and parameter block offsets are read here, before any
-            // direct load to LDS is issued: read after one, a header word becomes a vector load
-            // whose wait also waits for every load issued before it
-            os << "        const unsigned " << cnt << " = H[" << h << "], tab" << t << " = H[" << h + 1 << "], prm" << t << " = H["
-               << h + 2 << "];\n";
-            pbase[t] = psum;
-            wbase[t] = wsum;
-            vsum += " + " + cnt + " * " + std::to_string(ft.valueStride) + "u";
-            psum += " + " + cnt + " * " + std::to_string(ft.numParams) + "u";
-            wsum += " + ((" + cnt + " * " + std::to_string(ft.numNodes) + "u + 1u) >> 1)";
-        }
-        os << "        const unsigned nn = H[13];\n";
-        os << "        (void)B; (void)N; (void)m; (void)nch; (void)words; (void)nconst; (void)kper; (void)lane; (void)wv; (void)nn;\n"
-           << "        (void)hrec; (void)hcnt; (void)hslt; (void)hchk; (void)hcst; (void)hres;\n";
-        for (size_t t = 0; t < NT; ++t) os << "        (void)tab" << t << "; (void)prm" << t << ";\n";
-        // staged region: parameters, node coordinates (stagedDofs per node), slot tables
-        os << "        const unsigned np = " << psum << ", nsw = " << wsum << ";\n";
-        os << "        const unsigned nvals = " << vsum << ", nstaged = np + " << fp.stagedDofs << "u * nn + ((nsw + 1u) >> 1);\n";
+        if (earlyNodes) emitNodeNumbers("        ", "H", "node", true);
+        emitTileWords("        ");
+        emitTemplateWords("        ");
         // V[nvals]: the tile's zero (odd contribution lists are padded with its slot)
         if (variant == 0) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
         if (variant == 1) os << "        double* __restrict__ V = y + H[12];\n        double* __restrict__ Ast = lds;\n";
@@ -1116,43 +1310,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         os << "        {\n";
         if (sums) os << "            if (threadIdx.x == 0u) V[nvals] = 0.0;\n";
         if (stage && opt.ldsDma) {
-            // A0 with direct loads: each wave takes 32 tile nodes at a time (two lanes per node:
-            // low and high dword) and issues one direct load per staged dof; the parameter
-            // columns (contiguous in c) and the slot tables (contiguous in the pool) are copied
-            if (earlyNodes) {
-                for (size_t k = 0; k < nodeRounds; ++k) {
-                    const std::string ks = std::to_string(k);
-                    os << "            if (s" << ks << " < nn) {\n"
-                       << "                const unsigned* src = (const unsigned*)(x + B + node" << ks << ") + (lane & 1u);\n";
-                    for (int d = 0; d < fp.stagedDofs; ++d) {
-                        os << "                SYMLIB_LDS_DMA4(src";
-                        if (d) os << " + " << 2 * d << "u * N";
-                        os << ", Xs + " << d << "u * nn + (u" << ks << " << 5));\n";
-                    }
-                    os << "            }\n";
-                }
-            } else {
-                os << "            const INDEX* __restrict__ tn = H + " << fp.headerNodes << "u;\n"
-                   << "            for (unsigned u = wv; u < ((nn + 31u) >> 5); u += " << T / 64 << "u) {\n"
-                   << "                const unsigned s = (u << 5) + (lane >> 1);\n"
-                   << "                if (s < nn) {\n"
-                   << "                    const unsigned* src = (const unsigned*)(x + B + tn[s]) + (lane & 1u);\n";
-                for (int d = 0; d < fp.stagedDofs; ++d) {
-                    os << "                    SYMLIB_LDS_DMA4(src";
-                    if (d) os << " + " << 2 * d << "u * N";
-                    os << ", Xs + " << d << "u * nn + (u << 5));\n";
-                }
-                os << "                }\n"
-                   << "            }\n";
-            }
-            for (size_t t = 0; t < NT; ++t) {
-                const FusedTemplate& ft = fp.templates[t];
-                if (ft.numParams > 0)
-                    os << "            symlib_lds_copy32((const unsigned*)(c + prm" << t << "), (unsigned*)(Ast + " << pbase[t]
-                       << "), " << 2 * ft.numParams << "u * cnt" << t << ", wv, lane);\n";
-                os << "            symlib_lds_copy32(tiles + tab" << t << ", Es + " << wbase[t] << ", (cnt" << t << " * "
-                   << ft.numNodes << "u + 1u) >> 1, wv, lane);\n";
-            }
+            emitStageDma(earlyNodes, "node");
         } else if (stage) {
             // A0 through registers: node coordinates, parameters and slot tables, one value per
             // thread and step
@@ -1204,145 +1362,13 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             // fastest; each (template, group) segment is padded to whole waves, so every wave
             // runs one element function (no divergent pair of them) and idle lanes sit at the
             // ends of segments
-            std::string total = "0u";
-            for (size_t t = 0; t < NT; ++t) {
-                os << "        const unsigned pc" << t << " = (cnt" << t << " + 63u) & ~63u;\n";
-                total += " + " + std::to_string(groupsOf[t]) + "u * pc" + std::to_string(t);
-            }
-            os << "        {\n";
-            if (opt.ldsDma && sums)
-                os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n"
-                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
-            os << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
-               << "                unsigned j = w, Lb = 0;\n";
-            for (size_t t = 0; t < NT; ++t) {
-                const FusedTemplate& ft = fp.templates[t];
-                const std::string ts = std::to_string(t), cnt = "cnt" + ts, pc = "pc" + ts;
-                os << "                if (j < " << groupsOf[t] << "u * " << pc << ") {\n"
-                   << "                    unsigned g = 0;\n";
-                for (int g = 1; g < groupsOf[t]; ++g) os << "                    g += j >= " << g << "u * " << pc << " ? 1u : 0u;\n";
-                os << "                    const unsigned i = j - g * " << pc << ";\n"
-                   << "                    if (i < " << cnt << ") {\n";
-                for (int g = 0; g < groupsOf[t]; ++g)
-                    os << "                        " << (g ? "else " : "") << (g + 1 < groupsOf[t] ? "if (g == " + std::to_string(g) + "u) " : "")
-                       << "symlib_elem" << t << "_" << g << "(Xs, nn, (const unsigned short*)(Es + " << wbase[t] << "), Ast + "
-                       << pbase[t] << ", " << cnt << ", i, V + Lb + i * " << ft.valueStride << "u);\n";
-                os << "                    }\n"
-                   << "                    continue;\n"
-                   << "                }\n"
-                   << "                j -= " << groupsOf[t] << "u * " << pc << ";\n"
-                   << "                Lb += " << cnt << " * " << ft.valueStride << "u;\n";
-            }
-            os << "            }\n"
-               << "        }\n";
+            emitEval(opt.ldsDma && sums);
             if (opt.ldsDma && sums) os << "        __builtin_amdgcn_s_waitcnt(0);\n";
             if (sums) os << "        __syncthreads();\n";
             stamp(os, 2);
 
         }
-        if (sums) {
-            os << "        {\n"
-               << "            const unsigned r0 = hrec;\n"
-               << "            const INDEX* __restrict__ Rn = R;\n"
-               << "            const unsigned char* __restrict__ C8 = (const unsigned char*)(R + (hcnt - r0));\n"
-               << "            const unsigned short* __restrict__ S = (const unsigned short*)(R + (hslt - r0));\n"
-               << "            const INDEX* __restrict__ CH = R + (hchk - r0);\n"
-               << "            double* __restrict__ res = out + hres;\n"
-               << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
-               // periodic constants: output block j of kper holds the stored block's constants again
-               << "            const unsigned kblk = kper ? m / kper : 0u;\n"
-               << "            const unsigned* __restrict__ S2 = (const unsigned*)S;  // slot pairs\n"
-               ;
-            if (fp.maxPairs <= FUSED_UNROLLED_PAIRS) {
-                // SUM_CHUNKS chunks per wave and iteration, straight-line: every LDS read of the
-                // chunks is issued before the first add (one chunk: the 256-cloth batch ran 1.5 %
-                // faster than with two, whose extra waves' worth of work unbalances the last
-                // round of chunks more than it hides latency).  A lane reads the program's
-                // longest list length in pairs; words past its list (inside the slot table or the
-                // chunk records after it) become the zero slot's pair, so the sums keep the list
-                // order and the extra adds are + 0.0 (a sum that starts at +0.0 is never -0.0).
-                // A chunk past the last one reads the first chunk's record and stores nothing.
-                // Slots are byte offsets: an address is one mask or shift.
-                constexpr int SUM_CHUNKS = 1;
-                int maxPeriod = 0;
-                for (uint32_t h : fp.tileHeader) maxPeriod = std::max(maxPeriod, (int)(fp.pool[h + 11] >> 28));
-                os << "            const unsigned zp = (nvals << 3) | (nvals << 19);  // the zero slot's byte offset, twice\n"
-                   << "            for (unsigned ch0 = wv; ch0 < nch; ch0 += " << SUM_CHUNKS * (T / 64) << "u) {\n";
-                for (int u = 0; u < SUM_CHUNKS; ++u) {
-                    const std::string x = "_" + std::to_string(u);
-                    if (u == 0)
-                        os << "                const unsigned ch" << x << " = ch0;\n"
-                           << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * ch0;\n";
-                    else
-                        os << "                const unsigned ch" << x << " = ch0 + " << u * (T / 64) << "u;\n"
-                           << "                const INDEX* __restrict__ cr" << x << " = CH + " << FUSED_CHUNK_WORDS << "u * (ch" << x
-                           << " < nch ? ch" << x << " : ch0);\n";
-                    os << "                const unsigned oi" << x << " = ch" << x << " * 64u + lane;\n"
-                       << "                const bool act" << x << " = oi" << x << " < m;\n"
-                       << "                const unsigned b8" << x << " = act" << x << " ? (unsigned)C8[oi" << x << "] : 0u;\n"
-                       << "                const unsigned k" << x << " = b8" << x << " & 0x7fu, hc" << x << " = b8" << x << " >> 7;\n"
-                       << "                const unsigned ex" << x << " = symlib_excl_scan64(k" << x << " | (hc" << x << " << 16), lane);\n"
-                       << "                const unsigned b2" << x << " = (cr" << x << "[0] + (ex" << x << " & 0xffffu)) >> 1, np2" << x
-                       << " = k" << x << " >> 1;\n"
-                       << "                const unsigned r" << x << " = cr" << x << "[1] + (unsigned)__popcll((((unsigned long long)cr" << x
-                       << "[4] << 32) | cr" << x << "[3]) & below);\n"
-                       << "                const unsigned pos" << x << " = Rn[2u * r" << x << "] + (oi" << x << " - Rn[2u * r" << x << " + 1u]);\n"
-                       << "                unsigned kidx" << x << " = cr" << x << "[2] + (ex" << x << " >> 16);\n";
-                    if (maxPeriod > 1) {
-                        // dof blocks that start at or before the chunk: one scalar correction;
-                        // a block starting inside the chunk: a lane compare (chunk indices are
-                        // uniform, the wave index being a scalar)
-                        os << "                {\n"
-                           << "                    unsigned j = 1, past = 0;\n"
-                           << "                    for (; j < kper && j * kblk <= ch" << x << " * 64u; ++j) past += nconst;\n"
-                           << "                    kidx" << x << " -= past;\n"
-                           << "                    for (; j < kper && j * kblk < ch" << x << " * 64u + 64u; ++j) kidx" << x << " -= oi" << x
-                           << " >= j * kblk ? nconst : 0u;\n"
-                           << "                }\n";
-                    }
-                    // no constant: the zero slot
-                    os << "                const double kc" << x << " = *(hc" << x << " ? K + kidx" << x << " : V + nvals);\n";
-                    for (int q = 0; q < fp.maxPairs; ++q)
-                        os << "                const unsigned a" << q << x << " = S2[b2" << x << " + " << q << "u], w" << q << x << " = " << q
-                           << "u < np2" << x << " ? a" << q << x << " : zp;\n";
-                }
-                for (int u = 0; u < SUM_CHUNKS; ++u) {
-                    const std::string x = "_" + std::to_string(u);
-                    os << "                double s" << x << " = 0.0;\n";
-                    for (int q = 0; q < fp.maxPairs; ++q)
-                        os << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " & 0xffffu));\n"
-                           << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " >> 16));\n";
-                    os << "                if (act" << x << ") res[pos" << x << "] = s" << x << " + kc" << x << ";\n";
-                }
-                os << "            }\n";
-            } else {
-                os << "            for (unsigned ch = wv; ch < nch; ch += " << T / 64 << "u) {\n"
-                   << "                const unsigned oi = ch * 64u + lane;\n"
-                   << "                const bool act = oi < m;\n"
-                   // the chunk record and the lane's count byte in one round of LDS reads
-                   << "                const INDEX* __restrict__ cr = CH + " << FUSED_CHUNK_WORDS << "u * ch;\n"
-                   << "                const unsigned b8 = act ? (unsigned)C8[oi] : 0u;\n"
-                   << "                const unsigned k = b8 & 0x7fu, hc = b8 >> 7;\n"
-                   << "                const unsigned ex = symlib_excl_scan64(k | (hc << 16), lane);\n"
-                   << "                const unsigned base = cr[0] + (ex & 0xffffu);  // even: lists have even length\n"
-                   // the lane's run: the chunk's first run plus the runs starting at lanes 1..lane
-                   << "                const unsigned r = cr[1] + (unsigned)__popcll((((unsigned long long)cr[4] << 32) | cr[3]) & below);\n"
-                   << "                const unsigned pos = Rn[2u * r] + (oi - Rn[2u * r + 1u]);\n"
-                   << "                unsigned kidx = cr[2] + (ex >> 16);\n"
-                   << "                for (unsigned j = 1; j < kper; ++j) kidx -= oi >= j * kblk ? nconst : 0u;\n"
-                   << "                double kc = 0.0;\n"
-                   << "                if (hc) kc = K[kidx];\n"
-                   << "                double s = 0.0;\n"
-                   << "                for (unsigned q = 0; q < k; q += 2u) {\n"
-                   << "                    const unsigned w = S2[(base + q) >> 1];\n"
-                   << "                    s += *(const double*)((const char*)V + (w & 0xffffu));\n"
-                   << "                    s += *(const double*)((const char*)V + (w >> 16));\n"
-                   << "                }\n"
-                   << "                if (act) res[pos] = hc ? s + kc : s;\n"
-                   << "            }\n";
-            }
-            os << "        }\n";
-        }
+        if (sums) emitSums();
         os << "        __syncthreads();\n";
         if (variant != 1) stamp(os, 3);
         if (counter) {
