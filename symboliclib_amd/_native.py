@@ -20,7 +20,7 @@ class Options(ctypes.Structure):
                 ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int), ("fused", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("output_groups", ctypes.c_int), ("waves_per_eu", ctypes.c_int),
                 ("profile_tiles", ctypes.c_int), ("split", ctypes.c_int),
-                ("register_staging", ctypes.c_int)]
+                ("register_staging", ctypes.c_int), ("cpu_threads", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):
@@ -132,6 +132,14 @@ SIGNATURES = {
     "symcost_destroy": (None, [_VP]),
     "symcost_last_error": (ctypes.c_char_p, []),
     "symlib_last_error": (ctypes.c_char_p, []),
+    # symcpu.h
+    "symcpu_create": (_I, [ctypes.POINTER(_VP), ctypes.POINTER(ProgramDesc)]),
+    "symcpu_compile": (_I, [_VP, ctypes.c_char_p, ctypes.c_char_p]),
+    "symcpu_set_arg": (_I, [_VP, _I, _DP, _SZ]),
+    "symcpu_run": (_I, [_VP, _I]),
+    "symcpu_get_result": (_I, [_VP, _I, _DP, _SZ]),
+    "symcpu_destroy": (None, [_VP]),
+    "symcpu_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None
@@ -156,5 +164,5 @@ def check(rc, what):
     if rc != 0:
         L = lib()
         msg = ((L.symlib_last_error() or b"").decode() or (L.symhip_last_error() or b"").decode()
-               or (L.symcost_last_error() or b"").decode())
+               or (L.symcost_last_error() or b"").decode() or (L.symcpu_last_error() or b"").decode())
         raise SymlibError(f"{what} failed ({rc}): {msg}")

@@ -36,7 +36,9 @@ Sym::Device makeDevice(const symlib_options* opt) {
     symlib_options o;
     symlib_default_options(&o);
     if (opt) o = *opt;
-    Sym::Device d(Sym::UseHIP(), Sym::ThreadsPerBlock(o.threads_per_block > 0 ? o.threads_per_block : 256));
+    // cpu_threads > 0: the reference's CPU device, Device(VecWidth(4), NumThreads(n))
+    Sym::Device d = o.cpu_threads > 0 ? Sym::Device(Sym::VecWidth(4), Sym::NumThreads(o.cpu_threads))
+                                      : Sym::Device(Sym::UseHIP(), Sym::ThreadsPerBlock(o.threads_per_block > 0 ? o.threads_per_block : 256));
     if (o.decomposition_threshold >= 0) d.decompositionThreshold = (unsigned)o.decomposition_threshold;
     d.simplifyTemplates = o.simplify != 0;
     d.tiled = o.tiled != 0;
@@ -177,6 +179,7 @@ void symlib_default_options(symlib_options* opt) {
     opt->profile_tiles = 0;
     opt->split = 0;
     opt->register_staging = 0;
+    opt->cpu_threads = 0;
 }
 
 int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const int* rowidx, const symlib_options* opt) {

@@ -7,9 +7,8 @@
 
 namespace Sym {
 
-ComputeUnitBase::ComputeUnitBase(const Device& d) : device(d) {
-    if (!device.hipDevice)
-        throw std::invalid_argument("ComputeUnit: this build implements the UseHIP device only (add UseHIP() to the Device)");
+ComputeUnitBase::ComputeUnitBase(const Device& d) : device(d), cpuDevice(!d.hipDevice) {
+    if (device.cudaDevice) throw std::invalid_argument("ComputeUnit: this build has no CUDA device (use UseHIP())");
     if (device.singlePrecision) throw std::invalid_argument("ComputeUnit: this build evaluates in FP64 only");
 }
 
@@ -20,11 +19,16 @@ void ComputeUnitBase::close() {
         symhip_destroy(rt);
         rt = nullptr;
     }
+    if (cpu) {
+        symcpu_destroy(cpu);
+        cpu = nullptr;
+    }
 }
 
 void ComputeUnitBase::check(int rc, const char* what) const {
     if (rc != SYMHIP_SUCCESS)
-        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + symhip_last_error());
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " +
+                                 (cpuDevice ? symcpu_last_error() : symhip_last_error()));
 }
 
 // Inputs are containers of variables {0..len-1, g} of one group g; every other container is
@@ -77,6 +81,7 @@ void ComputeUnitBase::init() {
 void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
     BuildOptions opt;
     opt.simplifyTemplates = device.simplifyTemplates;
+    if (device.fused && cpuDevice) throw std::invalid_argument("ComputeUnit: the fused form runs on the UseHIP device only");
     if (device.fused) {
         FusedOptions fo;
         fo.ldsBytes = device.fusedLdsBytes;
@@ -109,6 +114,7 @@ void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
 }
 
 void ComputeUnitBase::initFromProducts(const ProductSum& ps) {
+    if (cpuDevice) throw std::invalid_argument("ComputeUnit: product tiles run on the UseHIP device only");
     FusedOptions fo;
     fo.ldsBytes = device.fusedLdsBytes;
     fo.threadsPerBlock = device.threadsPerBlock();
@@ -144,6 +150,12 @@ void ComputeUnitBase::finalize() {
     EmitOptions eo;
     eo.threadsPerBlock = device.threadsPerBlock();
     tiledForm = false;
+    if (cpuDevice) {
+        if (device.tiled) throw std::invalid_argument("ComputeUnit: the tiled form runs on the UseHIP device only");
+        code = emitCpu(program);
+        launches.clear();
+        return;
+    }
     if (device.tiled) {
         TileOptions to;
         to.ldsBytes = device.tileLdsBytes;
@@ -214,6 +226,18 @@ void ComputeUnitBase::compileProgram() {
     d.num_launches = (uint32_t)ls.size();
     d.launches = ls.data();
 
+    if (cpuDevice) {
+        symcpu_t hc = nullptr;
+        check(symcpu_create(&hc, &d), "symcpu_create");
+        const int rc = symcpu_compile(hc, code.c_str(), nullptr);
+        if (rc) {
+            const std::string msg = symcpu_last_error();
+            symcpu_destroy(hc);
+            throw std::runtime_error("symcpu_compile failed (" + std::to_string(rc) + "): " + msg);
+        }
+        cpu = hc;
+        return;
+    }
     symhip_t h = nullptr;
     check(symhip_create(&h, &d), "symhip_create");
     int rc = symhip_compile(h, code.c_str(), nullptr);
@@ -226,15 +250,17 @@ void ComputeUnitBase::compileProgram() {
 }
 
 void ComputeUnitBase::setArgImpl(int i, const double* d, size_t n) {
-    if (!rt) throw std::runtime_error("ComputeUnit: execute() before compile()");
+    if (!compiled()) throw std::runtime_error("ComputeUnit: execute() before compile()");
     if (i < 0 || (size_t)i >= io.inputSizes.size()) throw std::invalid_argument("ComputeUnit: too many input containers");
     if (n != io.inputSizes[i]) throw std::invalid_argument("ComputeUnit: input container " + std::to_string(i) + " has the wrong size");
-    check(symhip_set_arg(rt, i, d, n), "symhip_set_arg");
+    if (cpu) check(symcpu_set_arg(cpu, i, d, n), "symcpu_set_arg");
+    else check(symhip_set_arg(rt, i, d, n), "symhip_set_arg");
 }
 
 void ComputeUnitBase::runImpl(void* stream) {
-    if (!rt) throw std::runtime_error("ComputeUnit: run before compile()");
-    check(symhip_run(rt, stream), "symhip_run");
+    if (!compiled()) throw std::runtime_error("ComputeUnit: run before compile()");
+    if (cpu) check(symcpu_run(cpu, (int)std::max(1u, device.numThreads)), "symcpu_run");
+    else check(symhip_run(rt, stream), "symhip_run");
 }
 
 void ComputeUnitBase::retuneFused(const FusedOptions& fo) {
@@ -284,15 +310,16 @@ std::vector<uint64_t> ComputeUnitBase::tileProfile() const {
 }
 
 void ComputeUnitBase::runGraph(void* stream) {
-    if (!rt) throw std::runtime_error("ComputeUnit: run before compile()");
+    if (!rt) throw std::runtime_error("ComputeUnit: graph replay needs a compiled UseHIP unit");
     check(symhip_run_graph(rt, stream), "symhip_run_graph");
 }
 
 void ComputeUnitBase::getResultImpl(int k, double* d, size_t n) {
-    if (!rt) throw std::runtime_error("ComputeUnit: getResults() before compile()");
+    if (!compiled()) throw std::runtime_error("ComputeUnit: getResults() before compile()");
     if (k < 0 || (size_t)k >= io.outputSizes.size()) throw std::invalid_argument("ComputeUnit: too many output containers");
     if (n != io.outputSizes[k]) throw std::invalid_argument("ComputeUnit: output container " + std::to_string(k) + " has the wrong size");
-    check(symhip_get_result(rt, k, d, n), "symhip_get_result");
+    if (cpu) check(symcpu_get_result(cpu, k, d, n), "symcpu_get_result");
+    else check(symhip_get_result(rt, k, d, n), "symhip_get_result");
 }
 
 }  // namespace Sym

@@ -10,11 +10,13 @@
 // variables {0..len-1, group} are inputs, all others are outputs.  execute() takes the
 // numeric inputs in the same order, getResults() fills the outputs in the same order.
 // Underneath, the unit calls the C runtime declared in symhip.h: hipRTC compilation in
-// process, device-resident tables, one launch per wave of independent kernels.
+// process, device-resident tables, one launch per wave of independent kernels.  A Device
+// without UseHIP() is the reference's CPU device (VecWidth, NumThreads): the same program
+// emitted as host C++ with OpenMP, compiled with the reference's flags (symcpu.h); it runs the
+// waves form only.
 //
 // Differences from the reference: errors throw (std::invalid_argument, std::runtime_error)
-// instead of printing and degrading to no-ops; only the UseHIP device with RealT = double is
-// implemented (the CPU and CUDA back-ends are outside this build).
+// instead of printing and degrading to no-ops; RealT = double only; no CUDA device.
 #pragma once
 
 #include <memory>
@@ -27,6 +29,7 @@
 #include "ContainerSupport.hpp"
 #include "Program.hpp"
 #include "Symbolic.hpp"
+#include "symcpu.h"
 #include "symhip.h"
 
 namespace Sym {
@@ -159,7 +162,9 @@ protected:
     FusedCodeStats fusedStats;
     bool fusedForm = false;
     std::string tilingNote;
+    bool cpuDevice = false;  // a Device without UseHIP(): host code through symcpu
     symhip_t rt = nullptr;
+    symcpu_t cpu = nullptr;
     int numOutputContainers = 0;
 
     std::vector<SparseMatrix<double>> assemblyPatterns;
@@ -182,7 +187,8 @@ public:
     virtual ~ComputeUnitBase();
 
     void close();
-    bool compiled() const { return rt != nullptr; }
+    bool compiled() const { return rt != nullptr || cpu != nullptr; }
+    bool onCpu() const { return cpuDevice; }
 
     const Program& getProgram() const { return program; }
     const std::string& getCode() const { return code; }

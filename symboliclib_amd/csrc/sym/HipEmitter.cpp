@@ -701,6 +701,48 @@ void scheduleStats(const std::vector<Stmt>& S, const std::vector<size_t>& order,
 
 }  // namespace
 
+// CPU form of a program, after the reference's CPU path (src/ComputeUnit.cpp:409-437,
+// src/StringResources.h:312-480): every kernel is a loop over its instances running the same
+// straight-line template as the GPU waves form, split over OpenMP threads and left to the host
+// compiler to vectorise; kernels of one wave share one parallel region (no barrier between
+// them), waves run in order.  Entry point:
+//   extern "C" void symlib_cpu_run(const double* x, double* y, const double* c,
+//                                  const INDEX* p, const INDEX* o, double* out, int threads)
+std::string emitCpu(const Program& prog) {
+    const size_t maxIdx = std::max({prog.index.size(), prog.consts.size(), prog.outpos.size(), prog.numValues()});
+    const bool wide = maxIdx >= (size_t(1) << 31);
+    std::ostringstream os;
+    os << "// generated by symboliclib_amd: CPU form of a program (OpenMP over instances), FP64\n"
+       << "#include <math.h>\n"
+       << "#include <stddef.h>\n"
+       << "typedef unsigned int INDEX;\n\n";
+    for (int w = 0; w < prog.numWaves; ++w) {
+        os << "static void symlib_cpu_wave" << w
+           << "(const double* __restrict__ x, double* y, const double* __restrict__ c, const INDEX* __restrict__ p,\n"
+           << "    const INDEX* __restrict__ o, double* __restrict__ out, int threads) {\n"
+           << "    (void)x; (void)y; (void)c; (void)p; (void)o; (void)out;\n"
+           << "#pragma omp parallel num_threads(threads)\n"
+           << "    {\n";
+        for (const auto& K : prog.kernels) {
+            if (K.wave != w || K.n == 0) continue;
+            os << "#pragma omp for schedule(static) nowait\n"
+               << "        for (" << (wide ? "size_t" : "unsigned") << " i = 0; i < " << K.n << (wide ? "ull" : "u") << "; ++i) {\n";
+            KernelEmitter ke{prog, K, wide};
+            const std::vector<Stmt> S = flatten(K.code, true, true);
+            const std::vector<char> all(S.size(), 1);
+            for (size_t si : lowPressureOrder(S, all)) ke.statements(S[si].s, os, "            ");
+            os << "        }\n";
+        }
+        os << "    }\n"
+           << "}\n\n";
+    }
+    os << "extern \"C\" void symlib_cpu_run(const double* x, double* y, const double* c, const INDEX* p, const INDEX* o,\n"
+       << "    double* out, int threads) {\n";
+    for (int w = 0; w < prog.numWaves; ++w) os << "    symlib_cpu_wave" << w << "(x, y, c, p, o, out, threads);\n";
+    os << "}\n";
+    return os.str();
+}
+
 std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<LaunchDesc>& launches) {
     const int T = opt.threadsPerBlock;
     if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("emitHip: threads per block must be a multiple of 64 up to 1024");

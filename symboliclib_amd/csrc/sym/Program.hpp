@@ -83,4 +83,8 @@ struct EmitOptions {
 // HIP source with one extern "C" kernel per wave, plus the launch list.
 std::string emitHip(const Program& prog, const EmitOptions& opt, std::vector<LaunchDesc>& launches);
 
+// Host C++ source of the same program (OpenMP over instances, one parallel region per wave),
+// entry point symlib_cpu_run (see symcpu.h).
+std::string emitCpu(const Program& prog);
+
 }  // namespace Sym
