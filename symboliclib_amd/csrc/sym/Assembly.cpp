@@ -118,7 +118,7 @@ void HessianAssembly::addNodalConstant(int numNodes, const int* nodes, const dou
 namespace {
 
 // Store only base columns; a column equal to a stored one plus a constant reads that one.
-void compressColumns(std::vector<std::vector<uint32_t>>& cols, std::vector<Kernel::VariableAccessData>& access,
+void compressColumns(std::vector<std::vector<uint32_t>>& cols, std::vector<ColumnRef>& access,
                      const std::vector<char>& compressible) {
     std::vector<std::vector<uint32_t>> kept;
     std::vector<int> keptSrc;

@@ -1,267 +1,199 @@
 #include "ComputeKernel.hpp"
 
-#include <map>
+#include <algorithm>
 #include <stdexcept>
-#include <unordered_set>
+#include <unordered_map>
 
 #include "Decomposition.hpp"
 #include "Simplify.hpp"
 #include "Template.hpp"
-#include "Traverse.hpp"
-#include "Utilities.hpp"
 
 namespace Sym {
 
 namespace {
 
-hash_t hashSymbolic(const std::vector<Symbolic>& x) {
-    std::vector<hash_t> hashes;
-    hashes.reserve(x.size());
-    for (auto& xi : x) hashes.push_back(xi.ahash());
-    return hash(hashes);
-}
+using Column = std::vector<Symbolic>;  // the nodes of every instance at one template position
 
-// The same child position of every instance, visited together.
-struct StackData {
-    std::vector<Symbolic> x;
-    hash_t h = 0;
-    unsigned childCounter = 0;
+// Builds the template of a group by walking its instances in lock step.
+class GroupWalker {
+public:
+    GroupWalker(const std::vector<Symbolic>& instances, KernelTemplate& K) : roots_(instances), K_(K) {}
 
-    StackData(std::vector<Symbolic>&& data) : x(std::move(data)) { h = hashSymbolic(x); }
-
-    StackData(const std::vector<Symbolic>& parents, unsigned k) {
-        x.reserve(parents.size());
-        for (auto& d : parents) x.push_back(d[k]);
-        h = hashSymbolic(x);
+    Symbolic run() {
+        struct Frame {
+            Column col;
+            bool outputPosition;
+            unsigned next = 0;
+            std::vector<Symbolic> childs;
+        };
+        std::vector<Frame> frames;
+        frames.push_back({roots_, false, 0, {}});
+        Symbolic result;
+        while (!frames.empty()) {
+            Frame& f = frames.back();
+            if (f.next == 0 && f.childs.empty()) {
+                checkUniform(f.col);
+                const hash_t sig = signature(f.col);
+                auto hit = shared_.find(sig);
+                Symbolic leaf;
+                if (hit != shared_.end()) leaf = hit->second;
+                else if (f.col.front().op() == VAR) leaf = variableLeaf(f.col, f.outputPosition);
+                else if (f.col.front().op() == CONST) leaf = constantLeaf(f.col);
+                if (leaf.op() != NOOP) {
+                    if (hit == shared_.end()) shared_.emplace(sig, leaf);
+                    if (!deliver(frames, leaf, result)) break;
+                    continue;
+                }
+            }
+            const Symbolic& head = f.col.front();
+            if (f.next < head.numChilds()) {
+                const unsigned k = f.next++;
+                // output positions: the left side of an assignment, the output arguments of an
+                // expression template
+                bool out = head.op() == ASSIGN && k == 0;
+                if (head.op() == INSTANCE) out = (int)k < getTemplate(head.tag()).numOutputs;
+                Column child;
+                child.reserve(f.col.size());
+                for (const Symbolic& x : f.col) child.push_back(x[k]);
+                frames.push_back({std::move(child), out, 0, {}});
+                continue;
+            }
+            const Symbolic node =
+                head.op() == INSTANCE ? Symbolic::instance(head.tag(), f.childs, 0) : Symbolic(head.op(), f.childs);
+            shared_.emplace(signature(f.col), node);
+            if (!deliver(frames, node, result)) break;
+        }
+        return result;
     }
 
-    bool visit() { return childCounter++ == numChilds(); }
-    unsigned numChilds() const { return x.empty() ? 0 : x.front().numChilds(); }
-    OpType op() const { return x.empty() ? NOOP : x.front().op(); }
+private:
+    const std::vector<Symbolic>& roots_;
+    KernelTemplate& K_;
+    std::unordered_map<hash_t, Symbolic> shared_;  // column signature -> template node
 
-    bool isValid() const {
-        const OpType o = op();
-        const unsigned nc = numChilds();
-        const uint32_t tag = x.empty() ? 0 : x.front().tag();
-        for (auto& y : x)
-            if (y.op() != o || y.numChilds() != nc || (o == INSTANCE && y.tag() != tag)) return false;
+    // hand a finished node to the frame below; false when it was the root
+    template <class Frames>
+    static bool deliver(Frames& frames, const Symbolic& node, Symbolic& result) {
+        frames.pop_back();
+        if (frames.empty()) {
+            result = node;
+            return false;
+        }
+        frames.back().childs.push_back(node);
         return true;
     }
 
-    // per-instance variables of this position (never collapsed: every instance keeps its entry)
-    std::vector<std::array<int, 2>> getVariables() const {
-        std::vector<std::array<int, 2>> vals(x.size());
-        for (size_t i = 0; i < x.size(); ++i) vals[i] = x[i].variable();
-        return vals;
+    static hash_t signature(const Column& col) {
+        std::vector<hash_t> h;
+        h.reserve(col.size() + 1);
+        for (const Symbolic& x : col) h.push_back(x.ahash());
+        h.push_back((hash_t)col.size());
+        return hash(h);
     }
 
-    // per-instance constants; a single value when all instances agree
-    std::vector<double> getConstants() const {
-        std::vector<double> vals(x.size());
-        bool allIdentical = true;
-        for (size_t i = 0; i < x.size(); ++i) {
-            vals[i] = x[i].constant();
-            if (allIdentical && !(vals[0] == vals[i])) allIdentical = false;
-        }
-        if (allIdentical) vals.resize(1);
-        return vals;
+    static void checkUniform(const Column& col) {
+        const Symbolic& a = col.front();
+        for (const Symbolic& b : col)
+            if (b.op() != a.op() || b.numChilds() != a.numChilds() || (a.op() == INSTANCE && b.tag() != a.tag()))
+                throw std::logic_error("kernel: the instances of a group differ in structure");
     }
+
+    Symbolic variableLeaf(const Column& col, bool output) {
+        std::vector<std::array<int, 2>> vars;
+        vars.reserve(col.size());
+        for (const Symbolic& x : col) vars.push_back(x.variable());
+        if (output) {
+            K_.outputColumns.push_back(std::move(vars));
+            return Symbolic((int)K_.outputColumns.size() - 1, -2);
+        }
+        pendingInputs_.push_back(std::move(vars));
+        return Symbolic((int)pendingInputs_.size() - 1, -3);
+    }
+
+    Symbolic constantLeaf(const Column& col) {
+        const double c0 = col.front().constant();
+        bool uniform = true;
+        std::vector<double> vals;
+        vals.reserve(col.size());
+        for (const Symbolic& x : col) {
+            vals.push_back(x.constant());
+            uniform = uniform && x.constant() == c0;
+        }
+        if (uniform) return Symbolic(c0);
+        K_.constantColumns.push_back(std::move(vals));
+        return Symbolic((int)K_.constantColumns.size() - 1, -4);
+    }
+
+public:
+    // input leaves in template order; stored after compression
+    std::vector<std::vector<std::array<int, 2>>> pendingInputs_;
 };
+
+// Store the input columns, each column that is another stored column shifted by a constant
+// (one input group in every instance) as a reference to it.
+void storeInputColumns(std::vector<std::vector<std::array<int, 2>>>& leaves, KernelTemplate& K) {
+    std::unordered_map<hash_t, std::vector<int>> byPattern;  // (group, offsets from the first) -> stored columns
+    for (auto& col : leaves) {
+        const int g = col.front()[1];
+        bool oneGroup = g != GLOBAL_INTERMEDIATE && g != EXPLICIT_INTERMEDIATE;
+        std::vector<long long> pattern{g};
+        for (const auto& v : col) {
+            oneGroup = oneGroup && v[1] == g;
+            pattern.push_back((long long)v[0] - col.front()[0]);
+        }
+        int found = -1;
+        hash_t key = 0;
+        if (oneGroup) {
+            key = hash(pattern);
+            for (int s : byPattern[key]) {
+                const auto& stored = K.inputColumns[(size_t)s];
+                bool same = stored.size() == col.size() && stored.front()[1] == g;
+                for (size_t i = 1; same && i < col.size(); ++i)
+                    same = (long long)stored[i][0] - stored.front()[0] == pattern[i + 1];
+                if (same) {
+                    found = s;
+                    break;
+                }
+            }
+        }
+        if (found >= 0) {
+            K.inputLeaves.push_back({found, col.front()[0] - K.inputColumns[(size_t)found].front()[0]});
+            continue;
+        }
+        if (oneGroup) byPattern[key].push_back((int)K.inputColumns.size());
+        K.inputLeaves.push_back({(int)K.inputColumns.size(), 0});
+        K.inputColumns.push_back(std::move(col));
+    }
+}
 
 }  // namespace
 
-Kernel::Kernel(const std::vector<Symbolic>& expressions, const int simdWidth_, const bool simplifyTemplate)
-    : n(expressions.size()), simdWidth(simdWidth_) {
-    if (expressions.empty()) throw std::invalid_argument("Kernel: empty expression group");
+KernelTemplate buildKernelTemplate(const std::vector<Symbolic>& instances, bool simplifyTemplate) {
+    if (instances.empty()) throw std::invalid_argument("kernel: empty group");
+    KernelTemplate K;
+    K.instances = instances.size();
+    GroupWalker walker(instances, K);
+    const Symbolic t = walker.run();
+    storeInputColumns(walker.pendingInputs_, K);
+    K.code = optimizeTemplate(t, simplifyTemplate);
+    return K;
+}
 
-    std::map<hash_t, Symbolic> xprMap;
-    std::vector<StackData> stack;
-    stack.emplace_back(std::vector<Symbolic>(expressions));
-    std::vector<Symbolic> tStack;
-
-    while (!stack.empty()) {
-        StackData& curr = stack.back();
-        if (!curr.isValid()) throw std::logic_error("Kernel: instances of a group differ in structure");
-
-        if (curr.visit()) {
-            const unsigned nc = curr.numChilds();
-            std::vector<Symbolic> ch(tStack.end() - nc, tStack.end());
-            tStack.erase(tStack.end() - nc, tStack.end());
-            Symbolic txpr = curr.op() == INSTANCE ? Symbolic::instance(curr.x.front().tag(), ch, 0) : Symbolic(curr.op(), ch);
-            tStack.push_back(txpr);
-            if (xprMap.find(curr.h) == xprMap.end()) xprMap[curr.h] = txpr;
-            stack.pop_back();
-        } else {
-            const unsigned k = curr.childCounter - 1;
-            StackData data(curr.x, k);
-            auto it = xprMap.find(data.h);
-            if (it != xprMap.end()) {
-                tStack.push_back(it->second);
-            } else if (data.op() == VAR) {
-                bool isOutput = curr.op() == ASSIGN && k == 0;
-                if (curr.op() == INSTANCE) isOutput = (int)k < getTemplate(curr.x.front().tag()).numOutputs;
-                if (isOutput) {
-                    tStack.push_back(Symbolic((int)outputVariableTable.size(), -2));
-                    outputVariableTable.push_back(data.getVariables());
-                } else {
-                    tStack.push_back(Symbolic((int)variableTable.size(), -3));
-                    variableTable.push_back(data.getVariables());
-                }
-                xprMap[data.h] = tStack.back();
-            } else if (data.op() == CONST) {
-                auto values = data.getConstants();
-                if (values.size() == 1) {
-                    tStack.push_back(Symbolic(values.front()));
-                } else {
-                    tStack.push_back(Symbolic((int)constantTable.size(), -4));
-                    constantTable.push_back(std::move(values));
-                }
-                xprMap[data.h] = tStack.back();
-            } else {
-                stack.push_back(std::move(data));  // invalidates `curr`
-            }
-        }
+Symbolic optimizeTemplate(Symbolic t, bool simplifyTemplate) {
+    if (t.op() == INSTANCE) t = expandInstance(t);
+    t = foldConstants(t);
+    std::vector<DecomposedBlock> blocks = decompose({t}, 0, LOCAL_VAR, false);
+    // a block only reads blocks of lower height: ascending heights put definitions first
+    std::stable_sort(blocks.begin(), blocks.end(),
+                     [](const DecomposedBlock& a, const DecomposedBlock& b) { return a.height < b.height; });
+    std::vector<Symbolic> statements;
+    statements.reserve(blocks.size());
+    for (const DecomposedBlock& b : blocks) {
+        Symbolic s = simplifyTemplate ? simplify(b.code) : b.code;
+        if (s.op() == ASSIGN && s[0].op() == VAR && s[0].variable()[1] == LOCAL_VAR) s = Symbolic(DEFINE, s[0], s[1]);
+        statements.push_back(s);
     }
-
-    // pad to a multiple of the SIMD width (CPU code only)
-    if (simdWidth > 1 && n > (size_t)simdWidth && n % simdWidth) {
-        const size_t pad = simdWidth - n % simdWidth;
-        for (auto& cnst : constantTable) cnst.insert(cnst.end(), pad, 0.0);
-        for (auto& vars : variableTable) vars.insert(vars.end(), pad, vars.back());
-        for (auto& vars : outputVariableTable) vars.insert(vars.end(), pad, std::array<int, 2>{-1, -1});
-        n += pad;
-    }
-
-    assert(tStack.size() == 1);
-    templateExpression = tStack.front();
-    optimizeTemplateExpression(simplifyTemplate);
-}
-
-void Kernel::optimizeTemplateExpression(bool simplifyTemplate) {
-    templateExpression = optimizeTemplate(templateExpression, simplifyTemplate);
-}
-
-Symbolic optimizeTemplate(Symbolic templateExpression, bool simplifyTemplate) {
-    if (templateExpression.op() == INSTANCE) templateExpression = expandInstance(templateExpression);
-
-    // fold constants; algebraically equal sub-expressions become one shared node
-    templateExpression = removeConstantExpressions(templateExpression);
-
-    std::vector<Symbolic> vec{templateExpression};
-    auto blocks = decompose(vec, 0, LOCAL_VAR, false);
-
-    if (simplifyTemplate)
-        for (auto& b : blocks) b.x = simplify(b.x);
-
-    for (auto& b : blocks)
-        if (b.x.op() == ASSIGN && b.x[0].op() == VAR && b.x[0].variable()[1] == LOCAL_VAR)
-            b.x = Symbolic(DEFINE, b.x[0], b.x[1]);
-
-    // definitions before their uses
-    std::vector<std::unordered_set<int>> graph(blocks.size());
-    for (size_t i = 0; i < blocks.size(); ++i)
-        for (auto& c : blocks[i].childs) graph[c].insert((int)i);
-
-    auto order = topologicalOrder(graph);
-    if (order.size() != blocks.size()) throw std::logic_error("Kernel: cyclic local definitions");
-    std::vector<Symbolic> kernelExpressions;
-    kernelExpressions.reserve(order.size());
-    for (int i : order) kernelExpressions.push_back(blocks[i].x);
-    return Symbolic(BLOCK, kernelExpressions);
-}
-
-int Kernel::getPackLength() {
-    getConstantData();
-    getOutputVariableData();
-    getVariableData();
-    assert(constantsPackLength == variablePackLength && constantsPackLength == outputVariablePackLength);
-    return variablePackLength;
-}
-
-const std::vector<double>& Kernel::getConstantData() {
-    if (constantsPackLength < 0) constantsPackLength = appendTransposed(constantTable, constantData, simdWidth);
-    return constantData;
-}
-
-const std::vector<std::array<int, 2>>& Kernel::getOutputVariableData() {
-    if (outputVariablePackLength < 0)
-        outputVariablePackLength = appendTransposed(outputVariableTable, outputVariableData, simdWidth);
-    return outputVariableData;
-}
-
-const std::vector<std::array<int, 2>>& Kernel::getVariableData() {
-    if (variablePackLength < 0) {
-        findCorrelatedVariables();
-        variablePackLength = appendTransposed(variableTable, variableData, simdWidth);
-    }
-    return variableData;
-}
-
-const std::vector<Kernel::VariableAccessData>& Kernel::getVariableAccess() {
-    findCorrelatedVariables();
-    return variableAccessData;
-}
-
-size_t Kernel::numVariableSlots() {
-    findCorrelatedVariables();
-    return variableTable.size();
-}
-
-// A column whose entries equal another column's plus one constant (same input group in
-// every instance) is not stored; its leaves read the other column with that offset.
-void Kernel::findCorrelatedVariables() {
-    if (correlated) return;
-    correlated = true;
-
-    std::map<hash_t, std::vector<int>> varMap;  // pattern hash -> candidate base columns
-    std::vector<std::vector<int>> patterns(variableTable.size());
-    for (size_t i = 0; i < variableTable.size(); ++i) {
-        const auto& vars = variableTable[i];
-        bool flag = true;
-        std::vector<int>& vals = patterns[i];
-        vals.push_back(vars[0][1]);
-        for (auto& v : vars) {
-            if (v[1] != vars[0][1]) {
-                flag = false;
-                break;
-            }
-            vals.push_back(v[0] - vars[0][0]);
-        }
-        if (!flag || vars[0][1] == GLOBAL_INTERMEDIATE || vars[0][1] == EXPLICIT_INTERMEDIATE) {
-            variableAccessData.push_back({(int)i, 0});
-            continue;
-        }
-        const hash_t h = hash(vals);
-        auto& cands = varMap[h];
-        int found = -1;
-        for (int c : cands)
-            if (patterns[c] == vals) {
-                found = c;
-                break;
-            }
-        if (found < 0) {
-            cands.push_back((int)i);
-            variableAccessData.push_back({(int)i, 0});
-        } else {
-            variableAccessData.push_back({found, vars[0][0] - variableTable[found][0][0]});
-        }
-    }
-
-    // keep only base columns
-    std::map<int, int> varIds;
-    std::vector<std::vector<std::array<int, 2>>> variableTable2;
-    for (auto& va : variableAccessData) {
-        auto it = varIds.find(va.baseVariableId);
-        if (it == varIds.end()) {
-            const int nid = (int)variableTable2.size();
-            varIds[va.baseVariableId] = nid;
-            variableTable2.push_back(variableTable[va.baseVariableId]);
-            va.baseVariableId = nid;
-        } else {
-            va.baseVariableId = it->second;
-        }
-    }
-    variableTable.swap(variableTable2);
+    return Symbolic(BLOCK, statements);
 }
 
 }  // namespace Sym

@@ -1,15 +1,18 @@
-// A kernel = one template expression evaluated for many instances.
+// Kernel templates: one expression evaluated for many instances (the job of the reference's
+// src/ComputeKernel.cpp:27-484).
 //
-// Follows src/ComputeKernel.hpp and src/ComputeKernel.cpp:27-484 of the reference: all
-// instances of a structure group are traversed in lock-step; wherever the instances differ,
-// the template reads a per-instance table instead:
-//   VAR(k, -2)  output variable table k   (the left side of an assignment)
-//   VAR(k, -3)  input variable table k    (gathered values)
-//   VAR(k, -4)  constant table k          (constants that differ between instances)
-// Constants equal in every instance stay literals.  The template then goes through local
-// common-sub-expression extraction (LOCAL_VAR definitions) and algebraic simplification.
-// Tables are laid out column after column (packLength 0) for the GPU, or in blocks of
-// `simdWidth` rows for vectorised CPU code.
+// The instances of a group (blocks of equal shape) are walked position by position.  A
+// position where every instance holds the same constant stays a literal; any other leaf
+// becomes a per-instance column the template reads through a leaf variable:
+//   VAR(k, -2)  output column k    (the variables an instance assigns)
+//   VAR(k, -3)  input leaf k       (a variable an instance reads: column and offset in
+//                                   inputLeaves)
+//   VAR(k, -4)  constant column k  (constants that differ between instances)
+// Positions whose instance nodes are the same in every instance (equal per-instance
+// algebraic hashes) share one template node.  An input column equal to another stored column
+// plus a constant (one input group in every instance) is not stored again: its leaf reads the
+// stored column with that offset.  Tables are structure of arrays (column c, instance i at
+// c * n + i).  The template then goes through optimizeTemplate.
 #pragma once
 
 #include <array>
@@ -19,68 +22,28 @@
 
 namespace Sym {
 
-class Kernel {
-public:
-    struct VariableAccessData {
-        int baseVariableId;  // table column actually stored
-        int offset;          // constant added to the stored index
-    };
-
-private:
-    size_t n;
-    int simdWidth;
-    Symbolic templateExpression;
-    size_t globalOutputVariableOffset = 0;
-    size_t globalVariableOffset = 0;
-    size_t globalConstantsOffset = 0;
-
-    int outputVariablePackLength = -1;
-    std::vector<std::array<int, 2>> outputVariableData;
-    std::vector<std::vector<std::array<int, 2>>> outputVariableTable;
-
-    int constantsPackLength = -1;
-    std::vector<double> constantData;
-    std::vector<std::vector<double>> constantTable;
-
-    int variablePackLength = -1;
-    std::vector<std::array<int, 2>> variableData;
-    std::vector<std::vector<std::array<int, 2>>> variableTable;
-    std::vector<VariableAccessData> variableAccessData;
-    bool correlated = false;
-
-    void findCorrelatedVariables();
-
-public:
-    Kernel(const std::vector<Symbolic>& expressions, const int simdWidth = 1, const bool simplifyTemplate = true);
-
-    void optimizeTemplateExpression(bool simplifyTemplate);
-
-    int getPackLength();
-    void setGlobalOutputOffset(const size_t offset) { globalOutputVariableOffset = offset; }
-    void setGlobalIndexOffset(const size_t offset) { globalVariableOffset = offset; }
-    void setGlobalConstantsOffset(const size_t offset) { globalConstantsOffset = offset; }
-    size_t getGlobalOutputOffset() const { return globalOutputVariableOffset; }
-    size_t getGlobalIndexOffset() const { return globalVariableOffset; }
-    size_t getGlobalConstantsOffset() const { return globalConstantsOffset; }
-
-    int numInstances() const { return (int)n; }
-    size_t numOutputSlots() const { return outputVariableTable.size(); }
-    size_t numConstantSlots() const { return constantTable.size(); }
-    size_t numVariableSlots();  // after correlation compression
-
-    const std::vector<double>& getConstantData();
-    const std::vector<std::array<int, 2>>& getOutputVariableData();
-    const std::vector<std::array<int, 2>>& getVariableData();
-    const std::vector<VariableAccessData>& getVariableAccess();
-    const std::vector<std::vector<std::array<int, 2>>>& getOutputVariableTable() const { return outputVariableTable; }
-
-    Symbolic getTemplateExpression() const { return templateExpression; }
+struct ColumnRef {
+    int column;  // stored input column
+    int offset;  // constant added to the stored variable index
 };
 
-// Template optimisation shared by Kernel and the direct assembly builder
-// (src/ComputeKernel.cpp:270-323): expand an INSTANCE, fold constants and merge algebraically
-// equal sub-expressions, extract local definitions (LOCAL_VAR), simplify each block, order
-// definitions before uses.  Returns a BLOCK.
+struct KernelTemplate {
+    size_t instances = 0;
+    Symbolic code;  // BLOCK of statements (optimizeTemplate)
+    std::vector<std::vector<std::array<int, 2>>> outputColumns;  // per output leaf: variable per instance
+    std::vector<std::vector<std::array<int, 2>>> inputColumns;   // stored input columns
+    std::vector<ColumnRef> inputLeaves;                          // per input leaf of the template
+    std::vector<std::vector<double>> constantColumns;            // per constant leaf: value per instance
+};
+
+// instances: blocks of one group (equal shape); throws std::logic_error when they differ.
+KernelTemplate buildKernelTemplate(const std::vector<Symbolic>& instances, bool simplifyTemplate);
+
+// Template optimisation shared by the kernel builder and the direct assembly builders (the job
+// of src/ComputeKernel.cpp:270-323): expand an INSTANCE, fold constants (algebraically equal
+// sub-expressions become one node), turn every sub-expression used twice into a local
+// definition (LOCAL_VAR, DEFINE), simplify each statement, definitions before their uses.
+// Returns a BLOCK.
 Symbolic optimizeTemplate(Symbolic templateExpression, bool simplifyTemplate);
 
 }  // namespace Sym

@@ -45,19 +45,23 @@ DenseMatrix<Symbolic> makeSymbolic(const DenseMatrix<T>& A, const int objId) {
     return B;
 }
 
-// (row, col, value) tuples to an n x n sparse matrix; with makeSymmetric every off-diagonal
-// tuple is mirrored.  Duplicates are summed.
+// n x n sparse matrix from (row, col, value) entries, summing entries that meet at one
+// position.  With `mirror`, each off-diagonal entry is entered a second time at (col, row);
+// the mirrored entries come after all the given ones (so a position sums its given entries
+// first).
 template <class T>
-void toSparseMatrix(SparseMatrix<T>& A, const std::vector<std::tuple<int, int, T>>& tuples, const size_t n,
-                    const bool makeSymmetric = false) {
-    std::vector<Triplet<T>> triplets;
-    triplets.reserve((makeSymmetric ? 2 : 1) * tuples.size());
-    for (const auto& t : tuples) triplets.emplace_back(std::get<0>(t), std::get<1>(t), std::get<2>(t));
-    if (makeSymmetric)
-        for (const auto& t : tuples)
-            if (std::get<1>(t) != std::get<0>(t)) triplets.emplace_back(std::get<1>(t), std::get<0>(t), std::get<2>(t));
+void toSparseMatrix(SparseMatrix<T>& A, const std::vector<std::tuple<int, int, T>>& entries, const size_t n,
+                    const bool mirror = false) {
+    std::vector<Triplet<T>> all;
+    all.reserve(entries.size() * (mirror ? 2 : 1));
+    for (int pass = 0; pass < (mirror ? 2 : 1); ++pass)
+        for (const auto& e : entries) {
+            const int r = std::get<0>(e), c = std::get<1>(e);
+            if (pass == 0) all.emplace_back(r, c, std::get<2>(e));
+            else if (r != c) all.emplace_back(c, r, std::get<2>(e));
+        }
     A.resize((int)n, (int)n);
-    A.setFromTriplets(triplets.begin(), triplets.end());
+    A.setFromTriplets(all.begin(), all.end());
 }
 
 }  // namespace Sym

@@ -1,364 +1,308 @@
+// Decomposition into blocks, in five passes over an explicit graph of candidate nodes:
+//   1. census: walk every root, entering each distinct node (by identity) once, and count how
+//      often each node of complexity >= threshold is reached (by identity: a value written out
+//      twice as two separate trees is computed twice); a FIXED sub-expression names its sealed
+//      BLOCK, which is a node of its own and is not entered;
+//   2. graph: the nodes are the roots, the sealed blocks and the sub-expressions counted at
+//      least twice; walking a node's expression down to the next nodes gives its edges and its
+//      own cost (operations outside other nodes);
+//   3. inlining: a node that is cheap on its own, or (optionally) read by one node only, is
+//      written into its readers instead of becoming a block;
+//   4. blocks: the remaining nodes, each reading the blocks reached through inlined nodes;
+//      heights by a post-order walk of that graph;
+//   5. extraction: each block's expression with nodes that are blocks replaced by their
+//      variables, the operands of commutative operations ordered by shape hash, and its shape.
 #include "Decomposition.hpp"
 
 #include <algorithm>
-#include <chrono>
-#include <cstdio>
-#include <cstdlib>
-#include <map>
-#include <set>
+#include <stdexcept>
 #include <unordered_map>
-
-#include "IdMap.hpp"
-#include "Template.hpp"
-#include "Traverse.hpp"
-#include "Utilities.hpp"
+#include <unordered_set>
+#include <utility>
 
 namespace Sym {
 
 namespace {
 
-constexpr unsigned char MERGED = 1;
-constexpr unsigned char EXPLICITLY_MERGED = 2;
-constexpr unsigned char FIXED_NODE = 3;
-
-// Children of a node keyed by algebraic hash, kept sorted (iteration in key order, as an
-// ordered map); nodes have few children, so a flat vector beats a tree.
-template <class T>
-class FlatMap {
-    std::vector<std::pair<hash_t, T>> v_;
-
-public:
-    using iterator = typename std::vector<std::pair<hash_t, T>>::iterator;
-    iterator begin() { return v_.begin(); }
-    iterator end() { return v_.end(); }
-    iterator find(hash_t k) {
-        auto it = std::lower_bound(v_.begin(), v_.end(), k, [](const auto& e, hash_t key) { return e.first < key; });
-        return (it != v_.end() && it->first == k) ? it : v_.end();
-    }
-    bool insert(const std::pair<hash_t, T>& e) {
-        auto it = std::lower_bound(v_.begin(), v_.end(), e.first, [](const auto& x, hash_t key) { return x.first < key; });
-        if (it != v_.end() && it->first == e.first) return false;
-        v_.insert(it, e);
-        return true;
-    }
-    void erase(hash_t k) {
-        auto it = find(k);
-        if (it != v_.end()) v_.erase(it);
-    }
-    size_t size() const { return v_.size(); }
+struct Node {
+    Symbolic expr;     // the sub-expression (a sealed node: its BLOCK)
+    bool root = false;
+    bool sealed = false;
+    int ownCost = 0;
+    std::vector<int> edges;  // nodes reached directly, each once
+    int readers = 0;         // distinct nodes (or the caller, for roots) that reach this one
+    bool inlined = false;
+    int block = -1;
 };
 
-// Small identity cache: linear scan while short, hash map beyond.
-class ExtractCache {
-    std::vector<std::pair<long long, std::pair<Symbolic, hash_t>>> small_;
-    std::unordered_map<long long, std::pair<Symbolic, hash_t>, IdentityHash<long long>> big_;
-    bool useBig_ = false;
-
+class Decomposer {
 public:
-    const std::pair<Symbolic, hash_t>* find(long long k) const {
-        if (useBig_) {
-            auto it = big_.find(k);
-            return it == big_.end() ? nullptr : &it->second;
+    Decomposer(unsigned char threshold, int group, bool inlineSingleReader)
+        : threshold_(threshold), group_(group), inlineSingle_(inlineSingleReader) {}
+
+    std::vector<DecomposedBlock> run(const std::vector<Symbolic>& roots) {
+        census(roots);
+        for (const Symbolic& r : roots) {
+            Node n;
+            n.expr = r;
+            n.root = true;
+            n.readers = 1;
+            nodes_.push_back(n);
         }
-        for (auto& e : small_)
-            if (e.first == k) return &e.second;
-        return nullptr;
+        for (size_t i = 0; i < nodes_.size(); ++i) linkNode((int)i);
+        for (Node& n : nodes_)
+            for (int c : n.edges) ++nodes_[(size_t)c].readers;
+        for (Node& n : nodes_)
+            if (!n.root && !n.sealed) n.inlined = n.ownCost < threshold_ || (inlineSingle_ && n.readers == 1);
+        return buildBlocks();
     }
-    void put(long long k, const std::pair<Symbolic, hash_t>& v) {
-        if (!useBig_ && small_.size() < 32) {
-            small_.emplace_back(k, v);
+
+private:
+    const int threshold_;
+    const int group_;
+    const bool inlineSingle_;
+    std::vector<Node> nodes_;
+    std::unordered_map<hash_t, int> nodeOf_;  // algebraic hash -> node (shared and sealed nodes)
+
+    bool tracked(const Symbolic& y) const { return y.complexity() >= threshold_; }
+
+    // pass 1: occurrence counts; a node is made at its second occurrence (sealed blocks at the
+    // first), so the node order follows the walk; later nodes algebraically equal to one made
+    // before are that node
+    void census(const std::vector<Symbolic>& roots) {
+        std::unordered_map<long long, unsigned> seen;
+        std::unordered_set<long long> entered;
+        std::vector<Symbolic> todo;
+        for (const Symbolic& root : roots) {
+            todo.push_back(root);
+            while (!todo.empty()) {
+                const Symbolic y = todo.back();
+                todo.pop_back();
+                if (y.op() == FIXED) {
+                    const hash_t h = y[1].ahash();
+                    if (!nodeOf_.count(h)) {
+                        Node n;
+                        n.expr = y[1];
+                        n.sealed = true;
+                        nodeOf_.emplace(h, (int)nodes_.size());
+                        nodes_.push_back(n);
+                    }
+                    continue;
+                }
+                if (!tracked(y)) continue;
+                const hash_t h = y.ahash();
+                if (++seen[y.id()] == 2 && !nodeOf_.count(h)) {
+                    Node n;
+                    n.expr = y;
+                    nodeOf_.emplace(h, (int)nodes_.size());
+                    nodes_.push_back(n);
+                }
+                if (!entered.insert(y.id()).second) continue;
+                for (unsigned k = y.numChilds(); k-- > 0;) todo.push_back(y[k]);
+            }
+        }
+    }
+
+    // the node a sub-expression stands for, or -1 (never `self`: an expression algebraically
+    // equal to its own node, such as 1 * x inside x, is walked through)
+    int nodeFor(const Symbolic& y, int self) const {
+        if (y.op() == FIXED) return nodeOf_.at(y[1].ahash());
+        if (!tracked(y)) return -1;
+        auto it = nodeOf_.find(y.ahash());
+        return it == nodeOf_.end() || it->second == self ? -1 : it->second;
+    }
+
+    // pass 2: edges and own cost of node i
+    void linkNode(int i) {
+        Node& n = nodes_[(size_t)i];
+        if (n.sealed) {
+            n.ownCost = 255;
             return;
         }
-        if (!useBig_) {
-            for (auto& e : small_) big_.emplace(e.first, e.second);
-            small_.clear();
-            useBig_ = true;
-        }
-        big_[k] = v;
-    }
-};
-
-// Rebuild node `y` with new children (keeps the template id of INSTANCE nodes).
-Symbolic rebuild(const Symbolic& y, const std::vector<Symbolic>& childs) {
-    if (y.op() == INSTANCE) return Symbolic::instance(y.tag(), childs, 0);
-    return Symbolic(y.op(), childs);
-}
-
-class ExpressionNode {
-public:
-    Symbolic x;
-    int id;
-    Symbolic fullx;
-    unsigned char status = 0;
-    unsigned int level = (unsigned)-1;
-    unsigned char complexity = 0;
-    hash_t structureHash = 0;
-    unsigned int numParents = 0;
-    FlatMap<ExpressionNode*> childs;
-
-    ExpressionNode(const Symbolic& fx, int id_) : id(id_), fullx(fx) {}
-
-    hash_t hash() const { return fullx.algebraicHash(); }
-
-    // ids of all non-merged descendants reached through merged nodes
-    std::vector<int> allChildren() {
-        std::vector<int> ret;
-        std::vector<ExpressionNode*> stack{this};
-        while (!stack.empty()) {
-            auto curr = stack.back();
-            stack.pop_back();
-            for (auto& c : curr->childs) {
-                if (c.second->status == MERGED) stack.push_back(c.second);
-                else ret.push_back(c.second->id);
+        std::unordered_set<int> edges;
+        int cost = 0;
+        std::vector<Symbolic> todo{n.expr};
+        while (!todo.empty()) {
+            const Symbolic y = todo.back();
+            todo.pop_back();
+            const int c = nodeFor(y, i);
+            if (c >= 0) {
+                if (edges.insert(c).second) n.edges.push_back(c);
+                continue;
             }
-        }
-        deleteDuplicates(ret);
-        return ret;
-    }
-
-    void addChild(const std::pair<hash_t, ExpressionNode*>& c) {
-        if (childs.insert(c)) ++c.second->numParents;
-    }
-
-    // absorb child n into this node's computation
-    void merge(ExpressionNode& n) {
-        childs.erase(n.hash());
-        if (n.status != EXPLICITLY_MERGED) {
-            for (auto& c : n.childs) --c.second->numParents;
-            n.numParents = 0;
-            n.status = EXPLICITLY_MERGED;
-        }
-        for (auto& c : n.childs) addChild(c);  // n is never this node
-        complexity = (unsigned char)std::min(255, complexity + n.complexity);
-    }
-
-    Symbolic extractExpression(const int variableGroupId) {
-        if (x.op() != NOOP) return x;
-
-        std::vector<std::pair<Symbolic, hash_t>> stack;
-        ExtractCache cache;
-        std::vector<hash_t> hashes;
-
-        auto preFun = [&](const Symbolic& y) {
-            if (auto* hit = cache.find(y.id())) {
-                stack.push_back(*hit);
-                return false;
+            if (!tracked(y)) {
+                cost += y.complexity();
+                continue;
             }
-            if (y.op() == FIXED) {
-                stack.push_back(std::make_pair(y[0], VarHash));
-                return false;
+            cost += std::max(1, (int)y.numChilds()) * OpInfos[y.op()].cost;
+            for (unsigned k = y.numChilds(); k-- > 0;) todo.push_back(y[k]);
+        }
+        n.ownCost = std::min(cost, 255);
+    }
+
+    // pass 4 and 5
+    std::vector<DecomposedBlock> buildBlocks() {
+        std::vector<int> blockNodes;
+        for (size_t i = 0; i < nodes_.size(); ++i)
+            if (!nodes_[i].inlined) {
+                nodes_[i].block = (int)blockNodes.size();
+                blockNodes.push_back((int)i);
             }
-            auto it = childs.find(y.ahash());
-            if (it != childs.end()) {
-                if (it->second->status == MERGED) {
-                    const Symbolic xpr = it->second->extractExpression(variableGroupId);
-                    stack.push_back(std::make_pair(xpr, it->second->structureHash));
-                } else {
-                    stack.push_back(std::make_pair(Symbolic(it->second->id, variableGroupId), VarHash));
+        // blocks read by each node, looking through inlined nodes
+        std::vector<std::vector<int>> reads(nodes_.size());
+        std::vector<char> state(nodes_.size(), 0);  // 0 new, 1 open, 2 done
+        for (size_t start = 0; start < nodes_.size(); ++start) {
+            if (state[start]) continue;
+            std::vector<std::pair<int, size_t>> path{{(int)start, 0}};
+            state[start] = 1;
+            while (!path.empty()) {
+                auto& [v, next] = path.back();
+                const Node& n = nodes_[(size_t)v];
+                if (next < n.edges.size()) {
+                    const int c = n.edges[next++];
+                    if (state[(size_t)c] == 1) throw std::logic_error("decompose: cyclic sub-expressions");
+                    if (!state[(size_t)c]) {
+                        state[(size_t)c] = 1;
+                        path.push_back({c, 0});
+                    }
+                    continue;
                 }
-                return false;
+                std::vector<int> r;
+                for (int c : n.edges) {
+                    const Node& cn = nodes_[(size_t)c];
+                    if (cn.inlined) r.insert(r.end(), reads[(size_t)c].begin(), reads[(size_t)c].end());
+                    else r.push_back(cn.block);
+                }
+                std::sort(r.begin(), r.end());
+                r.erase(std::unique(r.begin(), r.end()), r.end());
+                reads[(size_t)v] = std::move(r);
+                state[(size_t)v] = 2;
+                path.pop_back();
             }
-            return true;
-        };
+        }
+        std::vector<DecomposedBlock> blocks(blockNodes.size());
+        for (size_t b = 0; b < blockNodes.size(); ++b) {
+            blocks[b].reads = reads[(size_t)blockNodes[b]];
+            extract(blockNodes[b], blocks[b]);
+        }
+        // heights: every block's reads have lower indices in a post-order of the read graph
+        std::vector<int> height(blocks.size(), -1);
+        for (size_t s = 0; s < blocks.size(); ++s) {
+            std::vector<std::pair<int, size_t>> path{{(int)s, 0}};
+            while (!path.empty()) {
+                auto& [b, next] = path.back();
+                if (height[(size_t)b] >= 0) {
+                    path.pop_back();
+                    continue;
+                }
+                const auto& r = blocks[(size_t)b].reads;
+                if (next < r.size()) {
+                    const int c = r[next++];
+                    if (height[(size_t)c] < 0) path.push_back({c, 0});
+                    continue;
+                }
+                int h = 0;
+                for (int c : r) h = std::max(h, height[(size_t)c] + 1);
+                height[(size_t)b] = h;
+                path.pop_back();
+            }
+        }
+        for (size_t b = 0; b < blocks.size(); ++b) blocks[b].height = height[b];
+        return blocks;
+    }
 
-        auto postFun = [&](const Symbolic& y) {
+    // pass 5: the expression of block node i and its shape
+    void extract(int i, DecomposedBlock& out) const {
+        const Node& n = nodes_[(size_t)i];
+        struct Built {
+            Symbolic x;
+            hash_t shape;
+        };
+        std::unordered_map<long long, Built> done;  // by identity, within this block
+        struct Frame {
+            Symbolic y;
+            unsigned next;
+        };
+        std::vector<Frame> frames{{n.expr, 0}};
+        std::vector<Built> results;
+        while (!frames.empty()) {
+            Frame& f = frames.back();
+            const Symbolic y = f.y;
+            if (f.next == 0) {
+                // leaves and replacements
+                auto hit = done.find(y.id());
+                if (hit != done.end()) {
+                    results.push_back(hit->second);
+                    frames.pop_back();
+                    continue;
+                }
+                if (y.op() == FIXED) {
+                    // the sealed block assigns this variable
+                    results.push_back({y[0], VarHash});
+                    frames.pop_back();
+                    continue;
+                }
+                // a sealed block is taken whole; elsewhere a node that is a block is read
+                // through its variable, an inlined one is written out in place
+                const int c = n.sealed ? -1 : nodeFor(y, i);
+                if (c >= 0 && !nodes_[(size_t)c].inlined) {
+                    results.push_back({Symbolic(c, group_), VarHash});
+                    frames.pop_back();
+                    continue;
+                }
+                if (y.numChilds() == 0) {
+                    results.push_back({y, y.op() == VAR ? VarHash : ConstHash});
+                    frames.pop_back();
+                    continue;
+                }
+            }
+            if (f.next < y.numChilds()) {
+                const Symbolic child = y[f.next++];
+                frames.push_back({child, 0});
+                continue;
+            }
+            // all operands built: order commutative operands by shape, then hash the shape
             const unsigned nc = y.numChilds();
-            if (nc == 0) {
-                stack.push_back(std::make_pair(y, y.op() == VAR ? VarHash : ConstHash));
-                return;
-            }
+            std::vector<Built> ops(results.end() - nc, results.end());
+            results.erase(results.end() - nc, results.end());
             if (OpInfos[y.op()].commutative)
-                sortBy(stack.end() - nc, stack.end(), [](const auto& e) { return e.second; });
-
-            hashes.clear();
-            const size_t first = stack.size() - nc;
-            for (size_t i = first; i < stack.size(); ++i) hashes.push_back(stack[i].second);
-            hashes.push_back(OpInfos[y.op()].hash);
-            if (y.op() == INSTANCE) hashes.push_back((hash_t)y.tag());
-            static_assert(sizeof(std::pair<Symbolic, hash_t>) == 2 * sizeof(void*), "layout");
-            // children are the first members of consecutive stack entries
-            std::vector<Symbolic> ch;
-            ch.reserve(nc);
-            for (size_t i = first; i < stack.size(); ++i) ch.push_back(std::move(stack[i].first));
-            stack.erase(stack.end() - nc, stack.end());
-            stack.push_back(std::make_pair(rebuild(y, ch), Sym::hash(hashes)));
-            cache.put(y.id(), stack.back());
-        };
-
-        prePostOrderTraverse(fullx, preFun, postFun);
-        assert(stack.size() == 1);
-        x = stack.front().first;
-        structureHash = stack.front().second;
-
-        if (x.op() != BLOCK && x.op() != ASSIGN && x.op() != INSTANCE && status != MERGED)
-            x = Symbolic(ASSIGN, Symbolic(id, variableGroupId), x);
-        return x;
+                std::stable_sort(ops.begin(), ops.end(), [](const Built& a, const Built& b) { return a.shape < b.shape; });
+            std::vector<hash_t> key;
+            key.reserve(nc + 2);
+            std::vector<Symbolic> childs;
+            childs.reserve(nc);
+            for (const Built& o : ops) {
+                key.push_back(o.shape);
+                childs.push_back(o.x);
+            }
+            key.push_back(OpInfos[y.op()].hash);
+            if (y.op() == INSTANCE) key.push_back((hash_t)y.tag());
+            const Symbolic rebuilt = y.op() == INSTANCE ? Symbolic::instance(y.tag(), childs, 0) : Symbolic(y.op(), childs);
+            const Built b{rebuilt, hash(key)};
+            done.emplace(y.id(), b);
+            results.push_back(b);
+            frames.pop_back();
+        }
+        Symbolic x = results.front().x;
+        out.shape = results.front().shape;
+        if (x.op() != BLOCK && x.op() != ASSIGN && x.op() != INSTANCE) x = Symbolic(ASSIGN, Symbolic(i, group_), x);
+        out.code = x;
     }
 };
 
 }  // namespace
 
-std::vector<ExpressionBlock> decompose(const std::vector<Symbolic>& expr, const unsigned char complexityThreshold,
-                                       const unsigned char variableGroupId, const bool mergeSingleParent) {
-    if (complexityThreshold == 255) {
-        std::vector<ExpressionBlock> ret;
-        ret.reserve(expr.size());
-        for (auto& e : expr) ret.emplace_back(e);
-        return ret;
-    }
-
-    std::vector<ExpressionNode> allNodes;
-    allNodes.reserve(2 * expr.size() + 16);
-    // node identity -> (occurrence count, index + 1 in allNodes)
-    IdMap<std::pair<unsigned, unsigned>> exprCount(4 * expr.size() + 16);
-    int id = 0;
-    const bool timing = std::getenv("SYMLIB_TIMING") != nullptr && expr.size() > 100000;
-    auto tprev = std::chrono::steady_clock::now();
-    auto phase = [&](const char* what) {
-        if (!timing) return;
-        auto now = std::chrono::steady_clock::now();
-        std::fprintf(stderr, "[symlib]   decompose: %-18s %8.3f s\n", what, std::chrono::duration<double>(now - tprev).count());
-        tprev = now;
-    };
-
-    // 1. count occurrences of every sufficiently complex sub-expression
-    for (const auto& x : expr) {
-        preOrderTraverse(x, [&](const Symbolic& y) {
-            if (y.op() == FIXED) {
-                auto& xc = exprCount[y[1].id()];
-                if (xc.first++ == 0) {
-                    allNodes.emplace_back(y[1], id++);
-                    allNodes.back().status = FIXED_NODE;
-                    xc.second = (unsigned)allNodes.size();
-                }
-                return false;
-            } else if (y.complexity() < complexityThreshold) {
-                return false;
-            } else {
-                auto& xc = exprCount[y.id()];
-                if (xc.first == 1) {
-                    allNodes.emplace_back(y, id++);
-                    xc.second = (unsigned)allNodes.size();
-                }
-                return xc.first++ == 0;
-            }
-        });
-    }
-
-    phase("count");
-    // 2. every output is a node
-    std::vector<unsigned> outputNodes;
-    outputNodes.reserve(expr.size());
-    for (auto& x : expr) {
-        allNodes.emplace_back(x, id++);
-        allNodes.back().numParents++;
-        outputNodes.push_back((unsigned)allNodes.size() - 1);
-        exprCount[x.id()].second = (unsigned)allNodes.size();
-    }
-
-    phase("outputs");
-    // 3. link nodes to the nodes they contain and estimate their own cost
-    for (auto& node : allNodes) {
-        if (node.status == FIXED_NODE) {
-            node.complexity = 255;
-            continue;
+std::vector<DecomposedBlock> decompose(const std::vector<Symbolic>& roots, unsigned char threshold, int group,
+                                       bool inlineSingleReader) {
+    if (threshold == 255) {
+        std::vector<DecomposedBlock> blocks(roots.size());
+        for (size_t i = 0; i < roots.size(); ++i) {
+            blocks[i].code = roots[i];
+            blocks[i].shape = roots[i].computeStructureHash();
         }
-        int comp = 0;
-        preOrderTraverse(node.fullx, [&](const Symbolic& y) {
-            if (y.op() == FIXED) {
-                auto& data = exprCount[y[1].id()];
-                node.addChild(std::make_pair(y[1].ahash(), allNodes.data() + data.second - 1));
-                return false;
-            }
-            if (y.complexity() < complexityThreshold) {
-                comp += y.complexity();
-                return false;
-            }
-            auto* it = exprCount.find(y.id());
-            const unsigned second = it ? it->second : 0;
-            if (second != 0 && y.ahash() != node.hash()) {
-                node.addChild(std::make_pair(y.ahash(), allNodes.data() + second - 1));
-                return false;
-            }
-            comp += std::max(1, (int)y.numChilds()) * OpInfos[y.op()].cost;
-            return true;
-        });
-        node.complexity = (unsigned char)std::min(255, comp);
+        return blocks;
     }
-
-    phase("children");
-    // 4. merge cheap or single-parent children into their parents and assign levels
-    std::vector<ExpressionNode*> stack;
-    // membership flags by node index; iteration in index order = allNodes order
-    std::vector<char> reachable(allNodes.size(), 0);
-    auto reach = [&](ExpressionNode* n) {
-        char& r = reachable[n - allNodes.data()];
-        const bool fresh = !r;
-        r = 1;
-        return fresh;
-    };
-    std::vector<ExpressionNode*> childStack;
-
-    for (auto i : outputNodes) {
-        ExpressionNode* x = allNodes.data() + i;
-        stack.clear();
-        stack.push_back(x);
-        x->level = 0;
-
-        while (!stack.empty()) {
-            ExpressionNode* node = stack.back();
-            stack.pop_back();
-
-            if (node->status == FIXED_NODE) {
-                reach(node);
-            } else if (node->status != 0 || reach(node)) {
-                childStack.clear();
-                for (auto& c : node->childs) childStack.push_back(c.second);
-
-                while (!childStack.empty()) {
-                    ExpressionNode* cnode = childStack.back();
-                    childStack.pop_back();
-                    if (cnode->status == FIXED_NODE) {
-                    } else if (cnode->status == EXPLICITLY_MERGED || cnode->complexity < complexityThreshold) {
-                        node->merge(*cnode);
-                        for (auto& c : cnode->childs) childStack.push_back(c.second);
-                    } else if (mergeSingleParent && cnode->numParents == 1) {
-                        cnode->status = MERGED;
-                        for (auto& c : cnode->childs) stack.push_back(c.second);
-                    }
-                }
-
-                for (auto& c : node->childs) {
-                    c.second->level = node->level + 1;
-                    stack.push_back(c.second);
-                }
-            }
-        }
-    }
-
-    phase("merge/levels");
-    // 5. extract one block per remaining node
-    std::vector<int> blockIdMap(allNodes.size(), -1);
-    std::vector<ExpressionBlock> blocks;
-    for (size_t xi = 0; xi < allNodes.size(); ++xi) {
-        if (!reachable[xi]) continue;
-        ExpressionNode* x = allNodes.data() + xi;
-        if (x->status != MERGED) {
-            blockIdMap[x->id] = (int)blocks.size();
-            ExpressionBlock block;
-            block.level = (int)x->level;
-            block.x = x->extractExpression(variableGroupId);
-            block.structureHash = x->structureHash;
-            block.childs = x->allChildren();
-            blocks.push_back(std::move(block));
-        }
-    }
-
-    phase("extract");
-    for (auto& b : blocks)
-        for (auto& i : b.childs) i = blockIdMap[i];
-
-    return blocks;
+    return Decomposer(threshold, group, inlineSingleReader).run(roots);
 }
 
 }  // namespace Sym

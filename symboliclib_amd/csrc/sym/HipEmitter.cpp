@@ -88,8 +88,8 @@ struct KernelEmitter {
             }
             case -3: {
                 const auto& va = K.access.at((size_t)slot);
-                const int kind = tiled->colKind[tiled->colOffset[kernelIndex] + va.baseVariableId];
-                const std::string e = "D[" + std::to_string(va.baseVariableId) + "u * cnt + j]";
+                const int kind = tiled->colKind[tiled->colOffset[kernelIndex] + va.column];
+                const std::string e = "D[" + std::to_string(va.column) + "u * cnt + j]";
                 const std::string off = va.offset ? ((va.offset > 0 ? " + " : " - ") + std::to_string(std::abs(va.offset)) + "u") : "";
                 if (kind == 0) return "x[" + e + off + "]";
                 if (kind == 1) return "lds[" + e + " & 0x7fffffffu]";
@@ -116,7 +116,7 @@ struct KernelEmitter {
             }
             case -3: {
                 const auto& va = K.access.at((size_t)slot);
-                std::string s = "x[p[" + idx(K.indexOffset + (size_t)va.baseVariableId * K.n, wide) + "]";
+                std::string s = "x[p[" + idx(K.indexOffset + (size_t)va.column * K.n, wide) + "]";
                 if (va.offset) s += (va.offset > 0 ? " + " : " - ") + std::to_string(std::abs(va.offset)) + (wide ? "" : "u");
                 return s + "]";
             }

@@ -9,18 +9,18 @@
 
 #include "Decomposition.hpp"
 #include "Timer.hpp"
-#include "Utilities.hpp"
 
 namespace Sym {
 
 namespace {
 
-// Blocks of equal structure at equal level form one group (src/ComputeUnit.cpp:33-56).
-std::vector<std::vector<int>> group(const std::vector<ExpressionBlock>& blocks) {
+// Blocks of equal shape at equal height form one group, the instances of one kernel (the
+// role of src/ComputeUnit.cpp:33-56); blocks of equal height never read each other.
+std::vector<std::vector<int>> group(const std::vector<DecomposedBlock>& blocks) {
     std::unordered_map<hash_t, int> groupMap;
     std::vector<std::vector<int>> ret;
     for (int i = 0; i < (int)blocks.size(); ++i) {
-        const hash_t h = hash(blocks[i].structureHash, (long long)blocks[i].level);
+        const hash_t h = hash(blocks[i].shape, (long long)blocks[i].height);
         int& id = groupMap[h];
         if (id) ret[id - 1].push_back(i);
         else {
@@ -44,15 +44,15 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
     timer("decompose");
     auto groupIds = group(blocks);
     std::vector<int> blockGroup(blocks.size());
-    std::vector<Kernel> kernels;
+    std::vector<KernelTemplate> kernels;
     kernels.reserve(groupIds.size());
     for (size_t g = 0; g < groupIds.size(); ++g) {
         std::vector<Symbolic> exprs(groupIds[g].size());
         for (size_t j = 0; j < groupIds[g].size(); ++j) {
-            exprs[j] = blocks[groupIds[g][j]].x;
+            exprs[j] = blocks[groupIds[g][j]].code;
             blockGroup[groupIds[g][j]] = (int)g;
         }
-        kernels.emplace_back(exprs, 0, opt.simplifyTemplates);
+        kernels.push_back(buildKernelTemplate(exprs, opt.simplifyTemplates));
     }
     timer("group + kernel templates");
 
@@ -60,9 +60,18 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
     const size_t nk = kernels.size();
     std::vector<std::set<int>> readers(nk);
     for (size_t i = 0; i < blocks.size(); ++i)
-        for (int j : blocks[i].childs)
+        for (int j : blocks[i].reads)
             if (j != -1 && blockGroup[j] != blockGroup[i]) readers[blockGroup[j]].insert(blockGroup[i]);
-    auto order = topologicalOrder(readers);
+    // kernels in dependency order (a kernel after every kernel it reads), each in the first
+    // wave after all of those
+    std::vector<int> pending(nk, 0), order;
+    for (const auto& rs : readers)
+        for (int r : rs) ++pending[(size_t)r];
+    for (size_t k = 0; k < nk; ++k)
+        if (!pending[k]) order.push_back((int)k);
+    for (size_t q = 0; q < order.size(); ++q)
+        for (int r : readers[(size_t)order[q]])
+            if (--pending[(size_t)r] == 0) order.push_back(r);
     if (order.size() != nk) throw std::logic_error("buildProgram: cyclic kernel dependencies");
 
     std::vector<int> wave(nk, 0);
@@ -82,15 +91,15 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
     prog.kernels.resize(nk);
     for (size_t r = 0; r < nk; ++r) {
         const int k = order[r];
-        Kernel& K = kernels[k];
+        const KernelTemplate& K = kernels[k];
         ProgramKernel& P = prog.kernels[r];
-        P.n = (size_t)K.numInstances();
+        P.n = K.instances;
         P.wave = wave[k];
         P.group = k;
-        P.code = K.getTemplateExpression();
-        P.numOutSlots = K.numOutputSlots();
+        P.code = K.code;
+        P.numOutSlots = K.outputColumns.size();
 
-        const auto& otab = K.getOutputVariableTable();
+        const auto& otab = K.outputColumns;
         bool anyUser = false, anyInter = false;
         for (auto& col : otab)
             for (auto& v : col) {
@@ -128,17 +137,17 @@ Program buildProgram(const std::vector<Symbolic>& outputs, const ProgramInputs& 
     timer("layout");
 
     for (size_t r = 0; r < nk; ++r) {
-        Kernel& K = kernels[order[r]];
+        const KernelTemplate& K = kernels[order[r]];
         ProgramKernel& P = prog.kernels[r];
         P.constOffset = prog.consts.size();
-        const auto& cd = K.getConstantData();
-        prog.consts.insert(prog.consts.end(), cd.begin(), cd.end());
-        P.numConstSlots = K.numConstantSlots();
+        for (const auto& col : K.constantColumns) prog.consts.insert(prog.consts.end(), col.begin(), col.end());
+        P.numConstSlots = K.constantColumns.size();
 
         P.indexOffset = prog.index.size();
-        const auto& vd = K.getVariableData();
-        P.access = K.getVariableAccess();
-        P.numVarSlots = K.numVariableSlots();
+        P.access = K.inputLeaves;
+        P.numVarSlots = K.inputColumns.size();
+        std::vector<std::array<int, 2>> vd;
+        for (const auto& col : K.inputColumns) vd.insert(vd.end(), col.begin(), col.end());
         for (auto& x : vd) {
             size_t pos;
             if (x[1] == GLOBAL_INTERMEDIATE || x[1] == EXPLICIT_INTERMEDIATE) {

@@ -29,7 +29,7 @@ struct ProgramKernel {
     size_t outposOffset = 0;  // first entry in Program::outpos (user outputs only)
     size_t interBase = 0;     // first intermediate slot (offset after the inputs) of this kernel
     size_t numVarSlots = 0, numConstSlots = 0, numOutSlots = 0;
-    std::vector<Kernel::VariableAccessData> access;
+    std::vector<ColumnRef> access;  // per input leaf of the template: stored column and offset
     int group = -1;
 };
 

@@ -2,328 +2,329 @@
 
 #include <algorithm>
 #include <cmath>
-#include <map>
-#include <set>
+#include <unordered_map>
+#include <vector>
 
 #include "Traverse.hpp"
-#include "Utilities.hpp"
 
 namespace Sym {
 
 namespace {
 
-bool isMul(const Symbolic& x) { return x.op() == MUL || x.op() == MULC; }
-
-Symbolic makePower(const Symbolic& x, const double d) {
-    if (x.op() == CONST) return Symbolic(std::pow(x.constant(), d));
-    if (x.op() == POW && x[1].op() == CONST) return Symbolic(POW, x[0], Symbolic(d * x[1].constant()));
-    return Symbolic(POW, x, Symbolic(d));
-}
-
-Symbolic pullConstant(const Symbolic& x_) {
-    Symbolic x = flattenMultiplications(x_);
-    if (x.op() == MULC) return x;
-    if (x.op() != MUL) return Symbolic(MULC, Symbolic(1.), x);
-    double f = 1.;
-    std::vector<Symbolic> childs;
-    for (auto& c : x) {
-        if (c.op() == CONST) f *= c.constant();
-        else childs.push_back(c);
+bool foldable(OpType op) {
+    switch (op) {
+        case ADD:
+        case MUL:
+        case MULC:
+        case DIV:
+        case SQRT:
+        case NEG:
+        case RECIPROCAL:
+        case POW:
+            return true;
+        default:
+            return false;
     }
-    return Symbolic(MULC, Symbolic(f), makeSymbolic(MUL, childs));
 }
 
-Symbolic simplifyExpression(const Symbolic& expr) {
-    if (isSmallConstant(expr.ahash())) return Symbolic((double)expr.ahash());
-    if (expr.numChilds() == 0) return expr;
-    Symbolic x = optimizeSum(expr);
-    x = pullFactor(x);
-    x = optimizeProducts(x);
-    return x;
+bool statement(OpType op) { return op == ASSIGN || op == DEFINE || op == BLOCK; }
+
+// One node from its (already folded) operands: constants combined, all-constant operations
+// evaluated, the small-integer hash rule applied.
+Symbolic fold(const Symbolic& y, const std::vector<Symbolic>& ops) {
+    const OpType op = y.op();
+    Symbolic out;
+    if (op == ADD || op == MUL || op == MULC) {
+        const bool sum = op == ADD;
+        double c = sum ? 0.0 : 1.0;
+        std::vector<Symbolic> rest;
+        for (const Symbolic& o : ops) {
+            if (o.op() == CONST) c = sum ? c + o.constant() : c * o.constant();
+            else rest.push_back(o);
+        }
+        if (!sum && c == 0.0) return Symbolic(0.0);
+        if (c != (sum ? 0.0 : 1.0) || rest.empty()) rest.push_back(Symbolic(c));
+        out = rest.size() == 1 ? rest.front() : Symbolic(sum ? ADD : MUL, rest);
+    } else if (foldable(op) && std::all_of(ops.begin(), ops.end(), [](const Symbolic& o) { return o.op() == CONST; })) {
+        std::vector<double> v;
+        for (const Symbolic& o : ops) v.push_back(o.constant());
+        out = Symbolic(evaluate(op, v));
+    } else {
+        out = op == INSTANCE ? Symbolic::instance(y.tag(), ops, 0) : Symbolic(op, ops);
+    }
+    if (!statement(out.op()) && out.op() != CONST && isSmallConstant(out.ahash())) return Symbolic((double)out.ahash());
+    return out;
+}
+
+// ---- products
+
+struct Power {
+    Symbolic base;
+    double exponent;
+};
+
+// Add the powers of the factors of x, raised to `e`, into `pw` (by algebraic hash of the base,
+// in order of first appearance); constants multiply `coef`.
+void collectPowers(const Symbolic& x, double e, std::vector<Power>& pw, std::unordered_map<hash_t, size_t>& at,
+                   double& coef) {
+    switch (x.op()) {
+        case CONST:
+            coef *= std::pow(x.constant(), e);
+            return;
+        case MUL:
+        case MULC:
+            for (const Symbolic& c : x) collectPowers(c, e, pw, at, coef);
+            return;
+        case DIV:
+            collectPowers(x[0], e, pw, at, coef);
+            collectPowers(x[1], -e, pw, at, coef);
+            return;
+        case SQRT:
+            collectPowers(x[0], 0.5 * e, pw, at, coef);
+            return;
+        case POW:
+            if (x[1].op() == CONST) {
+                collectPowers(x[0], e * x[1].constant(), pw, at, coef);
+                return;
+            }
+            break;
+        default:
+            break;
+    }
+    auto it = at.find(x.ahash());
+    if (it == at.end()) {
+        at.emplace(x.ahash(), pw.size());
+        pw.push_back({x, e});
+    } else {
+        pw[it->second].exponent += e;
+    }
+}
+
+Symbolic product(std::vector<Symbolic> f) {
+    if (f.empty()) return Symbolic(1.0);
+    if (f.size() == 1) return f.front();
+    return Symbolic(MUL, f);
+}
+
+Symbolic normalizeProduct(const Symbolic& x) {
+    std::vector<Power> pw;
+    std::unordered_map<hash_t, size_t> at;
+    double coef = 1.0;
+    collectPowers(x, 1.0, pw, at, coef);
+    if (coef == 0.0) return Symbolic(0.0);
+    std::vector<Symbolic> num, den, numHalf, denHalf, other;
+    for (const Power& p : pw) {
+        const double e = std::fabs(p.exponent);
+        if (e == 0.0) continue;
+        const double whole = std::floor(e);
+        const bool half = e - whole == 0.5;
+        if (e != whole && !half) {
+            other.emplace_back(POW, p.base, Symbolic(p.exponent));
+            continue;
+        }
+        auto& full = p.exponent > 0 ? num : den;
+        for (int k = 0; k < (int)whole; ++k) full.push_back(p.base);
+        if (half) (p.exponent > 0 ? numHalf : denHalf).push_back(p.base);
+    }
+    num.insert(num.end(), other.begin(), other.end());
+    if (!numHalf.empty()) {
+        const Symbolic r = denHalf.empty() ? product(numHalf) : Symbolic(DIV, product(numHalf), product(denHalf));
+        num.emplace_back(SQRT, r);
+    } else if (!denHalf.empty()) {
+        den.emplace_back(SQRT, product(denHalf));
+    }
+    if (coef != 1.0) num.push_back(Symbolic(coef));
+    Symbolic n = product(num);
+    if (den.empty()) return n;
+    return Symbolic(DIV, n, product(den));
+}
+
+// ---- sums
+
+struct Term {
+    double coef;
+    Symbolic t;
+};
+
+void addTerm(const Symbolic& x, double m, std::vector<Term>& terms, std::unordered_map<hash_t, size_t>& at) {
+    auto it = at.find(x.ahash());
+    if (it == at.end()) {
+        at.emplace(x.ahash(), terms.size());
+        terms.push_back({m, x});
+    } else {
+        terms[it->second].coef += m;
+    }
+}
+
+// Collect the terms of x times `m` (nested sums, negations, constant multiples and the
+// constant factors of products), merging algebraically equal terms.  A constant multiple of a
+// sum stays one term (it is not distributed).
+void collectTerms(const Symbolic& x, double m, std::vector<Term>& terms, std::unordered_map<hash_t, size_t>& at,
+                  double& constant) {
+    switch (x.op()) {
+        case CONST:
+            constant += m * x.constant();
+            return;
+        case ADD:
+            for (const Symbolic& c : x) collectTerms(c, m, terms, at, constant);
+            return;
+        case NEG:
+            collectTerms(x[0], -m, terms, at, constant);
+            return;
+        case MULC:
+        case MUL: {
+            double k = 1.0;
+            std::vector<Symbolic> rest;
+            for (const Symbolic& c : x) {
+                if (c.op() == CONST) k *= c.constant();
+                else rest.push_back(c);
+            }
+            if (rest.empty()) {
+                constant += m * k;
+                return;
+            }
+            if (k != 1.0) {
+                const Symbolic r = product(rest);
+                if (r.op() == ADD) addTerm(r, m * k, terms, at);
+                else collectTerms(r, m * k, terms, at, constant);
+                return;
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    addTerm(x, m, terms, at);
+}
+
+std::vector<Symbolic> factorsOf(const Symbolic& t) {
+    if (t.op() == MUL) return std::vector<Symbolic>(t.begin(), t.end());
+    return {t};
+}
+
+Symbolic scaled(double c, const Symbolic& t) {
+    if (c == 1.0) return t;
+    return Symbolic(MULC, Symbolic(c), t);
+}
+
+Symbolic buildSum(std::vector<Term> terms, double constant);
+
+// Factor out, while one is shared by two or more terms, the factor shared by the most terms
+// (then the most complex); the rest of the terms, and the constant, are added after.
+Symbolic factorSum(const std::vector<Term>& terms, double constant) {
+    std::vector<std::vector<Symbolic>> fs;
+    std::unordered_map<hash_t, std::vector<size_t>> holders;
+    std::vector<hash_t> order;
+    for (size_t i = 0; i < terms.size(); ++i) {
+        fs.push_back(factorsOf(terms[i].t));
+        std::vector<hash_t> seenHere;
+        for (const Symbolic& f : fs.back()) {
+            const hash_t h = f.ahash();
+            if (std::find(seenHere.begin(), seenHere.end(), h) != seenHere.end()) continue;
+            seenHere.push_back(h);
+            auto& hs = holders[h];
+            if (hs.empty()) order.push_back(h);
+            hs.push_back(i);
+        }
+    }
+    hash_t best = 0;
+    size_t bestCount = 1;
+    int bestComplexity = -1;
+    Symbolic bestFactor;
+    for (hash_t h : order) {
+        const auto& hs = holders[h];
+        if (hs.size() < 2) continue;
+        const Symbolic& f = *std::find_if(fs[hs.front()].begin(), fs[hs.front()].end(),
+                                          [&](const Symbolic& s) { return s.ahash() == h; });
+        if (hs.size() > bestCount || (hs.size() == bestCount && f.complexity() > bestComplexity)) {
+            best = h;
+            bestCount = hs.size();
+            bestComplexity = f.complexity();
+            bestFactor = f;
+        }
+    }
+    if (bestCount < 2) return Symbolic();  // nothing to factor
+    std::vector<Term> inner, outer;
+    const auto& hs = holders[best];
+    size_t k = 0;
+    for (size_t i = 0; i < terms.size(); ++i) {
+        if (k < hs.size() && hs[k] == i) {
+            ++k;
+            std::vector<Symbolic> rest = fs[i];
+            rest.erase(std::find_if(rest.begin(), rest.end(), [&](const Symbolic& s) { return s.ahash() == best; }));
+            inner.push_back({terms[i].coef, product(rest)});
+        } else {
+            outer.push_back(terms[i]);
+        }
+    }
+    const Symbolic factored = Symbolic(MUL, bestFactor, buildSum(inner, 0.0));
+    outer.push_back({1.0, factored});
+    return buildSum(outer, constant);
+}
+
+Symbolic buildSum(std::vector<Term> terms, double constant) {
+    for (const Term& t : terms)
+        if (t.t.op() == CONST) constant += t.coef * t.t.constant();
+    terms.erase(std::remove_if(terms.begin(), terms.end(), [](const Term& t) { return t.coef == 0.0 || t.t.op() == CONST; }),
+                terms.end());
+    if (terms.size() >= 2) {
+        const Symbolic f = factorSum(terms, constant);
+        if (f.op() != NOOP) return f;
+    }
+    std::vector<Symbolic> pos, neg;
+    for (const Term& t : terms) {
+        if (t.coef < 0) neg.push_back(scaled(-t.coef, t.t));
+        else pos.push_back(scaled(t.coef, t.t));
+    }
+    if (constant > 0) pos.push_back(Symbolic(constant));
+    else if (constant < 0) neg.push_back(Symbolic(-constant));
+    if (!neg.empty()) pos.emplace_back(NEG, neg.size() == 1 ? neg.front() : Symbolic(ADD, neg));
+    if (pos.empty()) return Symbolic(0.0);
+    return pos.size() == 1 ? pos.front() : Symbolic(ADD, pos);
+}
+
+Symbolic normalizeSum(const Symbolic& x) {
+    std::vector<Term> terms;
+    std::unordered_map<hash_t, size_t> at;
+    double constant = 0.0;
+    collectTerms(x, 1.0, terms, at, constant);
+    return buildSum(std::move(terms), constant);
+}
+
+Symbolic normalize(const Symbolic& y) {
+    switch (y.op()) {
+        case ADD:
+        case NEG:
+            return normalizeSum(y);
+        case MUL:
+        case MULC:
+        case DIV:
+        case SQRT:
+            return normalizeProduct(y);
+        default:
+            return y;
+    }
 }
 
 }  // namespace
 
-Symbolic makeMul(const std::vector<Symbolic>& childs) {
-    if (childs.empty()) return Symbolic(1.);
-    std::vector<Symbolic> childs2;
-    double f = 1.;
-    for (auto& c : childs) {
-        if (c.op() == CONST) f *= c.constant();
-        else if (isSmallConstant(c.ahash())) f *= (double)c.ahash();
-        else childs2.push_back(c);
-    }
-    if (f == 0.0) return Symbolic(0.0);
-    if (f != 1.) childs2.push_back(Symbolic(f));
-    if (childs2.empty()) return Symbolic(1.);
-    if (childs2.size() == 1) return childs2.front();
-    return Symbolic(MUL, childs2);
-}
-
-Symbolic makeAddition(std::vector<Symbolic> operands) {
-    double constant = 0.0;
-    operands.erase(std::remove_if(operands.begin(), operands.end(),
-                                  [&](const Symbolic& x) {
-                                      if (x.op() == CONST) {
-                                          constant += x.constant();
-                                          return true;
-                                      }
-                                      return false;
-                                  }),
-                   operands.end());
-    if (constant != 0) operands.push_back(Symbolic(constant));
-    if (operands.empty()) return Symbolic(0.0);
-    if (operands.size() == 1) return operands.front();
-    return Symbolic(ADD, operands);
-}
-
-Symbolic makeSymbolic(const unsigned char op, const std::vector<Symbolic>& operands) {
-    Symbolic ret;
-    switch (op) {
-        case ADD:
-            ret = makeAddition(operands);
-            break;
-        case MUL:
-        case MULC:
-            ret = makeMul(operands);
-            break;
-        default:
-            ret = Symbolic(op, operands);
-    }
-    if (op != ASSIGN && op != DEFINE && op != BLOCK && isSmallConstant(ret.ahash())) return Symbolic((double)ret.ahash());
-    return ret;
-}
-
-Symbolic flattenAdditions(const Symbolic& expr) {
-    if (expr.op() != ADD) return expr;
-    std::vector<Symbolic> operands;
-    preOrderTraverse(expr, [&](const Symbolic& x) {
-        if (!x.ahash()) return false;
-        if (x.op() != ADD) {
-            operands.push_back(x);
-            return false;
-        }
-        return true;
-    });
-    Symbolic ret(ADD, operands);
-    if (ret.ahash() == 0) ret = Symbolic(0.0);
-    else if (ret.numChilds() == 1) ret = Symbolic(ret[0]);
-    return ret;
-}
-
-Symbolic flattenMultiplications(const Symbolic& expr) {
-    if (!isMul(expr)) return expr;
-    std::vector<Symbolic> operands;
-    preOrderTraverse(expr, [&](const Symbolic& x) {
-        if (isSmallConstant(x.ahash())) {
-            if (x.ahash() != 1) operands.push_back(Symbolic((double)x.ahash()));
-            return false;
-        }
-        if (!isMul(x)) {
-            operands.push_back(x);
-            return false;
-        }
-        return true;
-    });
-    return makeSymbolic(MUL, operands);
-}
-
-Symbolic flattenDivAndMul(const Symbolic& expr) {
-    std::vector<Symbolic> childs;
-    switch (expr.op()) {
-        case MUL:
-        case MULC:
-            for (auto& c0 : expr) {
-                Symbolic c = flattenDivAndMul(c0);
-                if (isMul(c))
-                    for (auto& cc : c) childs.push_back(cc);
-                else childs.push_back(c);
-            }
-            break;
-        case SQRT: {
-            Symbolic rad = flattenDivAndMul(expr[0]);
-            if (isMul(rad))
-                for (auto& c : rad) childs.push_back(makePower(c, 0.5));
-            else childs.push_back(makePower(rad, 0.5));
-            break;
-        }
-        case DIV: {
-            Symbolic nom = flattenDivAndMul(expr[0]);
-            Symbolic den = flattenDivAndMul(expr[1]);
-            if (isMul(nom))
-                for (auto& c : nom) childs.push_back(c);
-            else childs.push_back(nom);
-            if (isMul(den))
-                for (auto& c : den) childs.push_back(makePower(c, -1.));
-            else childs.push_back(makePower(den, -1.));
-            break;
-        }
-        default:
-            return expr;
-    }
-    return makeMul(childs);
-}
-
-Symbolic optimizeProducts(const Symbolic& x) {
-    Symbolic xf = flattenDivAndMul(x);
-    if (!isMul(xf)) return x;
-
-    std::map<Symbolic, double, AlgebraicHashFunctor> factors;
-    for (auto& c : xf) {
-        if (c.op() == POW) {
-            if (c[1].op() != CONST) return x;
-            factors[c[0]] += c[1].constant();
-        } else factors[c] += 1.;
-    }
-
-    std::vector<Symbolic> nom, den, nomR, denR;
-    double f = 1.;
-    for (auto& c : factors) {
-        if (c.first.op() == CONST) {
-            f *= std::pow(c.first.constant(), c.second);
-        } else if (c.second < 0) {
-            const double e = -c.second;
-            if (std::floor(e) == e) {
-                for (int i = 0; i < (int)e; ++i) den.push_back(c.first);
-            } else if (std::floor(2 * e) == 2 * e) {
-                for (int i = 0; i < (int)std::floor(e); ++i) den.push_back(c.first);
-                denR.push_back(c.first);
-            } else return x;
-        } else if (c.second > 0) {
-            const double e = c.second;
-            if (std::floor(e) == e) {
-                for (int i = 0; i < (int)e; ++i) nom.push_back(c.first);
-            } else if (std::floor(2 * e) == 2 * e) {
-                for (int i = 0; i < (int)std::floor(e); ++i) nom.push_back(c.first);
-                nomR.push_back(c.first);
-            } else return x;
-        }
-    }
-
-    if (!nomR.empty()) {
-        if (denR.empty()) nom.emplace_back(SQRT, makeMul(nomR));
-        else nom.emplace_back(SQRT, makeMul(nomR) / makeMul(denR));
-    } else if (!denR.empty()) {
-        den.emplace_back(SQRT, makeMul(denR));
-    }
-
-    if (f != 1.) nom.push_back(Symbolic(f));
-    if (nom.empty()) nom.push_back(Symbolic(1.));
-    Symbolic ret = makeMul(nom);
-    if (!ret.ahash()) return Symbolic(0.0);
-    if (!den.empty()) ret = ret / makeMul(den);
-    return ret;
-}
-
-Symbolic optimizeSum(const Symbolic& x0) {
-    Symbolic x = flattenAdditions(x0);
-    if (x.op() != ADD) return x;
-
-    std::map<Symbolic, double, AlgebraicHashFunctor> multiples;
-    for (auto& c0 : x) {
-        Symbolic c = pullConstant(c0);
-        multiples[c[1]] += c[0].constant();
-    }
-    std::vector<Symbolic> childs;
-    for (auto& c : multiples) {
-        if (c.second != 0.0) {
-            if (c.second == 1.) childs.push_back(c.first);
-            else childs.push_back(flattenMultiplications(Symbolic(c.second) * c.first));
-        }
-    }
-    return makeSymbolic(ADD, childs);
-}
-
-Symbolic pullFactor(Symbolic x) {
-    if (x.op() != ADD) return x;
-    if (x.numChilds() == 1) return x[0];
-    if (x.numChilds() == 0) return Symbolic(0.0);
-    if (isSmallConstant(x.algebraicHash())) return Symbolic((double)x.algebraicHash());
-
-    std::vector<std::vector<Symbolic>> summands;
-    summands.reserve(x.numChilds());
-    for (const auto& c : x) {
-        if (isMul(c)) {
-            Symbolic fc = flattenMultiplications(c);
-            if (fc.op() == MUL) summands.emplace_back(fc.begin(), fc.end());
-            else summands.push_back({fc});
-        } else summands.push_back({c});
-    }
-
-    // the factor shared by the most summands (weighted by its complexity)
-    std::map<Symbolic, std::set<int>, AlgebraicHashFunctor> factors;
-    for (int i = 0; i < (int)summands.size(); ++i)
-        for (auto& y : summands[i]) factors[y].insert(i);
-
-    auto maxIt = std::max_element(factors.begin(), factors.end(), [](const auto& a, const auto& b) {
-        const int scoreA = std::max((int)a.second.size() - 2, 0) * (a.first.complexity() + 1);
-        const int scoreB = std::max((int)b.second.size() - 2, 0) * (b.first.complexity() + 1);
-        return scoreA < scoreB;
-    });
-    if (maxIt->second.size() < 2) return x;
-
-    auto it = maxIt->second.begin();
-    std::vector<Symbolic> factored, rest;
-    for (int i = 0; i < (int)summands.size(); ++i) {
-        if (it != maxIt->second.end() && i == *it) {
-            auto its = std::find(summands[i].begin(), summands[i].end(), maxIt->first);
-            assert(its != summands[i].end());
-            summands[i].erase(its);
-            if (summands[i].empty()) factored.push_back(Symbolic(1.));
-            else factored.emplace_back(MUL, summands[i]);
-            ++it;
-        } else {
-            rest.emplace_back(MUL, summands[i]);
-        }
-    }
-    rest.push_back(maxIt->first * pullFactor(makeSymbolic(ADD, factored)));
-    return pullFactor(flattenAdditions(makeSymbolic(ADD, rest)));
-}
-
-Symbolic findNegative(const Symbolic& expr) {
-    if (expr.op() == MULC && expr[0].op() == CONST && expr[0].constant() == -1) return Symbolic(NEG, expr[1]);
-    if (expr.op() == MUL) {
-        std::vector<Symbolic> childs;
-        int cnt = 0;
-        for (auto& c : expr) {
-            if (c.op() == CONST && c.constant() == -1.) ++cnt;
-            else childs.push_back(c);
-        }
-        if (cnt % 2) return Symbolic(NEG, makeMul(childs));
-        return makeMul(childs);
-    } else if (expr.op() == ADD) {
-        std::vector<Symbolic> positive, negative;
-        for (auto& c : expr) {
-            if (c.op() == NEG) negative.push_back(c[0]);
-            else if (c.op() == MULC && c[0].op() == CONST && c[0].constant() == -1.) negative.push_back(c[1]);
-            else positive.push_back(c);
-        }
-        if (!negative.empty()) positive.emplace_back(NEG, makeAddition(negative));
-        return makeAddition(positive);
-    }
-    return expr;
-}
-
-Symbolic simplify(const Symbolic& expr) {
-    return traverseGenerate<Symbolic>(
-        expr, [](const Symbolic& x) { return simplifyExpression(x); },
-        [](const Symbolic& x, std::vector<Symbolic>& childs) {
-            if (x.numChilds() == 0) return x;
-            return findNegative(simplifyExpression(makeSymbolic(x.op(), childs)));
-        },
-        true);
-}
-
-Symbolic removeConstantExpressions(const Symbolic& x) {
+Symbolic foldConstants(const Symbolic& x) {
     return traverseGenerate<Symbolic>(
         x, [](const Symbolic& y) { return y; },
-        [](const Symbolic& y, std::vector<Symbolic>& childs) {
-            if (y.numChilds() == 0) return y;
-            return makeSymbolic(y.op(), childs);
+        [](const Symbolic& y, std::vector<Symbolic>& ops) { return y.numChilds() ? fold(y, ops) : y; }, true);
+}
+
+Symbolic simplify(const Symbolic& x) {
+    return traverseGenerate<Symbolic>(
+        x, [](const Symbolic& y) { return y; },
+        [](const Symbolic& y, std::vector<Symbolic>& ops) {
+            if (!y.numChilds()) return y;
+            const Symbolic f = fold(y, ops);
+            if (f.op() == CONST || statement(f.op())) return f;
+            const Symbolic n = normalize(f);
+            return isSmallConstant(n.ahash()) && n.op() != CONST ? Symbolic((double)n.ahash()) : n;
         },
         true);
 }
-
-Symbolic referenceRedundant(const Symbolic& x) { return removeConstantExpressions(x); }
 
 }  // namespace Sym

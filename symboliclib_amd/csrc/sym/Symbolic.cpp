@@ -267,7 +267,6 @@ hash_t Symbolic::computeStructureHash() const {
     });
 }
 
-ExpressionBlock::ExpressionBlock(const Symbolic& x_) : x(x_), structureHash(x_.computeStructureHash()), level(0) {}
 
 /* evaluation ****************************************************************/
 

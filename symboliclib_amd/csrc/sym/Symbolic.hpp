@@ -142,42 +142,21 @@ struct AlgebraicHashFunctor {
     inline bool operator()(const Symbolic& s0, const Symbolic& s1) const { return s0.ahash() < s1.ahash(); }
 };
 
-// Hash-consing factory used by the cached differentiation (src/Symbolic.hpp:175-191):
-// expressions with |hash| <= 32 collapse to that constant, all others are shared by hash.
-class CachedFactory {
-    std::unordered_map<hash_t, Symbolic> cache;
+// Interning pool of the cached differentiation: an expression algebraically equal to one
+// made through the pool before comes back as that earlier node, so equal derivative terms
+// share one node, and one whose algebraic hash is a small integer k (the hash of the
+// constant k) comes back as the constant k.
+class InternPool {
+    std::unordered_map<hash_t, Symbolic> nodes_;
 
 public:
     template <class... Args>
-    Symbolic operator()(const Args&... args) {
-        Symbolic x(args...);
-        const hash_t h = x.ahash();
-        if (h >= -32 && h <= 32) return Symbolic((double)h);
-        auto& y = cache[h];
-        if (y.op() == NOOP) y = x;
-        return y;
+    Symbolic make(const Args&... args) {
+        const Symbolic built(args...);
+        const hash_t h = built.ahash();
+        if (isSmallConstant(h)) return Symbolic((double)h);
+        return nodes_.emplace(h, built).first->second;
     }
-};
-
-class Factory {
-public:
-    template <class... Args>
-    Symbolic operator()(const Args&... args) {
-        return Symbolic(args...);
-    }
-};
-
-// One unit of work after decomposition: an assignment (or block) plus the indices of the
-// blocks it reads (src/Symbolic.hpp:198-212).
-class ExpressionBlock {
-public:
-    Symbolic x;
-    hash_t structureHash = 0;
-    int level = 0;
-    std::vector<int> childs;
-
-    ExpressionBlock() {}
-    ExpressionBlock(const Symbolic& x_);
 };
 
 }  // namespace Sym

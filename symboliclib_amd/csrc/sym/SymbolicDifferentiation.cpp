@@ -63,7 +63,7 @@ inline bool nz(const Symbolic& s) { return s.ahash() != 0; }
 
 // d/dvars of node y given the derivatives of its children (forward mode, shared factory).
 std::vector<Symbolic> forwardRule(const Symbolic& y, std::vector<std::vector<Symbolic>>& cg,
-                                  const std::unordered_map<hash_t, size_t>& varId, size_t nv, CachedFactory& cf,
+                                  const std::unordered_map<hash_t, size_t>& varId, size_t nv, InternPool& pool,
                                   const Symbolic& zero) {
     std::vector<Symbolic> ret(nv, zero);
     switch (y.op()) {
@@ -73,14 +73,14 @@ std::vector<Symbolic> forwardRule(const Symbolic& y, std::vector<std::vector<Sym
                 for (auto& g : cg)
                     if (nz(g[j])) terms.push_back(g[j]);
                 if (terms.size() == 1) ret[j] = terms[0];
-                else if (terms.size() == 2) ret[j] = cf(ADD, terms[0], terms[1]);
-                else if (terms.size() > 2) ret[j] = cf(ADD, terms);
+                else if (terms.size() == 2) ret[j] = pool.make(ADD, terms[0], terms[1]);
+                else if (terms.size() > 2) ret[j] = pool.make(ADD, terms);
             }
             break;
         case MULC:
             if (y[0].op() == CONST) {
                 for (size_t j = 0; j < nv; ++j)
-                    if (nz(cg[1][j])) ret[j] = cf(MULC, y[0], cg[1][j]);
+                    if (nz(cg[1][j])) ret[j] = pool.make(MULC, y[0], cg[1][j]);
                 break;
             }
             // fallthrough: general product rule
@@ -91,18 +91,18 @@ std::vector<Symbolic> forwardRule(const Symbolic& y, std::vector<std::vector<Sym
                 for (unsigned i = 0; i < n; ++i) {
                     if (!nz(cg[i][j])) continue;
                     if (n == 2) {
-                        terms.push_back(cf(MUL, y[1 - i], cg[i][j]));
+                        terms.push_back(pool.make(MUL, y[1 - i], cg[i][j]));
                     } else {
                         std::vector<Symbolic> f;
                         for (unsigned k = 0; k < n; ++k)
                             if (k != i) f.push_back(y[k]);
                         f.push_back(cg[i][j]);
-                        terms.push_back(cf(MUL, f));
+                        terms.push_back(pool.make(MUL, f));
                     }
                 }
                 if (terms.size() == 1) ret[j] = terms[0];
-                else if (terms.size() == 2) ret[j] = cf(ADD, terms[0], terms[1]);
-                else if (terms.size() > 2) ret[j] = cf(ADD, terms);
+                else if (terms.size() == 2) ret[j] = pool.make(ADD, terms[0], terms[1]);
+                else if (terms.size() > 2) ret[j] = pool.make(ADD, terms);
             }
             break;
         }
@@ -110,24 +110,24 @@ std::vector<Symbolic> forwardRule(const Symbolic& y, std::vector<std::vector<Sym
             for (size_t j = 0; j < nv; ++j) {
                 const bool a = nz(cg[0][j]), b = nz(cg[1][j]);
                 Symbolic t0, t1;
-                if (a) t0 = cf(DIV, cg[0][j], y[1]);
-                if (b) t1 = cf(MULC, Symbolic(-1.), cf(DIV, cf(MUL, y[0], cg[1][j]), cf(MUL, y[1], y[1])));
-                if (a && b) ret[j] = cf(ADD, t0, t1);
+                if (a) t0 = pool.make(DIV, cg[0][j], y[1]);
+                if (b) t1 = pool.make(MULC, Symbolic(-1.), pool.make(DIV, pool.make(MUL, y[0], cg[1][j]), pool.make(MUL, y[1], y[1])));
+                if (a && b) ret[j] = pool.make(ADD, t0, t1);
                 else if (a) ret[j] = t0;
                 else if (b) ret[j] = t1;
             }
             break;
         case SQRT:
             for (size_t j = 0; j < nv; ++j)
-                if (nz(cg[0][j])) ret[j] = cf(DIV, cg[0][j], cf(MULC, Symbolic(2.), y));
+                if (nz(cg[0][j])) ret[j] = pool.make(DIV, cg[0][j], pool.make(MULC, Symbolic(2.), y));
             break;
         case NEG:
             for (size_t j = 0; j < nv; ++j)
-                if (nz(cg[0][j])) ret[j] = cf(MULC, Symbolic(-1.), cg[0][j]);
+                if (nz(cg[0][j])) ret[j] = pool.make(MULC, Symbolic(-1.), cg[0][j]);
             break;
         case RECIPROCAL:
             for (size_t j = 0; j < nv; ++j)
-                if (nz(cg[0][j])) ret[j] = cf(MULC, Symbolic(-1.), cf(DIV, cg[0][j], cf(MUL, y[0], y[0])));
+                if (nz(cg[0][j])) ret[j] = pool.make(MULC, Symbolic(-1.), pool.make(DIV, cg[0][j], pool.make(MUL, y[0], y[0])));
             break;
         case VAR: {
             auto it = varId.find(y.ahash());
@@ -169,10 +169,10 @@ std::vector<std::vector<Symbolic>> hessianCached(const Symbolic& x, const std::v
     const size_t nv = vars.size();
     std::unordered_map<hash_t, size_t> varId;
     for (size_t i = 0; i < nv; ++i) varId[vars[i].ahash()] = i;
-    CachedFactory cf;
+    InternPool pool;
     const Symbolic zero(0.0);
     auto diff = [&](const Symbolic& y, std::vector<std::vector<Symbolic>>& cg) {
-        return forwardRule(y, cg, varId, nv, cf, zero);
+        return forwardRule(y, cg, varId, nv, pool, zero);
     };
     std::unordered_map<long long, std::vector<Symbolic>, IdentityHash<long long>> cache;
     auto grad = traverseGenerateMap<std::vector<Symbolic>>(x, diff, cache, true);
@@ -185,11 +185,11 @@ std::vector<Symbolic> gradientCached(const Symbolic& x, const std::vector<Symbol
     const size_t nv = vars.size();
     std::unordered_map<hash_t, size_t> varId;
     for (size_t i = 0; i < nv; ++i) varId[vars[i].ahash()] = i;
-    CachedFactory cf;
+    InternPool pool;
     const Symbolic zero(0.0);
     std::unordered_map<long long, std::vector<Symbolic>, IdentityHash<long long>> cache;
     return traverseGenerateMap<std::vector<Symbolic>>(
-        x, [&](const Symbolic& y, std::vector<std::vector<Symbolic>>& cg) { return forwardRule(y, cg, varId, nv, cf, zero); },
+        x, [&](const Symbolic& y, std::vector<std::vector<Symbolic>>& cg) { return forwardRule(y, cg, varId, nv, pool, zero); },
         cache, true);
 }
 
@@ -330,6 +330,22 @@ const ExprTemplate& hessianTemplateFor(const ElementShape& s) {
     return t ? *t : buildHessianTemplate(s, sh);
 }
 
+void sealBlock(std::vector<std::vector<Symbolic>>& entries) {
+    std::unordered_map<hash_t, size_t> slotOf;  // algebraic hash -> index into assigns
+    std::vector<Symbolic> assigns, vars;
+    for (const auto& row : entries)
+        for (const auto& e : row) {
+            if (slotOf.count(e.ahash())) continue;
+            slotOf.emplace(e.ahash(), assigns.size());
+            vars.push_back(Symbolic::generateUniqueVariable());
+            assigns.emplace_back(ASSIGN, vars.back(), e);
+        }
+    if (assigns.empty()) return;
+    const Symbolic block(BLOCK, assigns);
+    for (auto& row : entries)
+        for (auto& e : row) e = Symbolic(FIXED, vars[slotOf.at(e.ahash())], block);
+}
+
 std::vector<std::tuple<int, int, Symbolic>> hessianSparseVars(const Symbolic& expr, const std::vector<Symbolic>& vars,
                                                               bool shareTemplates) {
     std::unordered_map<hash_t, size_t> varId;
@@ -349,7 +365,7 @@ std::vector<std::tuple<int, int, Symbolic>> hessianSparseVars(const Symbolic& ex
                 }
             }
             auto h = hessianCached(c, vars2);
-            makeFixed(h);
+            sealBlock(h);
             for (size_t i = 0; i < varIds.size(); ++i)
                 for (size_t j = 0; j < varIds.size(); ++j)
                     if (varIds[i] <= varIds[j]) ret.emplace_back((int)varIds[i], (int)varIds[j], h[i][j]);

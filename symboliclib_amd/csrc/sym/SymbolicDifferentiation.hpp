@@ -4,7 +4,8 @@
 //  * differentiate: reverse mode, returns d expr / d vars;
 //  * hessianCached: forward-over-forward differentiation through a hash-consing factory;
 //  * hessianSparse: for a sum of element terms, the upper-triangle (i <= j) Hessian entries of
-//    every term, each wrapped by makeFixed so that one element becomes one kernel instance.
+//    every term, each term's entries sealed into one FIXED block (sealBlock) so that one element
+//    becomes one kernel instance.
 //    Elements with the same shape share one differentiated template (Template.hpp).
 #pragma once
 
@@ -14,10 +15,29 @@
 
 #include "ContainerSupport.hpp"
 #include "Symbolic.hpp"
-#include "SymbolicUtilities.hpp"
 #include "Traverse.hpp"
 
 namespace Sym {
+
+// Every value of the given containers of Symbolic (anything with contPtr/contSize), in order.
+inline void appendSymbols(std::vector<Symbolic>&) {}
+template <class C, class... Rest>
+void appendSymbols(std::vector<Symbolic>& out, const C& c, const Rest&... rest) {
+    const Symbolic* p = contPtr(c);
+    out.insert(out.end(), p, p + contSize(c));
+    appendSymbols(out, rest...);
+}
+template <class... Cs>
+std::vector<Symbolic> symbolsOf(const Cs&... cs) {
+    std::vector<Symbolic> out;
+    appendSymbols(out, cs...);
+    return out;
+}
+
+// Seal a matrix of expressions into one indivisible unit of work: every algebraically
+// distinct entry is assigned to a fresh variable in one BLOCK, and each entry becomes
+// FIXED(its variable, that block), which decomposition keeps whole.
+void sealBlock(std::vector<std::vector<Symbolic>>& entries);
 
 template <class Cont>
 double evaluate(const Symbolic& x, const Cont& c) {
@@ -36,7 +56,7 @@ std::vector<Symbolic> differentiate(const Symbolic& x, const std::vector<Symboli
 
 template <class... Args>
 std::vector<Symbolic> differentiate(const Symbolic& x, Args&... args) {
-    return differentiate(x, flattenContainerData(args...));
+    return differentiate(x, symbolsOf(args...));
 }
 
 std::vector<std::vector<Symbolic>> hessianCached(const Symbolic& x, const std::vector<Symbolic>& vars);
@@ -49,12 +69,12 @@ std::vector<std::tuple<int, int, Symbolic>> hessianSparseVars(const Symbolic& ex
 
 template <class... Args>
 std::vector<std::tuple<int, int, Symbolic>> hessianSparse(const Symbolic& expr, Args&... args) {
-    return hessianSparseVars(expr, flattenContainerData(args...), true);
+    return hessianSparseVars(expr, symbolsOf(args...), true);
 }
 
 template <class... Args>
 std::vector<std::tuple<int, int, Symbolic>> hessianSparseDirect(const Symbolic& expr, Args&... args) {
-    return hessianSparseVars(expr, flattenContainerData(args...), false);
+    return hessianSparseVars(expr, symbolsOf(args...), false);
 }
 
 }  // namespace Sym

@@ -351,7 +351,7 @@ void validateTiles(const Program& prog, const TiledProgram& tp) {
     std::vector<long> maxOff(tp.colOffset[K], 0), minOff(tp.colOffset[K], 0);
     for (size_t r = 0; r < K; ++r)
         for (const auto& va : prog.kernels[r].access) {
-            const size_t col = tp.colOffset[r] + va.baseVariableId;
+            const size_t col = tp.colOffset[r] + va.column;
             maxOff[col] = std::max<long>(maxOff[col], va.offset);
             minOff[col] = std::min<long>(minOff[col], va.offset);
         }
