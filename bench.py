@@ -3,9 +3,11 @@
 Default workload (c4): 256 Baraff-Witkin cloth Hessians (stretch + shear by symbolic
 differentiation, plus bending) of log-uniform sides 32..256, 345.8 M output nonzeros, batched
 into one program in the fused assembly form (owner-node tiles, element Hessians staged in LDS,
-one persistent work-queue launch replayed from a HIP graph) with the tuned tile shape and
-schedule, against the plain single-instance form launched once per instance (one waves-form
-program per cloth, two launches each) on the same stream.
+one launch replayed from a HIP graph), against the plain single-instance form launched once per
+instance (one waves-form program per cloth, two launches each) on the same stream.  The tile
+shape, block size and schedule are chosen in the run by the learned cost model: every
+configuration of the tuner's space is predicted, the three best predicted are timed, the
+fastest is kept (--committed: the committed selection instead, --untuned: defaults).
 
 Single-instance workloads: c3 (one 256 x 256 cloth, 4,890,648 nonzeros), c2 (cotangent
 Laplacian of a 512 x 512 grid), c1 (A^T A + A of an icosphere); --form selects waves, tiled
@@ -14,8 +16,10 @@ or fused (cloth only).
 One step = one evaluation of the compiled program (every launch) over one value set whose
 inputs are already resident in HBM.  Prints one JSON line with the throughput, the roofline
 of the dominant kernel (algorithmic bytes per launch / its average launch time, against the
-8 TB/s HBM peak) and the CPU baseline: the FP64 oracle (OpenMP over the host cores) timed on
-a bounded sample of the same workload.
+8 TB/s HBM peak and against the measured ceiling of a hand-written STREAM copy) and the CPU
+baseline: the reference's CPU path restated (the CPU device: the same programs as host C++
+with OpenMP, compiled with the reference's flags) on the host cores, over a size-stratified
+sample of the batch (--cpu-full: the whole batch).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c3|c2|c1] [--no-cpu]
 """
@@ -34,6 +38,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TUNED = os.path.join(ROOT, "symboliclib_amd", "tuning", "c4_selected.json")
+MODEL = os.path.join(ROOT, "symboliclib_amd", "tuning", "cost_model.txt")
+FP64_PEAK_TFLOPS = 78.6
 TRAFFIC = os.path.join(ROOT, "profiles", "symlib_fused_bw_hessian_batch256_traffic.json")
 
 
@@ -72,13 +78,27 @@ def time_events(fn, stream, reps):
 
 
 def copy_bandwidth(dev, stream):
+    """The measured HBM ceiling: a hand-written STREAM copy (symhip_stream_copy) of 1 GiB, the
+    best of several grid sizes, HIP events on the launch stream; torch's copy for reference."""
     import torch
 
+    from symboliclib_amd import _native
+
+    L = _native.lib()
     cbuf = torch.empty(1 << 27, dtype=torch.float64, device=dev)
     dbuf = torch.empty_like(cbuf)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    best = 0.0
+    for blocks in (8192, 16384, 32768):
+        run = lambda b=blocks: _native.check(  # noqa: E731
+            L.symhip_stream_copy(ctypes.c_void_p(dbuf.data_ptr()), ctypes.c_void_p(cbuf.data_ptr()), cbuf.numel(), b,
+                                 sp), "symhip_stream_copy")
+        run()
+        ms = time_events(run, stream, 10)
+        best = max(best, 2 * cbuf.numel() * 8 / (ms * 1e-3) / 1e9)
     dbuf.copy_(cbuf)
-    ms = time_events(lambda: dbuf.copy_(cbuf), stream, 10)
-    return 2 * cbuf.numel() * 8 / (ms * 1e-3) / 1e9
+    torch_gbs = 2 * cbuf.numel() * 8 / (time_events(lambda: dbuf.copy_(cbuf), stream, 10) * 1e-3) / 1e9
+    return best, torch_gbs
 
 
 def cpu_baseline(evals, seconds, O):
@@ -187,7 +207,7 @@ def run_single(args, dev, stream, sp):
         cold.append(time_events(step, stream, 1))
     del flush
     cold_ms = float(np.median(cold))
-    copy_gbs = copy_bandwidth(dev, stream)
+    copy_gbs, torch_copy_gbs = copy_bandwidth(dev, stream)
 
     dominant = int(np.argmax(launch_ms))
     names = {"fused": "symlib_fused", "tiled": "symlib_tiled"}
@@ -204,10 +224,12 @@ def run_single(args, dev, stream, sp):
         "config": dict(desc, output_nnz=n_out, form=args.form, tiling=info["tiling"], options=opt, launches=nl,
                        graph=bool(args.graph), parallelism="single GPU"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                     "frac": achieved / HBM_PEAK_GBS, "frac_of_measured_copy": achieved / copy_gbs, "traffic": None,
+                     "kernel": kernel,
                      "algorithmic_bytes": balg, "kernel_ms": sum(launch_ms), "launch_ms": launch_ms,
                      "note": "achieved = algorithmic bytes of the evaluation / summed launch times"},
-        "cold_ms_per_step": cold_ms, "cold_nnz_per_s": n_out / (cold_ms * 1e-3), "copy_bandwidth_gbs": copy_gbs,
+        "cold_ms_per_step": cold_ms, "cold_nnz_per_s": n_out / (cold_ms * 1e-3),
+        "copy_bandwidth_gbs": copy_gbs, "torch_copy_gbs": torch_copy_gbs,
         "parity": {"pattern_bit_exact": pattern_ok, "max_rel_err": float(err.max()),
                    "median_rel_err": float(np.median(err))},
         "build_seconds": {"frontend": info["frontend_seconds"], "build": info["build_seconds"],
@@ -225,6 +247,96 @@ def run_single(args, dev, stream, sp):
 # batched cloths (c4)
 
 
+def select_batched(cloths, inputs, args, committed=None):
+    """Model-driven choice of the fused form's configuration, in this run: build the tiles of
+    every LDS budget of the tuner's space, predict every configuration with the committed cost
+    model, time the best `args.trials` predicted (and the committed selection, when it is not
+    among them) and keep the fastest.  Returns (unit compiled and executed with that
+    configuration, its config, trial records, seconds, configurations)."""
+    from symboliclib_amd import costmodel, tuner
+
+    t0 = time.perf_counter()
+    tun = tuner.cloth_tuner(cloths, log=log)
+    model = costmodel.CostModel.load(MODEL)
+    cfgs, pred, units, _ = tun.predicted(model)
+    model.close()
+    order = [int(i) for i in np.argsort(pred)[: args.trials]]
+    if committed is not None:
+        extra = [i for i, c in enumerate(cfgs) if all(c[k] == committed[k] for k in c)]
+        order += [i for i in extra if i not in order]
+    top = [cfgs[i] for i in order]
+    for lds in {c["lds"] for c in top}:
+        tuner.precompile(units[lds], [c for c in top if c["lds"] == lds])
+    timer = tuner.Timer()
+    trials = []
+    for i in order:
+        c = cfgs[i]
+        u = units[c["lds"]]
+        u.retune(**tuner.unit_options(c))
+        if not u.runtime():
+            u.compile().execute(*inputs)
+        ms = timer.time(u, reps=20)
+        trials.append({"config": c, "predicted_ms": float(np.exp(pred[i])), "ms": ms,
+                       "predicted_rank": int(np.argsort(np.argsort(pred))[i]) + 1})
+        log(f"[bench] trial {c}: predicted {trials[-1]['predicted_ms']:.3f} ms, measured {ms:.4f} ms")
+    best = min(trials, key=lambda t: t["ms"])
+    for lds, u in units.items():
+        if lds != best["config"]["lds"]:
+            u.close()
+    unit = units[best["config"]["lds"]]
+    unit.retune(**tuner.unit_options(best["config"]))
+    (unit if unit.runtime() else unit.compile()).execute(*inputs)
+    return unit, best["config"], trials, time.perf_counter() - t0, len(cfgs)
+
+
+def fp64_flops(stats):
+    """FP64 operations of one launch of the fused cloth form from the emitted code: every
+    element evaluation's adds, multiplies, divisions and square roots, one add per element
+    contribution of an output."""
+    per_elem = stats["adds"] + stats["muls"] + stats["divs"] + stats["sqrts"]
+    return stats["element_evaluations"] * per_elem + stats["contributions"]
+
+
+def cpu_device_baseline(cloths, sides, args):
+    """The reference's CPU path restated (CPU device, symcpu.h) on the host cores: one CPU batch
+    program over a size-stratified sample of the cloths (16 at evenly spaced size ranks, the
+    smallest and the largest included), or over all of them (--cpu-full); execute() (inputs
+    copied, every kernel run) repeated for about args.cpu_seconds after one warm-up."""
+    from symboliclib_amd import workloads as W
+    from symboliclib_amd.unit import Unit
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    order = np.argsort(sides, kind="stable")
+    if args.cpu_full:
+        pick = list(range(len(cloths)))
+    else:
+        ranks = np.unique(np.linspace(0, len(order) - 1, min(16, len(order))).round().astype(int))
+        pick = sorted(int(order[r]) for r in ranks)
+    sub = [cloths[k] for k in pick]
+    t0 = time.perf_counter()
+    u = Unit.bw_hessian_batch([(r, F) for r, F, _ in sub], W.K_STRETCH, W.K_SHEAR, W.K_BEND, cpu_threads=threads)
+    u.compile()
+    build = time.perf_counter() - t0
+    inputs = [W.to_colmajor(X) for _, _, X in sub]
+    u.execute(*inputs)
+    nnz = int(u.info()["num_results"])
+    n, ts = 0, time.perf_counter()
+    while True:
+        u.execute(*inputs)
+        n += 1
+        if time.perf_counter() - ts >= args.cpu_seconds:
+            break
+    tc = time.perf_counter() - ts
+    u.close()
+    sample = (f"{'all' if args.cpu_full else len(pick)} of the {len(cloths)} cloths"
+              f"{'' if args.cpu_full else ' at evenly spaced size ranks'} (sides {min(sides[k] for k in pick)}.."
+              f"{max(sides[k] for k in pick)}, {nnz} output nonzeros) as one program of the CPU device (the "
+              f"reference's VecWidth/NumThreads path restated: host C++ from the same program, OpenMP over "
+              f"{threads} threads, clang with the reference's flags); {n} execute() calls in {tc:.1f} s after a "
+              f"warm-up (build and compile {build:.0f} s, untimed)")
+    return {"value": nnz * n / tc, "unit": "nnz/s", "cores": threads, "kind": "port", "sample": sample}
+
+
 def run_batched(args, dev, stream, sp):
     import torch
 
@@ -234,27 +346,39 @@ def run_batched(args, dev, stream, sp):
 
     import oracle as O
 
-    tuned = None
-    if os.path.exists(TUNED) and not args.untuned:
-        with open(TUNED) as f:
-            tuned = json.load(f)["selected"]
-    opt = fused_options(args, tuned)
     sides = W.c4_sides(count=args.count)
-    log(f"[bench] c4: {len(sides)} cloths, sides {min(sides)}..{max(sides)}, fused options {opt}")
+    log(f"[bench] c4: {len(sides)} cloths, sides {min(sides)}..{max(sides)}")
     t0 = time.perf_counter()
     cloths = W.c4_cloths(count=args.count)
     counts = [W.bw_algorithmic_counts(len(r), F) for r, F, _ in cloths]
-    batched = Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, **opt)
-    t1 = time.perf_counter()
-    batched.compile()
-    t2 = time.perf_counter()
-    info = batched.info()
-    log(f"[bench] batched build {t1 - t0:.1f}s compile {t2 - t1:.1f}s, {info['tiling']}")
     inputs = [W.to_colmajor(X) for _, _, X in cloths]
-    batched.execute(*inputs)
+    committed = None
+    if os.path.exists(TUNED):
+        with open(TUNED) as f:
+            committed = json.load(f)["selected"]
+    selection = None
+    if args.untuned or args.committed or any(v is not None for v in (args.tile_lds, args.groups, args.wpe, args.tpb,
+                                                                     args.schedule, args.split)):
+        opt = fused_options(args, committed if args.committed and not args.untuned else None)
+        batched = Unit.bw_hessian_batch([(r, F) for r, F, _ in cloths], W.K_STRETCH, W.K_SHEAR, W.K_BEND, **opt)
+        batched.compile().execute(*inputs)
+        source = "committed selection" if args.committed else "command line / defaults"
+    else:
+        batched, cfg, trials, secs, ncfg = select_batched(cloths, inputs, args, committed)
+        opt = fused_options(args, {**cfg, "split": 0})
+        selection = {"by": f"cost model {os.path.relpath(MODEL, ROOT)}: {ncfg} configurations predicted, the best "
+                           f"{args.trials} and the committed selection timed in this run, the fastest kept",
+                     "seconds": secs,
+                     "trials": trials, "chosen": cfg, "committed": committed,
+                     "chosen_equals_committed": committed is not None and all(committed[k] == cfg[k] for k in cfg)}
+        source = "selected in this run by the cost model"
+        log(f"[bench] selection: {cfg} in {secs:.0f} s")
     torch.cuda.synchronize()
+    info = batched.info()
+    log(f"[bench] fused batch ({source}) {opt}: {info['tiling']}, set up in {time.perf_counter() - t0:.1f}s")
 
-    # parity on a spread of instances (smallest, largest, four more)
+    # parity on a spread of instances (smallest, largest, four more); every cloth of the batch
+    # is checked by the GPU tests
     order = np.argsort(sides, kind="stable")
     check = sorted({int(order[0]), int(order[-1]), *[int(k) for k in np.linspace(0, len(sides) - 1, 4).astype(int)]})
     worst, med = 0.0, []
@@ -288,6 +412,8 @@ def run_batched(args, dev, stream, sp):
                  for i in range(batched.num_launches())]
     kernel_ms = sum(launch_ms)
     balg = sum(b_alg(c, 0) for c in counts) + 12 * n_out
+    stats = batched.fused_stats()
+    flops = fp64_flops(stats)
 
     # plain single-instance form, one program per distinct side, launched once per instance
     plain_line = None
@@ -332,17 +458,21 @@ def run_batched(args, dev, stream, sp):
             u.close()
         log(f"[bench] plain per-instance: {plain_ms:.3f} ms/step")
 
-    copy_gbs = copy_bandwidth(dev, stream)
+    copy_gbs, torch_copy_gbs = copy_bandwidth(dev, stream)
     achieved = balg / (kernel_ms * 1e-3) / 1e9
     value = n_out / (ms_per_step * 1e-3)
-    # HBM bytes per launch measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed
-    # under profiles/ (used when they were taken on this exact program and configuration)
-    traffic = None
+    # HBM bytes per launch measured by rocprofv3 counter runs on this program's tiles (committed
+    # under profiles/; the bytes are those of the tile pool and outputs, the options it was
+    # measured with are reported beside it), else null
+    traffic, traffic_src = None, None
     if os.path.exists(TRAFFIC):
         with open(TRAFFIC) as f:
-            tr = json.load(f)
-        if tr.get("tiling") == info["tiling"] and tr.get("options") == opt:
-            traffic = tr["fetch_bytes"] + tr["write_bytes"]
+            entries = json.load(f)
+        for tr in entries if isinstance(entries, list) else [entries]:
+            if tr.get("tiling") == info["tiling"]:
+                traffic = tr["read_bytes"] + tr["write_bytes"]
+                traffic_src = {"profile": os.path.relpath(TRAFFIC, ROOT), "options": tr.get("options"),
+                               "read_bytes": tr["read_bytes"], "write_bytes": tr["write_bytes"]}
     line = {
         "metric": "output nnz/s FP64, batched+learned-tuned vs plain HIP; HBM GB/s vs peak",
         "value": value,
@@ -355,48 +485,45 @@ def run_batched(args, dev, stream, sp):
                    "faces": int(sum(len(F) for _, F, _ in cloths)), "output_nnz": n_out,
                    "form": ("fused assembly in two launches (element images, then sums)" if opt.get("split")
                             else "fused assembly, one launch") + ", replayed from a HIP graph",
-                   "options": opt, "tuned_from": os.path.relpath(TUNED, ROOT) if tuned else None,
+                   "options": opt, "configuration_source": source,
                    "tiling": info["tiling"], "grid": batched.launch_grid(0), "parallelism": "single GPU"},
+        "selection": selection,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "+".join(names),
-                     "algorithmic_bytes": balg, "kernel_ms": kernel_ms,
-                     "launch_ms": dict(zip(names, launch_ms))},
+                     "frac": achieved / HBM_PEAK_GBS, "frac_of_measured_copy": achieved / copy_gbs,
+                     "traffic": traffic, "traffic_source": traffic_src, "kernel": "+".join(names),
+                     "algorithmic_bytes": balg, "kernel_ms": kernel_ms, "launch_ms": dict(zip(names, launch_ms))},
+        "fp64": {"flops_per_launch": flops, "achieved_tflops": flops / (kernel_ms * 1e-3) / 1e12,
+                 "peak_tflops": FP64_PEAK_TFLOPS, "frac": flops / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                 "note": "operations of the emitted code: element evaluations x (adds + multiplies + divisions + "
+                         "square roots per element) + one add per element contribution"},
         "plain_per_instance": plain_line,
         "speedup_vs_plain": (value / plain_line["value"]) if plain_line else None,
-        "copy_bandwidth_gbs": copy_gbs,
+        "copy_bandwidth_gbs": copy_gbs, "torch_copy_gbs": torch_copy_gbs,
         "parity": {"pattern_bit_exact": True, "checked_instances": check, "max_rel_err": worst,
                    "median_rel_err": float(np.median(med))},
         "build_seconds": {"frontend": info["frontend_seconds"], "build": info["build_seconds"],
                           "compile": info["compile_seconds"]},
     }
+    batched.close()
     if not args.no_cpu:
-        plans = {}
-        evals = []
-        for k in order[: max(1, len(order) // 2)][:: 8]:
-            rest, F, X = cloths[int(k)]
-            s = sides[int(k)]
-            if s not in plans:
-                plans[s] = O.BWPlan(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
-            evals.append((lambda p=plans[s], X=X: p.values(X, with_scale=False), int(batched.output_shape(int(k))[2])))
-        v, n, tc = cpu_baseline(evals, args.cpu_seconds, O)
-        line["cpu_baseline"] = {"value": v, "unit": "nnz/s", "cores": O.num_threads(), "kind": "port",
-                                "sample": f"{n} cloth evaluations by the FP64 oracle (OpenMP, {O.num_threads()} "
-                                          f"threads) round-robin over {len(evals)} cloths of the batch "
-                                          f"(sides {min(sides[int(k)] for k in order[:len(order) // 2:8])}.."
-                                          f"{max(sides[int(k)] for k in order[:len(order) // 2:8])}) in {tc:.1f} s"}
+        line["cpu_baseline"] = cpu_device_baseline(cloths, sides, args)
+        log(f"[bench] cpu baseline: {line['cpu_baseline']['value']:.3g} nnz/s")
     return line
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--count", type=int, default=256, help="c4: number of cloths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="c4: skip the plain per-instance baseline")
-    ap.add_argument("--untuned", action="store_true", help="c4: ignore the tuned configuration")
+    ap.add_argument("--untuned", action="store_true", help="c4: default options, no selection")
+    ap.add_argument("--committed", action="store_true", help="c4: the committed selection, no selection in the run")
+    ap.add_argument("--trials", type=int, default=3, help="c4: configurations timed in the in-run selection")
+    ap.add_argument("--cpu-full", action="store_true", help="c4: CPU baseline over the whole batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--form", default="waves", choices=["tiled", "waves", "fused"], help="c1-c3 execution form")
     ap.add_argument("--graph", action="store_true", help="c1-c3: replay the launches from a HIP graph")

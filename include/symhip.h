@@ -21,7 +21,8 @@
  *    (src/StringResources.h:174-181) or a silent no-op when linking failed
  *    (src/ComputeUnit.cpp:155-165);
  *  - results are written in the caller's output order by the program itself: there is no
- *    separate gather kernel (src/StringResources.h:218-221).
+ *    separate gather kernel (src/StringResources.h:218-221), unless a program in the
+ *    reference's own kernel form asks for one (symhip_set_result_gather).
  *
  * Threading: one handle must not be used from two threads at once.  Calls on different
  * handles are independent.
@@ -168,6 +169,27 @@ int symhip_set_launch(symhip_t h, uint32_t idx, const symhip_launch* launch);
 int symhip_set_launches(symhip_t h, uint32_t n, const symhip_launch* launches);
 int symhip_set_work_words(symhip_t h, uint64_t n);
 int symhip_read_work(symhip_t h, uint64_t offset, uint64_t n, uint32_t* out);
+/*
+ * Programs in the reference's kernel form.  The reference emits every group kernel as
+ * k<i>(const RealT* x, const RealT* c, RealT* y, const unsigned* p) and launches it as
+ * k<i>(x, c + constantsOffset[i], x + outputOffset[i], p + indexOffset[i]) (the K(num) macro,
+ * src/StringResources.h:196); its results stay in x and are gathered afterwards as
+ * result[j] = x[outIds[j]] (gatherResult, src/StringResources.h:218-221).
+ * symhip_set_launch_offsets switches launch `idx` to that four-argument form with these
+ * offsets (elements of the constant table, of x and of the index table; each at most its
+ * table's length).  symhip_set_launches returns every launch to this runtime's form;
+ * symhip_set_launch keeps the form of the launch it replaces.
+ * symhip_set_result_gather(h, 1) makes symhip_run, symhip_run_graph and symhip_run_noalloc
+ * end with that gather, outpos then holding one position inside x per result (checked here:
+ * as many entries as results, each below num_values); 0 turns it off.
+ */
+int symhip_set_launch_offsets(symhip_t h, uint32_t idx, uint64_t c_offset, uint64_t y_offset, uint64_t p_offset);
+int symhip_set_result_gather(symhip_t h, int on);
+/* Text to put before the reference's kernels in place of its hipHeader
+ * (src/StringResources.h:163-215) when they are compiled with symhip_compile: KERNEL as an
+ * extern "C" kernel (hipRTC looks kernels up by unmangled name), RealT = double, INDEX,
+ * restrict, and the hipThreadIdx_x / hipBlockDim_x / hipBlockIdx_x names hipRTC lacks. */
+const char* symhip_reference_prelude(void);
 /* Registers and residency of launch `idx` after compile: VGPRs per work-item, workgroups per
  * compute unit at its block and LDS size, compute units of the device. */
 int symhip_launch_info(symhip_t h, uint32_t idx, int* vgprs, int* blocks_per_cu, int* cus);

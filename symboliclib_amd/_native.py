@@ -71,6 +71,9 @@ SIGNATURES = {
     "symhip_update_tiles": (_I, [_VP, _U64, _U64, _VP]),
     "symhip_set_launch": (_I, [_VP, _U32, ctypes.POINTER(Launch)]),
     "symhip_set_launches": (_I, [_VP, _U32, ctypes.POINTER(Launch)]),
+    "symhip_set_launch_offsets": (_I, [_VP, _U32, _U64, _U64, _U64]),
+    "symhip_set_result_gather": (_I, [_VP, _I]),
+    "symhip_reference_prelude": (ctypes.c_char_p, []),
     "symhip_set_work_words": (_I, [_VP, _U64]),
     "symhip_read_work": (_I, [_VP, _U64, _U64, _VP]),
     "symhip_launch_info": (_I, [_VP, _U32, _IP, _IP, _IP]),

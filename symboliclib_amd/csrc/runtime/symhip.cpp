@@ -40,6 +40,18 @@ int hipFail(hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hipFail(e_, what); \
     } while (0)
 
+// A captured graph holds the launch arguments: drop it when they change.
+void dropGraph(symhip_t h) {
+    if (h->graphExec) {
+        (void)hipGraphExecDestroy(h->graphExec);
+        h->graphExec = nullptr;
+    }
+    if (h->graph) {
+        (void)hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+}
+
 std::string defaultArch() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
@@ -200,6 +212,11 @@ int symhip_create(symhip_t* out, const symhip_program_desc* d) {
         h->names.push_back(d->launches[i].name ? d->launches[i].name : "");
         h->launches.push_back(d->launches[i]);
     }
+    h->refForm.assign(h->launches.size(), 0);
+    h->refOffsets.assign(h->launches.size(), {0, 0, 0});
+    h->numConst = d->num_const;
+    h->numIndex = d->num_index;
+    h->numOutpos = d->num_outpos;
 
     auto upload = [&](void** dst, const void* src, size_t bytes, const char* what) -> int {
         if (bytes == 0) return SYMHIP_SUCCESS;
@@ -366,7 +383,7 @@ int symhip_run_graph(symhip_t h, void* stream) {
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         int rc = SYMHIP_SUCCESS;
         if (e == hipSuccess) {
-            for (uint32_t i = 0; i < h->launches.size() && !rc; ++i) rc = symhip_launch_one(h, i, cs);
+            rc = symhipLaunchAll(h, cs, h->work);
             hipGraph_t g = nullptr;
             hipError_t e2 = hipStreamEndCapture(cs, &g);
             if (!rc && e2 != hipSuccess) rc = hipFail(e2, "hipStreamEndCapture");
@@ -391,10 +408,51 @@ int symhip_run_graph(symhip_t h, void* stream) {
 
 int symhip_run(symhip_t h, void* stream) {
     if (!h) return fail(SYMHIP_E_INVALID, "symhip_run: null handle");
-    for (uint32_t i = 0; i < h->launches.size(); ++i) {
-        int rc = symhip_launch_one(h, i, stream);
-        if (rc) return rc;
+    return symhipLaunchAll(h, stream, h->work);
+}
+
+int symhip_set_launch_offsets(symhip_t h, uint32_t idx, uint64_t c_offset, uint64_t y_offset, uint64_t p_offset) {
+    if (!h || idx >= h->launches.size()) return fail(SYMHIP_E_INVALID, "symhip_set_launch_offsets: bad launch index");
+    if (c_offset > h->numConst || y_offset > h->numValues || p_offset > h->numIndex)
+        return fail(SYMHIP_E_INVALID, "symhip_set_launch_offsets: offset beyond its table (c " + std::to_string(c_offset) +
+                                          " of " + std::to_string(h->numConst) + ", y " + std::to_string(y_offset) +
+                                          " of " + std::to_string(h->numValues) + ", p " + std::to_string(p_offset) +
+                                          " of " + std::to_string(h->numIndex) + ")");
+    h->refForm[idx] = 1;
+    h->refOffsets[idx] = {c_offset, y_offset, p_offset};
+    dropGraph(h);
+    return SYMHIP_SUCCESS;
+}
+
+const char* symhip_reference_prelude(void) {
+    return "#define KERNEL extern \"C\" __global__\n"
+           "typedef unsigned int INDEX;\n"
+           "typedef double RealT;\n"
+           "#define restrict __restrict__\n"
+           "#define hipThreadIdx_x threadIdx.x\n"
+           "#define hipBlockDim_x blockDim.x\n"
+           "#define hipBlockIdx_x blockIdx.x\n"
+           "#define hipGridDim_x gridDim.x\n";
+}
+
+int symhip_set_result_gather(symhip_t h, int on) {
+    if (!h) return fail(SYMHIP_E_INVALID, "symhip_set_result_gather: null handle");
+    if (on) {
+        if (h->numOutpos != h->numResults)
+            return fail(SYMHIP_E_INVALID, "symhip_set_result_gather: the placement table has " +
+                                              std::to_string(h->numOutpos) + " entries for " +
+                                              std::to_string(h->numResults) + " results");
+        // every gather position must lie inside x
+        std::vector<uint32_t> pos(h->numOutpos);
+        if (!pos.empty())
+            HIP_TRY(hipMemcpy(pos.data(), h->o, sizeof(uint32_t) * pos.size(), hipMemcpyDeviceToHost), "hipMemcpy(outpos)");
+        for (uint32_t q : pos)
+            if (q >= h->numValues)
+                return fail(SYMHIP_E_INVALID, "symhip_set_result_gather: position " + std::to_string(q) +
+                                                  " outside the value buffer of " + std::to_string(h->numValues));
     }
+    h->gather = on != 0;
+    dropGraph(h);
     return SYMHIP_SUCCESS;
 }
 
@@ -431,6 +489,8 @@ int symhip_set_launches(symhip_t h, uint32_t n, const symhip_launch* launches) {
         h->names.push_back(launches[i].name ? launches[i].name : "");
         h->launches.push_back(launches[i]);
     }
+    h->refForm.assign(n, 0);
+    h->refOffsets.assign(n, {0, 0, 0});
     h->functions.clear();  // compile again before running
     h->grids.clear();
     if (h->graphExec) {
@@ -532,11 +592,32 @@ int symhipLaunch(symhip_t h, uint32_t idx, void* stream, uint32_t* work) {
     const symhip_launch& L = h->launches[idx];
     const uint32_t grid = h->grids[idx];
     if (grid == 0) return SYMHIP_SUCCESS;
+    if (h->refForm[idx]) {
+        // the reference's K(num): k<i>(x, c + constantsOffset[i], x + outputOffset[i], p + indexOffset[i])
+        const auto& off = h->refOffsets[idx];
+        double* c = h->c ? h->c + off[0] : nullptr;
+        double* y = h->x ? h->x + off[1] : nullptr;
+        uint32_t* p = h->p ? h->p + off[2] : nullptr;
+        void* args[] = {&h->x, &c, &y, &p};
+        HIP_TRY(hipModuleLaunchKernel(h->functions[idx], grid, 1, 1, L.block, 1, 1, L.shared, (hipStream_t)stream, args,
+                                      nullptr),
+                "hipModuleLaunchKernel");
+        return SYMHIP_SUCCESS;
+    }
     void* args[] = {&h->x, &h->x, &h->c, &h->p, &h->o, &h->result, &h->tiles, &work};
     HIP_TRY(hipModuleLaunchKernel(h->functions[idx], grid, 1, 1, L.block, 1, 1, L.shared, (hipStream_t)stream, args,
                                   nullptr),
             "hipModuleLaunchKernel");
     return SYMHIP_SUCCESS;
+}
+
+int symhipLaunchAll(symhip_t h, void* stream, uint32_t* work) {
+    if (h->functions.size() != h->launches.size()) return fail(SYMHIP_E_NOT_COMPILED, "symhip: program not compiled");
+    for (uint32_t i = 0; i < h->launches.size(); ++i) {
+        const int rc = symhipLaunch(h, i, stream, work);
+        if (rc) return rc;
+    }
+    return h->gather ? symhipGather(h, stream) : SYMHIP_SUCCESS;
 }
 
 int symhipFail(int code, const std::string& msg) { return fail(code, msg); }

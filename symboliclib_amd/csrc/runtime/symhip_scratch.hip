@@ -70,7 +70,5 @@ extern "C" int symhip_run_noalloc(symhip_t h, void* stream) {
         if (e != hipSuccess) return symhipFail((int)e, std::string("symhip_run_noalloc: clearing the scratch words: ") +
                                                            hipGetErrorString(e));
     }
-    for (uint32_t i = 0; i < h->launches.size(); ++i)
-        if ((rc = symhipLaunch(h, i, stream, ptr))) return rc;
-    return SYMHIP_SUCCESS;
+    return symhipLaunchAll(h, stream, ptr);
 }
