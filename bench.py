@@ -10,8 +10,9 @@ configuration of the tuner's space is predicted, the three best predicted are ti
 fastest is kept (--committed: the committed selection instead, --untuned: defaults).
 
 Single-instance workloads: c3 (one 256 x 256 cloth, 4,890,648 nonzeros), c2 (cotangent
-Laplacian of a 512 x 512 grid), c1 (A^T A + A of an icosphere); --form selects waves, tiled
-or fused (cloth only).
+Laplacian of a 512 x 512 grid), c1 (A^T A + A of an icosphere); --form selects waves, tiled,
+fused (element tiles for c2 and c3, product tiles for c1) or direct (the waves form of the
+direct element assembly, c2 and c3).
 
 One step = one evaluation of the compiled program (every launch) over one value set whose
 inputs are already resident in HBM.  Prints one JSON line with the throughput, the roofline
@@ -79,7 +80,8 @@ def time_events(fn, stream, reps):
 
 def copy_bandwidth(dev, stream):
     """The measured HBM ceiling: a hand-written STREAM copy (symhip_stream_copy) of 1 GiB, the
-    best of several grid sizes, HIP events on the launch stream; torch's copy for reference."""
+    best of its one-pass form and two grid-stride grids, HIP events on the launch stream;
+    torch's copy for reference."""
     import torch
 
     from symboliclib_amd import _native
@@ -89,7 +91,7 @@ def copy_bandwidth(dev, stream):
     dbuf = torch.empty_like(cbuf)
     sp = ctypes.c_void_p(stream.cuda_stream)
     best = 0.0
-    for blocks in (8192, 16384, 32768):
+    for blocks in (0, 256, 16384):
         run = lambda b=blocks: _native.check(  # noqa: E731
             L.symhip_stream_copy(ctypes.c_void_p(dbuf.data_ptr()), ctypes.c_void_p(cbuf.data_ptr()), cbuf.numel(), b,
                                  sp), "symhip_stream_copy")
@@ -99,22 +101,6 @@ def copy_bandwidth(dev, stream):
     dbuf.copy_(cbuf)
     torch_gbs = 2 * cbuf.numel() * 8 / (time_events(lambda: dbuf.copy_(cbuf), stream, 10) * 1e-3) / 1e9
     return best, torch_gbs
-
-
-def cpu_baseline(evals, seconds, O):
-    """evals: list of (callable, nnz); runs them round-robin for about `seconds`."""
-    evals[0][0]()  # warm
-    n, done = 0, 0
-    t0 = time.perf_counter()
-    while True:
-        f, nnz = evals[n % len(evals)]
-        f()
-        done += nnz
-        n += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    tc = time.perf_counter() - t0
-    return done / tc, n, tc
 
 
 # ---------------------------------------------------------------------------------------------
@@ -132,23 +118,19 @@ def build_single(name, opt):
         unit = Unit.ata_plus_a(cp, ri, **opt)
         counts = {"n_in": len(ri), "n_const": 0, "n_conn": len(ri) + n + 1}
         desc = {"workload": "A^T A + A, icosphere subdivision 5", "rows": n, "nnz_A": int(len(ri))}
-        return unit, [a], counts, desc, lambda: O.ata_plus_a(cp, ri, a), lambda: O.ata_plus_a(cp, ri, a)
+        return unit, [a], counts, desc, lambda: O.ata_plus_a(cp, ri, a)
     if name == "c2":
         V, F = W.c2_mesh()
         unit = Unit.cotan(len(V), F, **opt)
         counts = {"n_in": 3 * len(V), "n_const": 0, "n_conn": 3 * len(F)}
         desc = {"workload": "cotangent Laplacian, 512x512 jittered grid", "vertices": len(V), "faces": len(F)}
-        plan = O.CotanPlan(len(V), F)
-        nnz = O.cotan_pattern(len(V), F)[0][-1]
-        return unit, [W.to_colmajor(V)], counts, desc, lambda: O.cotan(V, F), lambda: plan.values(V, nnz)
+        return unit, [W.to_colmajor(V)], counts, desc, lambda: O.cotan(V, F)
     rest, F, X = W.c3_cloth()
     unit = Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, **opt)
     counts = W.bw_algorithmic_counts(len(rest), F)
     desc = {"workload": "Baraff-Witkin cloth Hessian (stretch+shear AD + bend), 256x256 sheet",
             "vertices": len(rest), "faces": len(F), "interior_edges": counts["interior_edges"]}
-    plan = O.BWPlan(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
-    return (unit, [W.to_colmajor(X)], counts, desc, lambda: O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND),
-            lambda: plan.values(X, with_scale=False))
+    return unit, [W.to_colmajor(X)], counts, desc, lambda: O.bw_hessian(rest, F, X, W.K_STRETCH, W.K_SHEAR, W.K_BEND)
 
 
 def run_single(args, dev, stream, sp):
@@ -157,15 +139,17 @@ def run_single(args, dev, stream, sp):
     import oracle as O
 
     if args.form == "fused":
-        if args.workload == "c2":
-            raise SystemExit("bench.py: the fused forms apply to the cloth Hessian (c3, c4) and A^T A + A (c1)")
         opt = fused_options(args)
     else:
         opt = {"tiled": args.form == "tiled", "tile_lds_bytes": args.tile_lds or 0,
                "threads_per_block": args.tpb or 256}
+        if args.form == "direct":
+            if args.workload == "c1":
+                raise SystemExit("bench.py: the direct element assembly applies to c2 and c3")
+            opt = {"direct_assembly": True, "threads_per_block": args.tpb or 256}
     log(f"[bench] building workload {args.workload} ({args.form})")
     t0 = time.perf_counter()
-    unit, inputs, counts, desc, oracle_fn, cpu_eval = build_single(args.workload, opt)
+    unit, inputs, counts, desc, oracle_fn = build_single(args.workload, opt)
     t1 = time.perf_counter()
     unit.compile()
     t2 = time.perf_counter()
@@ -235,12 +219,40 @@ def run_single(args, dev, stream, sp):
         "build_seconds": {"frontend": info["frontend_seconds"], "build": info["build_seconds"],
                           "compile": info["compile_seconds"]},
     }
+    unit.close()
     if not args.no_cpu:
-        v, n, tc = cpu_baseline([(cpu_eval, n_out)], args.cpu_seconds, O)
-        line["cpu_baseline"] = {"value": v, "unit": "nnz/s", "cores": O.num_threads(), "kind": "port",
-                                "sample": f"{n} full evaluations of the same workload by the FP64 oracle "
-                                          f"(OpenMP, {O.num_threads()} threads) in {tc:.1f} s"}
+        line["cpu_baseline"] = cpu_device_single(args)
+        log(f"[bench] cpu baseline: {line['cpu_baseline']['value']:.3g} nnz/s")
     return line
+
+
+def cpu_device_single(args):
+    """The reference's CPU path restated (CPU device, symcpu.h) on the same workload: the program
+    of the reference's own front-end for c1 (A^T A + A from makeSymbolic), the direct element
+    assembly for c2 and c3 (the same element and sum kernels, built without per-nonzero trees),
+    execute() repeated for about args.cpu_seconds after one warm-up."""
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    opt = {"cpu_threads": threads} if args.workload == "c1" else {"cpu_threads": threads, "direct_assembly": True}
+    unit, inputs, _, _, _ = build_single(args.workload, opt)
+    unit.compile()
+    build = time.perf_counter() - t0
+    unit.execute(*inputs)
+    nnz = int(unit.info()["num_results"])
+    n, ts = 0, time.perf_counter()
+    while True:
+        unit.execute(*inputs)
+        n += 1
+        if time.perf_counter() - ts >= args.cpu_seconds:
+            break
+    tc = time.perf_counter() - ts
+    unit.close()
+    form = "the reference front-end's program" if args.workload == "c1" else "the direct element assembly's program"
+    return {"value": nnz * n / tc, "unit": "nnz/s", "cores": threads, "kind": "port",
+            "sample": f"the whole workload ({nnz} output nonzeros), {form} on the CPU device (the reference's "
+                      f"VecWidth/NumThreads path restated: host C++ from the same program, OpenMP over {threads} "
+                      f"threads, clang with the reference's flags); {n} execute() calls in {tc:.1f} s after a warm-up "
+                      f"(build and compile {build:.0f} s, untimed)"}
 
 
 # ---------------------------------------------------------------------------------------------
@@ -515,7 +527,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--count", type=int, default=256, help="c4: number of cloths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -525,7 +537,8 @@ def main():
     ap.add_argument("--trials", type=int, default=3, help="c4: configurations timed in the in-run selection")
     ap.add_argument("--cpu-full", action="store_true", help="c4: CPU baseline over the whole batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--form", default="waves", choices=["tiled", "waves", "fused"], help="c1-c3 execution form")
+    ap.add_argument("--form", default="waves", choices=["tiled", "waves", "fused", "direct"],
+                    help="c1-c3 execution form (direct: waves form of the direct element assembly, c2/c3)")
     ap.add_argument("--graph", action="store_true", help="c1-c3: replay the launches from a HIP graph")
     ap.add_argument("--tile-lds", type=int, default=None, help="LDS bytes per tile")
     ap.add_argument("--tpb", type=int, default=None, help="threads per block")

@@ -193,10 +193,29 @@ const char* symhip_reference_prelude(void);
 /* Registers and residency of launch `idx` after compile: VGPRs per work-item, workgroups per
  * compute unit at its block and LDS size, compute units of the device. */
 int symhip_launch_info(symhip_t h, uint32_t idx, int* vgprs, int* blocks_per_cu, int* cus);
-/* STREAM-style copy of n doubles between 16-byte aligned device buffers with `blocks`
- * workgroups of 256 work-items, enqueued on `stream` (the measured HBM ceiling; not part of
- * the reference). */
+/* STREAM-style copy of n doubles between 16-byte aligned device buffers, enqueued on `stream`
+ * (the measured HBM ceiling; not part of the reference): blocks = 0 copies in one pass, one
+ * 16-byte vector per work-item (the fastest form on MI355X); otherwise `blocks` workgroups of
+ * 256 work-items loop over the buffers. */
 int symhip_stream_copy(double* dst, const double* src, size_t n, uint32_t blocks, void* stream);
+/*
+ * The implicit-Euler system of a cloth step, the step after the Hessian (reference:
+ * examples/BaraffWitkin/implementation/cloth.cpp:97-112, Cloth::generateMatrixAndRHS):
+ *   A = M - h (D + h K),   b = M v + h (f - D2 v)
+ * for a stiffness K and damping D on one symmetric n x n CSC pattern (outer n + 1, inner nnz,
+ * rows sorted per column; every column holds its diagonal entry), the lumped mass m (M =
+ * diag(m)), velocities v and forces f (n each).  D = NULL means no damping.  D2 is the
+ * symmetric matrix of D's lower triangle (selfadjointView<Lower>), which equals D for the
+ * symmetric D taken here.  A is written on K's pattern, b as n values.  Every pointer is device
+ * memory (K may be a program's output, symhip_output_device_ptr); enqueued on `stream`.
+ */
+int symhip_implicit_euler_system(int n, const int* outer, const int* inner, const double* K, const double* D,
+                                 const double* m, const double* v, const double* f, double h, double* A, double* b,
+                                 void* stream);
+/* y = A x for a symmetric n x n CSC A (the product of a conjugate-gradient solve of the system
+ * above; column j is read as row j).  Device memory, enqueued on `stream`. */
+int symhip_csc_symv(int n, const int* outer, const int* inner, const double* A, const double* x, double* y,
+                    void* stream);
 /* Rate of the device wall clock read by profiled kernels, in kHz. */
 int symhip_wall_clock_khz(int* khz);
 

@@ -295,6 +295,50 @@ def cotan_extended(V, F):
     return vals
 
 
+def implicit_euler_system(indptr, indices, K, D, m, v, f, h):
+    """The implicit-Euler system of a cloth step (examples/BaraffWitkin/implementation/
+    cloth.cpp:97-112, Cloth::generateMatrixAndRHS): A = M - h (D + h K) on K's CSC pattern with
+    M = diag(m), and b = M v + h (f - D2 v), D2 the symmetric matrix of D's lower triangle
+    (cloth.cpp:107, selfadjointView<Lower>: a stored entry D(i, j) with i >= j stands for
+    D2(i, j) and D2(j, i); entries above the diagonal are not read).  D = None: no damping.
+    Returns (A, b, scale_A, scale_b), the scales being the sums of the absolute terms."""
+    indptr = np.asarray(indptr, np.int64)
+    indices = np.asarray(indices, np.int64)
+    n = len(indptr) - 1
+    cols = np.repeat(np.arange(n), np.diff(indptr))
+    K = np.asarray(K, np.float64)
+    D = np.zeros_like(K) if D is None else np.asarray(D, np.float64)
+    m, v, f = (np.asarray(x, np.float64) for x in (m, v, f))
+    diag = indices == cols
+    a = -h * (D + h * K)
+    A = np.where(diag, m[cols] + a, a)
+    scale_A = np.where(diag, np.abs(m[cols]), 0.0) + abs(h) * (np.abs(D) + abs(h) * np.abs(K))
+    lower = indices >= cols
+    strict = indices > cols
+    D2v = np.zeros(n)
+    D2abs = np.zeros(n)
+    np.add.at(D2v, indices[lower], D[lower] * v[cols[lower]])
+    np.add.at(D2v, cols[strict], D[strict] * v[indices[strict]])
+    np.add.at(D2abs, indices[lower], np.abs(D[lower] * v[cols[lower]]))
+    np.add.at(D2abs, cols[strict], np.abs(D[strict] * v[indices[strict]]))
+    b = m * v + h * (f - D2v)
+    scale_b = np.abs(m * v) + abs(h) * (np.abs(f) + D2abs)
+    return A, b, scale_A, scale_b
+
+
+def csc_symv(indptr, indices, A, x):
+    """y = A x for a symmetric CSC A, and the scale sum_j |A(i, j) x(j)| of each y(i)."""
+    indptr = np.asarray(indptr, np.int64)
+    indices = np.asarray(indices, np.int64)
+    n = len(indptr) - 1
+    cols = np.repeat(np.arange(n), np.diff(indptr))
+    y = np.zeros(n)
+    s = np.zeros(n)
+    np.add.at(y, indices, A * x[cols])
+    np.add.at(s, indices, np.abs(A * x[cols]))
+    return y, s
+
+
 def rel_errors(values, ref_values, scale):
     """Per-value |v - v_ref| / scale (scale 0 -> absolute error)."""
     scale = np.where(scale > 0, scale, 1.0)

@@ -79,6 +79,8 @@ SIGNATURES = {
     "symhip_launch_info": (_I, [_VP, _U32, _IP, _IP, _IP]),
     "symhip_wall_clock_khz": (_I, [_IP]),
     "symhip_stream_copy": (_I, [_VP, _VP, _SZ, _U32, _VP]),
+    "symhip_implicit_euler_system": (_I, [_I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _VP, _VP, _VP]),
+    "symhip_csc_symv": (_I, [_I, _VP, _VP, _VP, _VP, _VP, _VP]),
     "symhip_set_arg": (_I, [_VP, _I, _DP, _SZ]),
     "symhip_set_arg_async": (_I, [_VP, _I, _DP, _SZ, _VP]),
     "symhip_set_arg_device": (_I, [_VP, _I, _VP, _SZ, _VP]),

@@ -217,6 +217,14 @@ int symlib_unit_cotan(symlib_unit_t* out, int nv, int nf, const int* faces, cons
         checkFaces(nv, nf, faces);
         std::unique_ptr<symlib_unit> u(new symlib_unit());
         const auto t0 = Clock::now();
+        if (opt && (opt->direct_assembly || opt->fused)) {
+            Sym::HessianAssembly A;
+            Sym::workloads::CotanShape shape;
+            Sym::workloads::cotanAssembly(A, shape, nv, toFaces(nf, faces), 0);
+            finishAssembly(u.get(), makeDevice(opt), t0, A);
+            *out = u.release();
+            return;
+        }
         Sym::DenseMatrix<double> V(nv, 3);
         auto VS = Sym::makeSymbolic(V, 0);
         Sym::SparseMatrix<Sym::Symbolic> LS;

@@ -1,7 +1,10 @@
 // symhip_stream_copy: a STREAM-style device copy, the measured HBM ceiling that kernel
-// throughputs are compared with besides the nominal 8 TB/s.  Each work-item moves 64 bytes
-// per iteration as four 16-byte loads issued before four 16-byte stores, over a grid-stride
-// loop; stores are non-temporal (the copy's destination is not read again).
+// throughputs are compared with besides the nominal 8 TB/s.  Two forms: one pass (every
+// work-item moves one 16-byte vector, as many workgroups as that takes: the most requests in
+// flight the dispatcher can keep resident, the fastest form on MI355X), or a grid-stride loop
+// over a given number of workgroups in which each work-item moves 64 bytes per iteration as
+// four 16-byte loads issued before four 16-byte stores.  Stores are non-temporal (the copy's
+// destination is not read again).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -25,6 +28,12 @@ __global__ void __launch_bounds__(256) symhipStreamCopy(v2d* __restrict__ dst, c
     for (; i < n2; i += stride) __builtin_nontemporal_store(src[i], dst + i);
 }
 
+__global__ void __launch_bounds__(256) symhipStreamCopyOnePass(v2d* __restrict__ dst, const v2d* __restrict__ src,
+                                                                size_t n2) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n2) __builtin_nontemporal_store(src[i], dst + i);
+}
+
 __global__ void __launch_bounds__(64) symhipStreamTail(double* __restrict__ dst, const double* __restrict__ src) {
     if (threadIdx.x == 0) dst[0] = src[0];
 }
@@ -32,12 +41,18 @@ __global__ void __launch_bounds__(64) symhipStreamTail(double* __restrict__ dst,
 }  // namespace
 
 extern "C" int symhip_stream_copy(double* dst, const double* src, size_t n, uint32_t blocks, void* stream) {
-    if ((n && (!dst || !src)) || blocks == 0) return symhipFail(SYMHIP_E_INVALID, "symhip_stream_copy: bad argument");
+    if (n && (!dst || !src)) return symhipFail(SYMHIP_E_INVALID, "symhip_stream_copy: bad argument");
     if (((uintptr_t)dst | (uintptr_t)src) & 15u)
         return symhipFail(SYMHIP_E_INVALID, "symhip_stream_copy: buffers must be 16-byte aligned");
     const size_t n2 = n / 2;
-    if (n2) hipLaunchKernelGGL(symhipStreamCopy, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (v2d*)dst,
-                               (const v2d*)src, n2);
+    if (blocks == 0 && (n2 + 255u) / 256u > 0x7fffffffu)
+        return symhipFail(SYMHIP_E_INVALID, "symhip_stream_copy: too long for one pass");
+    if (n2 && blocks == 0)
+        hipLaunchKernelGGL(symhipStreamCopyOnePass, dim3((uint32_t)((n2 + 255u) / 256u)), dim3(256), 0,
+                           (hipStream_t)stream, (v2d*)dst, (const v2d*)src, n2);
+    else if (n2)
+        hipLaunchKernelGGL(symhipStreamCopy, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (v2d*)dst,
+                           (const v2d*)src, n2);
     if (n & 1) hipLaunchKernelGGL(symhipStreamTail, dim3(1), dim3(64), 0, (hipStream_t)stream, dst + n - 1, src + n - 1);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return symhipFail((int)e, std::string("symhip_stream_copy: ") + hipGetErrorString(e));

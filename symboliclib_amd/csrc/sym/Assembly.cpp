@@ -3,12 +3,14 @@
 #include <algorithm>
 #include <limits>
 #include <map>
+#include <memory>
 #include <stdexcept>
 
 #include "ComputeKernel.hpp"
 #include "Parallel.hpp"
 #include "Template.hpp"
 #include "Timer.hpp"
+#include "Traverse.hpp"
 
 namespace Sym {
 
@@ -64,15 +66,97 @@ int HessianAssembly::defineShape(const Symbolic& energy, int numLocals, int numP
             const auto id = v.variable();
             if (id[1] != ELEMENT_PARAM || id[0] < 0 || id[0] >= numParams)
                 throw std::invalid_argument("HessianAssembly: element shape reads an unknown variable");
+            sh.otherLocal.push_back(-1);
             sh.otherParam.push_back(id[0]);
             sh.otherConst.push_back(0.0);
         } else {
+            sh.otherLocal.push_back(-1);
             sh.otherParam.push_back(-1);
             sh.otherConst.push_back(v.constant());
         }
     }
     shapes_.push_back(std::move(sh));
     return (int)shapes_.size() - 1;
+}
+
+int HessianAssembly::defineMatrixShape(int k, const Symbolic* entries, const int* patternLocals, int numLocals,
+                                       int numParams) {
+    if (k <= 0 || k > 255 || !entries || !patternLocals || numLocals < k || numParams < 0)
+        throw std::invalid_argument("HessianAssembly: bad element matrix shape");
+    // template slots: the k row locals, the remaining locals in order, the parameters
+    std::vector<int> slotOfLocal((size_t)numLocals, -1);
+    Shape sh;
+    sh.numLocals = numLocals;
+    sh.numParams = numParams;
+    for (int i = 0; i < k; ++i) {
+        const int q = patternLocals[i];
+        if (q < 0 || q >= numLocals || slotOfLocal[(size_t)q] >= 0)
+            throw std::invalid_argument("HessianAssembly: element matrix rows must be distinct locals");
+        slotOfLocal[(size_t)q] = i;
+        sh.diffLocal.push_back(q);
+    }
+    int slot = k;
+    for (int q = 0; q < numLocals; ++q)
+        if (slotOfLocal[(size_t)q] < 0) {
+            slotOfLocal[(size_t)q] = slot++;
+            sh.otherLocal.push_back(q);
+            sh.otherParam.push_back(-1);
+            sh.otherConst.push_back(0.0);
+        }
+    for (int j = 0; j < numParams; ++j) {
+        sh.otherLocal.push_back(-1);
+        sh.otherParam.push_back(j);
+        sh.otherConst.push_back(0.0);
+    }
+    const int firstParamSlot = slot;
+    auto toArgs = [&](const Symbolic& x) {
+        return traverseGenerate<Symbolic>(x, [&](const Symbolic& y, std::vector<Symbolic>& childs) {
+            if (y.op() == VAR) {
+                const auto v = y.variable();
+                if (v[1] == ELEMENT_LOCAL && v[0] >= 0 && v[0] < numLocals)
+                    return Symbolic(slotOfLocal[(size_t)v[0]], TEMPLATE_ARG);
+                if (v[1] == ELEMENT_PARAM && v[0] >= 0 && v[0] < numParams)
+                    return Symbolic(firstParamSlot + v[0], TEMPLATE_ARG);
+                throw std::invalid_argument("HessianAssembly: element matrix reads an unknown variable");
+            }
+            if (y.numChilds() == 0) return y;
+            return Symbolic(y.op(), childs);
+        });
+    };
+    auto t = std::make_unique<ExprTemplate>();
+    t->numArgs = firstParamSlot + numParams;
+    t->numDiffVars = k;
+    t->entryToOutput.assign((size_t)k * k, -1);
+    for (int i = 0; i < k; ++i)
+        for (int j = i; j < k; ++j) {
+            const Symbolic& a = entries[(size_t)i * k + j];
+            const Symbolic& b = entries[(size_t)j * k + i];
+            if (a.ahash() != b.ahash()) throw std::invalid_argument("HessianAssembly: element matrix is not symmetric");
+            t->entryToOutput[(size_t)i * k + j] = t->entryToOutput[(size_t)j * k + i] = t->numOutputs++;
+            t->outputs.push_back(toArgs(a));
+        }
+    // a signature no energy shape has: matrix templates are never shared by lookup
+    static int64_t matrixTemplates = 0;
+    t->signature = {std::numeric_limits<int64_t>::min(), ++matrixTemplates};
+    t->signatureHash = hash(std::vector<hash_t>{(hash_t)0x6d61747269785ull, (hash_t)matrixTemplates});
+    sh.tmpl = registerTemplate(std::move(t));
+    shapes_.push_back(std::move(sh));
+    return (int)shapes_.size() - 1;
+}
+
+void HessianAssembly::setNodePattern() {
+    if (inst_.empty()) throw std::logic_error("HessianAssembly: setNodePattern before addInstance");
+    inst_.back().nodePattern = true;
+}
+
+SparseMatrix<double> HessianAssembly::reportedPattern(size_t k, SparseMatrix<double>&& full) const {
+    const Instance& I = inst_[k];
+    if (!I.nodePattern || I.dofs == 1) return std::move(full);
+    const int n = I.numVars / I.dofs;
+    std::vector<int> outer(full.outer().begin(), full.outer().begin() + n + 1);
+    SparseMatrix<double> P(n, n);
+    P.setPattern(outer, full.inner());
+    return P;
 }
 
 void HessianAssembly::addElement(int shape, const int* vars, const double* params) {
@@ -87,8 +171,15 @@ void HessianAssembly::addElement(int shape, const int* vars, const double* param
         if (id < 0 || id >= I.numVars) throw std::invalid_argument("HessianAssembly: variable outside its instance");
         args_.push_back({0.0, id, 1});
     }
-    for (size_t q = 0; q < sh.otherParam.size(); ++q)
-        args_.push_back({sh.otherParam[q] >= 0 ? params[sh.otherParam[q]] : sh.otherConst[q], 0, 0});
+    for (size_t q = 0; q < sh.otherParam.size(); ++q) {
+        if (sh.otherLocal[q] >= 0) {
+            const int id = vars[sh.otherLocal[q]];
+            if (id < 0 || id >= I.numVars) throw std::invalid_argument("HessianAssembly: variable outside its instance");
+            args_.push_back({0.0, id, 1});
+        } else {
+            args_.push_back({sh.otherParam[q] >= 0 ? params[sh.otherParam[q]] : sh.otherConst[q], 0, 0});
+        }
+    }
     ++I.numElems;
 }
 
@@ -186,6 +277,9 @@ void HessianAssembly::instancePattern(size_t k, InstancePattern& out) const {
         const ExprTemplate& t = getTemplate(elemTmpl_[e]);
         const int nd = t.numDiffVars;
         const Arg* a = &args_[elemArg_[e]];
+        if (I.nodePattern)
+            for (int i = 0; i < nd; ++i)
+                if (a[i].id >= Nn) throw std::invalid_argument("HessianAssembly: node pattern with a row beyond dof 0");
         for (int i = 0; i < nd; ++i)
             for (int j = 0; j < nd; ++j) {
                 const int slot = t.entryToOutput[(size_t)i * nd + j];
@@ -194,6 +288,9 @@ void HessianAssembly::instancePattern(size_t k, InstancePattern& out) const {
                 src.push_back((uint64_t)e << 16 | (uint64_t)slot);
             }
     }
+    if (I.nodePattern && (!I.ncVal.empty() || std::any_of(I.cRow.begin(), I.cRow.end(), [&](int r) { return r >= Nn; }) ||
+                          std::any_of(I.cCol.begin(), I.cCol.end(), [&](int c) { return c >= Nn; })))
+        throw std::invalid_argument("HessianAssembly: node pattern with constants beyond dof 0");
     for (size_t c = 0; c < I.cRow.size(); ++c) {
         tr.push_back((uint32_t)I.cRow[c]);
         tc.push_back((uint32_t)I.cCol[c]);
@@ -382,7 +479,7 @@ Program HessianAssembly::build(const BuildOptions& opt, ProgramInputs& io, std::
         prog.outLength.push_back(nout);
         io.outputSizes.push_back(nout);
         resultBase += nout;
-        patterns.push_back(std::move(r.pattern));
+        patterns.push_back(reportedPattern(k, std::move(r.pattern)));
         r = InstancePattern();
     }
 

@@ -42,9 +42,20 @@ public:
     // Element shape: an energy written over local variables {k, ELEMENT_LOCAL}, k < numLocals,
     // and parameters {j, ELEMENT_PARAM}, j < numParams.  Differentiated once; returns its id.
     int defineShape(const Symbolic& energy, int numLocals, int numParams);
+    // Element shape given by its symmetric k x k element matrix instead of an energy (assembly
+    // operators such as the cotangent Laplacian): entries[i * k + j] over locals
+    // {q, ELEMENT_LOCAL}, q < numLocals, and parameters {j, ELEMENT_PARAM}, j < numParams;
+    // entries[i * k + j] and entries[j * k + i] must be the same expression.  Row and column i
+    // of the matrix is the variable of local patternLocals[i]; the other locals are inputs the
+    // entries read.  Returns its id (used like a shape of defineShape).
+    int defineMatrixShape(int k, const Symbolic* entries, const int* patternLocals, int numLocals, int numParams);
     // Element of the current instance with shape `shape`: local k is variable vars[k] of the
     // instance, parameter j has value params[j].  Builds no expression.
     void addElement(int shape, const int* vars, const double* params);
+    // The current instance's output is its first dof block only: an n x n CSC (n = numVars /
+    // dofs) whose rows and columns are nodes.  Every element row and column (and constant)
+    // must then be a dof-0 variable.
+    void setNodePattern();
     // Constant Hessian contribution (row, col) of the current instance (not mirrored).
     void addConstant(int row, int col, double value);
     // Constant block on the nodes of the current instance (dofs > 1): for every dof d,
@@ -66,6 +77,7 @@ public:
 private:
     struct Instance {
         int numVars = 0, group = 0, dofs = 1;
+        bool nodePattern = false;
         size_t firstElem = 0, numElems = 0;
         std::vector<int> cRow, cCol;
         std::vector<double> cVal;
@@ -83,6 +95,7 @@ private:
         uint32_t tmpl;
         int numLocals, numParams;
         std::vector<int> diffLocal;   // template diff slot -> local variable
+        std::vector<int> otherLocal;  // template other slot -> local variable (-1: none)
         std::vector<int> otherParam;  // template other slot -> parameter (-1: constant below)
         std::vector<double> otherConst;
     };
@@ -95,6 +108,8 @@ private:
         SparseMatrix<double> pattern;
     };
     void instancePattern(size_t k, InstancePattern& out) const;
+    // the pattern an instance reports (its first dof block for node patterns)
+    SparseMatrix<double> reportedPattern(size_t k, SparseMatrix<double>&& full) const;
     void inputLayout(ProgramInputs& io, std::vector<size_t>& inOff) const;
 
     std::vector<Shape> shapes_;

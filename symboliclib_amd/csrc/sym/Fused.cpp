@@ -576,7 +576,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         fp.outStart.push_back(resBase[k]);
         fp.outLength.push_back(nout);
         io.outputSizes.push_back(nout);
-        patterns.push_back(std::move(pats[k]));
+        patterns.push_back(reportedPattern(k, std::move(pats[k])));
     }
     fp.numResults = rb;
     fp.tileHeader.resize(nTiles);

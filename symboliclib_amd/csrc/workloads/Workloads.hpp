@@ -89,6 +89,50 @@ void buildCotan(const DenseMatrix<T>& V, const Faces& F, SparseMatrix<T>& L) {
     L.setFromTriplets(triplets.begin(), triplets.end());
 }
 
+// The same Laplacian as one instance of a direct assembly (no per-nonzero expression): one
+// element per face whose symmetric 3 x 3 element matrix holds the face's corner weights,
+// M(j1, j2) = M(j2, j1) = w_j for the corner j opposite edge (j1, j2) and
+// M(a, a) = -(w_b + w_c) over the two other corners, so that summing the element matrices adds
+// exactly the twelve triplets of buildCotan per face.  Inputs: the n x 3 position matrix
+// {v + d n, group}; output: the n x n Laplacian over the vertices (a node pattern).
+class CotanShape {
+    int id_ = -1;
+
+public:
+    int get(HessianAssembly& A) {
+        if (id_ >= 0) return id_;
+        // local a + 3 d: coordinate d of the face's vertex a, so locals 0..2 are the rows
+        auto P = [](int a) {
+            return Vec3<Symbolic>{Symbolic(a, ELEMENT_LOCAL), Symbolic(a + 3, ELEMENT_LOCAL), Symbolic(a + 6, ELEMENT_LOCAL)};
+        };
+        const Vec3<Symbolic> e0 = P(0) - P(1), e1 = P(0) - P(2);
+        const Symbolic area2 = norm(e0.cross(e1));
+        Symbolic w[3];
+        for (int j = 0; j < 3; ++j) w[j] = (P(j) - P((j + 1) % 3)).dot(P(j) - P((j + 2) % 3)) / area2;
+        Symbolic M[9];
+        for (int j = 0; j < 3; ++j) {
+            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            M[j1 * 3 + j2] = M[j2 * 3 + j1] = w[j];
+            M[j * 3 + j] = -(w[j1] + w[j2]);
+        }
+        const int rows[3] = {0, 1, 2};
+        id_ = A.defineMatrixShape(3, M, rows, 9, 0);
+        return id_;
+    }
+};
+
+inline void cotanAssembly(HessianAssembly& A, CotanShape& shape, int n, const Faces& F, int group) {
+    A.addInstance(3 * n, group, 3);
+    A.setNodePattern();
+    const int s = shape.get(A);
+    int vars[9];
+    for (const auto& f : F) {
+        for (int a = 0; a < 3; ++a)
+            for (int d = 0; d < 3; ++d) vars[a + 3 * d] = f[a] + d * n;
+        A.addElement(s, vars, nullptr);
+    }
+}
+
 /* Baraff-Witkin stretch + shear ***********************************************/
 
 struct RestShape {

@@ -144,6 +144,13 @@ class Unit:
                       "symhip_get_result_device")
         return self
 
+    def output_device_ptr(self, k=0):
+        """Device address of output container k (valid until close; compile() first)."""
+        rt = self.runtime()
+        if not rt:
+            raise _native.SymlibError("output_device_ptr: compile() first")
+        return _native.lib().symhip_output_device_ptr(rt, k)
+
     def run(self, stream=None):
         _native.check(_native.lib().symlib_unit_run(self._h, stream), "symlib_unit_run")
         return self

@@ -56,6 +56,29 @@ class DeviceArray:
             self.ptr = ctypes.c_void_p()
 
 
+class DeviceBuffer:
+    """A copy of a numpy array (any dtype) in device memory."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a)
+        self.dtype, self.n = a.dtype, a.size
+        self.ptr = ctypes.c_void_p()
+        _ok(hip().hipMalloc(ctypes.byref(self.ptr), max(8, a.nbytes)), "hipMalloc")
+        if a.nbytes:
+            _ok(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        if out.nbytes:
+            _ok(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, D2H), "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _ok(hip().hipFree(self.ptr), "hipFree")
+            self.ptr = ctypes.c_void_p()
+
+
 class Stream:
     def __init__(self):
         self.handle = ctypes.c_void_p()

@@ -56,6 +56,18 @@ def test_cotan_on_cpu(side):
         _check(f"cotan {side}", u, *O.cotan(V, F), u.get_results())
 
 
+@pytest.mark.parametrize("side", [2, 5, 64])
+def test_cotan_direct_assembly_on_cpu(side):
+    """The element-matrix assembly of the Laplacian (one 3 x 3 symmetric element matrix per
+    face, a node pattern over the vertices): same pattern and values as the per-nonzero trees."""
+    V, F = W.c2_mesh(side=side)
+    with Unit.cotan(len(V), F, cpu_threads=THREADS, direct_assembly=True) as u:
+        info = u.info()
+        assert info["num_kernels"] >= 2 and info["num_waves"] == 2, info
+        u.compile().execute(W.to_colmajor(V))
+        _check(f"cotan direct {side}", u, *O.cotan(V, F), u.get_results())
+
+
 @pytest.mark.parametrize("side,share,direct,kb", [(4, False, False, W.K_BEND), (8, True, False, W.K_BEND),
                                                   (8, True, True, W.K_BEND), (9, True, False, 0.0),
                                                   (12, True, True, 0.0)])

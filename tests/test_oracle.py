@@ -314,3 +314,70 @@ def test_scales_bound_values():
     n, cp, ri, a = W.c1_matrix(subdiv=2)
     _, _, v1, s1 = O.ata_plus_a(cp, ri, np.abs(a))
     assert np.allclose(s1, v1, rtol=1e-14)  # all terms positive: S = v
+
+
+# ---- the implicit-Euler system (examples/BaraffWitkin/implementation/cloth.cpp:97-112)
+
+def _dense_csc(M):
+    """CSC arrays of every entry of a dense matrix (a full pattern)."""
+    n = M.shape[0]
+    indptr = np.arange(0, n * n + 1, n, dtype=np.int32)
+    indices = np.tile(np.arange(n, dtype=np.int32), n)
+    return indptr, indices, M.T.reshape(-1).copy()  # column-major values
+
+
+def test_implicit_euler_system_known_answer():
+    # K = [[2, -1], [-1, 3]], D = [[0.5, 0.1], [0.1, 0.2]], m = (1, 2), v = (1, -1),
+    # f = (0.3, 0.4), h = 0.5 by hand: A = M - h (D + h K) = [[0.25, 0.2], [0.2, 1.15]],
+    # b = M v + h (f - D v) = (1, -2) + 0.5 ((0.3, 0.4) - (0.4, -0.1)) = (0.95, -1.75)
+    K = np.array([[2.0, -1.0], [-1.0, 3.0]])
+    D = np.array([[0.5, 0.1], [0.1, 0.2]])
+    indptr, indices, k = _dense_csc(K)
+    _, _, d = _dense_csc(D)
+    A, b, sa, sb = O.implicit_euler_system(indptr, indices, k, d, [1.0, 2.0], [1.0, -1.0], [0.3, 0.4], 0.5)
+    assert np.allclose(A, [0.25, 0.2, 0.2, 1.15], rtol=0, atol=1e-15)
+    assert np.allclose(b, [0.95, -1.75], rtol=0, atol=1e-15)
+    assert (sa >= np.abs(A) - 1e-15).all() and (sb >= np.abs(b) - 1e-15).all()
+
+
+def test_implicit_euler_system_reads_the_lower_triangle_of_the_damping():
+    """b uses D2 = tril(D) + tril(D, -1)^T (selfadjointView<Lower>): entries above the
+    diagonal are ignored, so a non-symmetric D gives the same b as its lower triangle
+    mirrored; checked against a dense computation."""
+    rng = np.random.default_rng(5)
+    n = 7
+    K = rng.standard_normal((n, n))
+    K = K + K.T
+    D = rng.standard_normal((n, n))  # not symmetric
+    m, v, f = rng.uniform(0.5, 2, n), rng.standard_normal(n), rng.standard_normal(n)
+    h = 0.05
+    indptr, indices, k = _dense_csc(K)
+    _, _, d = _dense_csc(D)
+    A, b, _, _ = O.implicit_euler_system(indptr, indices, k, d, m, v, f, h)
+    D2 = np.tril(D) + np.tril(D, -1).T
+    assert np.allclose(b, m * v + h * (f - D2 @ v), rtol=0, atol=1e-13)
+    Adense = np.diag(m) - h * (D + h * K)
+    assert np.allclose(A, Adense.T.reshape(-1), rtol=0, atol=1e-13)
+    # without damping: b = M v + h f exactly
+    _, b0, _, _ = O.implicit_euler_system(indptr, indices, k, None, m, v, f, h)
+    assert np.array_equal(b0, m * v + h * f)
+
+
+def test_csc_symv_against_dense_product():
+    rng = np.random.default_rng(6)
+    n = 9
+    S = rng.standard_normal((n, n))
+    S = S + S.T
+    S[np.abs(S) < 0.8] = 0.0  # a sparse symmetric pattern
+    np.fill_diagonal(S, 1.0 + np.arange(n))
+    indptr = np.zeros(n + 1, np.int32)
+    indices, vals = [], []
+    for j in range(n):
+        rows = np.nonzero(S[:, j])[0]
+        indices += list(rows)
+        vals += list(S[rows, j])
+        indptr[j + 1] = len(indices)
+    x = rng.standard_normal(n)
+    y, s = O.csc_symv(indptr, np.array(indices, np.int32), np.array(vals), x)
+    assert np.allclose(y, S @ x, rtol=0, atol=1e-13)
+    assert (s >= np.abs(y) - 1e-13).all()

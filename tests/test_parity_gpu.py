@@ -111,6 +111,31 @@ def test_cotan(require_device, side, tiled):
         _check(f"cotan grid {side}x{side}", u, O.cotan(V, F), u.get_results())
 
 
+@pytest.mark.parametrize("side", [3, 64, 512])
+def test_cotan_direct_assembly(require_device, side):
+    V, F = W.c2_mesh(side=side)
+    with Unit.cotan(len(V), F, direct_assembly=True) as u:
+        u.compile().execute(W.to_colmajor(V))
+        _check(f"cotan direct {side}x{side}", u, O.cotan(V, F), u.get_results())
+
+
+@pytest.mark.parametrize("side,lds,groups,schedule", [(3, 53760, 0, "queue"), (64, 40448, 1, "static"),
+                                                      (64, 81408, 2, "largest"), (512, 53760, 0, "queue"),
+                                                      (512, 81408, 1, "static"), (512, 81408, 3, "strided")])
+def test_cotan_fused(require_device, side, lds, groups, schedule):
+    """The cotangent Laplacian through the fused element tiles (element matrices in LDS, sums
+    per owned vertex column), replayed from a graph a second time with new positions."""
+    V, F = W.c2_mesh(side=side)
+    with Unit.cotan(len(V), F, fused=True, tile_lds_bytes=lds, output_groups=groups, schedule=schedule) as u:
+        assert u.info()["tiled"] == 2, u.info()["tiling"]
+        u.compile().execute(W.to_colmajor(V))
+        _check(f"cotan fused {side}x{side} lds={lds} groups={groups} {schedule}", u, O.cotan(V, F), u.get_results())
+        V2, _ = W.c2_mesh(side=side, seed=77)
+        u.execute(W.to_colmajor(V2))
+        u.run_graph()
+        _check(f"cotan fused {side}x{side} second positions", u, O.cotan(V2, F), u.get_results())
+
+
 @pytest.mark.parametrize("side,share,tiled", [(4, False, True), (4, True, True), (4, True, False), (32, True, True),
                                               (32, True, False), (256, True, True)])
 def test_bw_hessian(require_device, side, share, tiled):
