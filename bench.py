@@ -219,11 +219,64 @@ def run_single(args, dev, stream, sp):
         "build_seconds": {"frontend": info["frontend_seconds"], "build": info["build_seconds"],
                           "compile": info["compile_seconds"]},
     }
+    if args.workload == "c3":
+        line["consumer"] = consumer_step(unit, dev, stream, sp, reps)
     unit.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_device_single(args)
         log(f"[bench] cpu baseline: {line['cpu_baseline']['value']:.3g} nnz/s")
     return line
+
+
+def consumer_step(unit, dev, stream, sp, reps):
+    """The step after the Hessian on the cloth's K in place: the implicit-Euler system
+    A = M - h (D + h K), b = M v + h (f - D v) (symhip_implicit_euler_system, without damping)
+    and the symmetric product y = A x a solver runs (symhip_csc_symv), each timed with HIP
+    events; algorithmic bytes per launch: system 20 B per nonzero (K, row index, A) + 36 B per
+    column (column pointer, m, v, f, b), product 12 B per nonzero (A, row index) + 20 B per
+    column (column pointer, x, y)."""
+    import torch
+
+    from symboliclib_amd import _native
+
+    L = _native.lib()
+    outer, inner = unit.output_pattern()
+    n, nnz = len(outer) - 1, len(inner)
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)  # noqa: E731
+    o, i = t(outer, torch.int32), t(inner, torch.int32)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    m = torch.full((n,), 1.0 / (n // 3), dtype=torch.float64, device=dev)
+    v = (0.1 * torch.rand(n, generator=g, dtype=torch.float64)).to(dev)
+    f = torch.randn(n, generator=g, dtype=torch.float64).to(dev)
+    x = torch.randn(n, generator=g, dtype=torch.float64).to(dev)
+    A = torch.empty(nnz, dtype=torch.float64, device=dev)
+    b, y = torch.empty_like(m), torch.empty_like(m)
+    P = lambda a: ctypes.c_void_p(a.data_ptr())  # noqa: E731
+    K = unit.output_device_ptr(0)
+
+    def system():
+        _native.check(L.symhip_implicit_euler_system(n, P(o), P(i), K, None, P(m), P(v), P(f), 0.05, P(A), P(b), sp),
+                      "symhip_implicit_euler_system")
+
+    def symv():
+        _native.check(L.symhip_csc_symv(n, P(o), P(i), P(A), P(x), P(y), sp), "symhip_csc_symv")
+
+    system()
+    symv()
+    torch.cuda.synchronize()
+    ms_sys = time_events(system, stream, reps)
+    ms_mv = time_events(symv, stream, reps)
+    b_sys, b_mv = 20 * nnz + 36 * n, 12 * nnz + 20 * n
+    out = {"workload": "implicit-Euler system of the cloth (h = 0.05, no damping) and y = A x on its pattern",
+           "n": n, "nnz": nnz,
+           "system": {"kernel": "symhipImplicitEulerSystem", "ms": ms_sys, "algorithmic_bytes": b_sys,
+                      "achieved_gbs": b_sys / (ms_sys * 1e-3) / 1e9,
+                      "frac": b_sys / (ms_sys * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "symv": {"kernel": "symhipCscSymv", "ms": ms_mv, "algorithmic_bytes": b_mv,
+                    "achieved_gbs": b_mv / (ms_mv * 1e-3) / 1e9, "frac": b_mv / (ms_mv * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    log(f"[bench] consumer: system {ms_sys:.4f} ms ({out['system']['achieved_gbs']:.0f} GB/s), "
+        f"symv {ms_mv:.4f} ms ({out['symv']['achieved_gbs']:.0f} GB/s)")
+    return out
 
 
 def cpu_device_single(args):

@@ -97,6 +97,16 @@ int symlib_unit_bw_hessian_batch(symlib_unit_t* out, int count, const int* num_v
                                  const int* const* faces, const double* const* rest, double k_stretch, double k_shear,
                                  double k_bend, const symlib_options* opt);
 
+/* The built program on disk (reference: the constData / indexData / outIndexData files init()
+ * writes and the generated library reads, src/ComputeUnit.cpp:364-366,
+ * src/StringResources.h:233-269): save writes the unit's tables, templates, source, launches,
+ * tile pools and output patterns to one file; load constructs the same unit from it without
+ * front-end or build (compile then finds the code object in the runtime's cache).  opt picks
+ * the device kind (cpu_threads > 0: the CPU device), which must be the saved one; every other
+ * setting comes from the file.  Files are checked on load (magic, version, sizes, indices). */
+int symlib_unit_save(symlib_unit_t u, const char* path);
+int symlib_unit_load(symlib_unit_t* out, const char* path, const symlib_options* opt);
+
 int symlib_unit_compile(symlib_unit_t u);
 /* inputs[i] has lengths[i] values; one entry per input container, in order */
 int symlib_unit_execute(symlib_unit_t u, int num_inputs, const double* const* inputs, const size_t* lengths);
@@ -107,6 +117,9 @@ int symlib_unit_run_graph(symlib_unit_t u, void* stream);
 int symlib_unit_get_results(symlib_unit_t u, int k, double* out, size_t n);
 
 int symlib_unit_num_outputs(symlib_unit_t u);
+/* input containers and the number of values of input container i */
+int symlib_unit_num_inputs(symlib_unit_t u);
+int symlib_unit_input_length(symlib_unit_t u, int i, uint64_t* n);
 /* rows, cols, nnz of output container k */
 int symlib_unit_output_shape(symlib_unit_t u, int k, int* rows, int* cols, uint64_t* nnz);
 /* CSC outer (cols + 1) and inner (nnz) arrays of output container k */

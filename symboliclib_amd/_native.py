@@ -98,6 +98,10 @@ SIGNATURES = {
     "symhip_last_error": (ctypes.c_char_p, []),
     # symlib.h
     "symlib_default_options": (None, [ctypes.POINTER(Options)]),
+    "symlib_unit_save": (_I, [_VP, ctypes.c_char_p]),
+    "symlib_unit_num_inputs": (_I, [_VP]),
+    "symlib_unit_input_length": (_I, [_VP, _I, ctypes.POINTER(_U64)]),
+    "symlib_unit_load": (_I, [ctypes.POINTER(_VP), ctypes.c_char_p, ctypes.POINTER(Options)]),
     "symlib_unit_ata_plus_a": (_I, [ctypes.POINTER(_VP), _I, _VP, _VP, ctypes.POINTER(Options)]),
     "symlib_unit_cotan": (_I, [ctypes.POINTER(_VP), _I, _I, _VP, ctypes.POINTER(Options)]),
     "symlib_unit_bw_hessian": (_I, [ctypes.POINTER(_VP), _I, _I, _VP, _VP, _D, _D, _D, ctypes.POINTER(Options)]),

@@ -61,7 +61,8 @@ Sym::Device makeDevice(const symlib_options* opt) {
 struct symlib_unit {
     std::unique_ptr<Sym::ComputeUnit<double>> unit;
     std::string scratch;  // fused_source result
-    std::vector<Sym::SparseMatrix<double>> patterns;  // output patterns (values unused)
+    // output patterns (kept by the unit, so saved with it)
+    const std::vector<Sym::SparseMatrix<double>>& patterns() const { return unit->outputPatterns(); }
     symlib_unit_info info{};
 };
 
@@ -89,10 +90,7 @@ void finishUnit(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, Cl
     u->info.build_seconds = seconds(t1, t2);
 }
 
-void finishAssembly(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, const Sym::HessianAssembly& A) {
-    const Clock::time_point t1 = Clock::now();
-    u->unit.reset(new Sym::ComputeUnit<double>(dev, A));
-    const Clock::time_point t2 = Clock::now();
+void fillInfo(symlib_unit* u) {
     const auto& P = u->unit->getProgram();
     u->info.num_inputs = P.numInputs;
     u->info.num_intermediates = P.numIntermediates;
@@ -115,9 +113,15 @@ void finishAssembly(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0
         u->info.tile_pool_entries = F.pool.size();
     }
     std::snprintf(u->info.tiling, sizeof(u->info.tiling), "%s", u->unit->tilingInfo().c_str());
+}
+
+void finishAssembly(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, const Sym::HessianAssembly& A) {
+    const Clock::time_point t1 = Clock::now();
+    u->unit.reset(new Sym::ComputeUnit<double>(dev, A));
+    const Clock::time_point t2 = Clock::now();
+    fillInfo(u);
     u->info.frontend_seconds = seconds(t0, t1);
     u->info.build_seconds = seconds(t1, t2);
-    u->patterns = u->unit->outputPatterns();
 }
 
 void finishProducts(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0, const Sym::ProductSum& ps) {
@@ -136,7 +140,6 @@ void finishProducts(symlib_unit* u, const Sym::Device& dev, Clock::time_point t0
     std::snprintf(u->info.tiling, sizeof(u->info.tiling), "%s", u->unit->tilingInfo().c_str());
     u->info.frontend_seconds = seconds(t0, t1);
     u->info.build_seconds = seconds(t1, t2);
-    u->patterns = u->unit->outputPatterns();
 }
 
 Sym::workloads::Faces toFaces(int nf, const int* faces) {
@@ -147,7 +150,7 @@ Sym::workloads::Faces toFaces(int nf, const int* faces) {
 
 Sym::SparseMatrix<double> patternOf(const Sym::SparseMatrix<Sym::Symbolic>& S) {
     Sym::SparseMatrix<double> P(S.rows(), S.cols());
-    P.setPattern(S.outer(), S.inner());
+    P.setStructure(S.outer(), S.inner());
     return P;
 }
 
@@ -205,7 +208,7 @@ int symlib_unit_ata_plus_a(symlib_unit_t* out, int n, const int* colptr, const i
         auto BS = Sym::workloads::ataPlusA(AS);
         const auto t1 = Clock::now();
         finishUnit(u.get(), makeDevice(opt), t0, t1, AS, BS);
-        u->patterns.push_back(patternOf(BS));
+        u->unit->setOutputPatterns({patternOf(BS)});
         *out = u.release();
     });
 }
@@ -231,7 +234,7 @@ int symlib_unit_cotan(symlib_unit_t* out, int nv, int nf, const int* faces, cons
         Sym::workloads::buildCotan(VS, toFaces(nf, faces), LS);
         const auto t1 = Clock::now();
         finishUnit(u.get(), makeDevice(opt), t0, t1, VS, LS);
-        u->patterns.push_back(patternOf(LS));
+        u->unit->setOutputPatterns({patternOf(LS)});
         *out = u.release();
     });
 }
@@ -260,7 +263,7 @@ int symlib_unit_bw_hessian(symlib_unit_t* out, int nv, int nf, const int* faces,
         auto HS = Sym::workloads::baraffWitkinHessian(XS, R, toFaces(nf, faces), k_stretch, k_shear, k_bend, share);
         const auto t1 = Clock::now();
         finishUnit(u.get(), makeDevice(opt), t0, t1, XS, HS);
-        u->patterns.push_back(patternOf(HS));
+        u->unit->setOutputPatterns({patternOf(HS)});
         *out = u.release();
     });
 }
@@ -284,6 +287,24 @@ int symlib_unit_bw_hessian_batch(symlib_unit_t* out, int count, const int* nv, c
         }
         finishAssembly(u.get(), makeDevice(opt), t0, A);
         *out = u.release();
+    });
+}
+
+int symlib_unit_save(symlib_unit_t u, const char* path) {
+    if (!u || !path) return 1;
+    return guarded([&] { u->unit->save(path); });
+}
+
+int symlib_unit_load(symlib_unit_t* out, const char* path, const symlib_options* opt) {
+    if (!out || !path) return 1;
+    *out = nullptr;
+    return guarded([&] {
+        std::unique_ptr<symlib_unit> u(new symlib_unit());
+        const auto t0 = Clock::now();
+        u->unit.reset(new Sym::ComputeUnit<double>(makeDevice(opt), Sym::SavedProgram{path}));
+        fillInfo(u.get());
+        u->info.build_seconds = seconds(t0, Clock::now());
+            *out = u.release();
     });
 }
 
@@ -408,14 +429,25 @@ int symlib_unit_get_results(symlib_unit_t u, int k, double* out, size_t n) {
     return guarded([&] { u->unit->getOutput(k, out, n); });
 }
 
-int symlib_unit_num_outputs(symlib_unit_t u) { return u ? (int)u->patterns.size() : 0; }
+int symlib_unit_num_outputs(symlib_unit_t u) { return u ? (int)u->patterns().size() : 0; }
+
+int symlib_unit_num_inputs(symlib_unit_t u) { return u ? (int)u->unit->numInputs() : 0; }
+
+int symlib_unit_input_length(symlib_unit_t u, int i, uint64_t* n) {
+    if (!u || !n || i < 0 || (size_t)i >= u->unit->numInputs()) {
+        g_err = "bad input index";
+        return 1;
+    }
+    *n = u->unit->inputSize(i);
+    return 0;
+}
 
 int symlib_unit_output_shape(symlib_unit_t u, int k, int* rows, int* cols, uint64_t* nnz) {
-    if (!u || k < 0 || k >= (int)u->patterns.size()) {
+    if (!u || k < 0 || k >= (int)u->patterns().size()) {
         g_err = "bad output index";
         return 1;
     }
-    const auto& P = u->patterns[k];
+    const auto& P = u->patterns()[k];
     if (rows) *rows = P.rows();
     if (cols) *cols = P.cols();
     if (nnz) *nnz = P.nonZeros();
@@ -423,11 +455,11 @@ int symlib_unit_output_shape(symlib_unit_t u, int k, int* rows, int* cols, uint6
 }
 
 int symlib_unit_output_pattern(symlib_unit_t u, int k, int* outer, int* inner) {
-    if (!u || k < 0 || k >= (int)u->patterns.size()) {
+    if (!u || k < 0 || k >= (int)u->patterns().size()) {
         g_err = "bad output index";
         return 1;
     }
-    const auto& P = u->patterns[k];
+    const auto& P = u->patterns()[k];
     if (outer) std::copy(P.outer().begin(), P.outer().end(), outer);
     if (inner) std::copy(P.inner().begin(), P.inner().end(), inner);
     return 0;

@@ -155,7 +155,7 @@ SparseMatrix<double> HessianAssembly::reportedPattern(size_t k, SparseMatrix<dou
     const int n = I.numVars / I.dofs;
     std::vector<int> outer(full.outer().begin(), full.outer().begin() + n + 1);
     SparseMatrix<double> P(n, n);
-    P.setPattern(outer, full.inner());
+    P.setStructure(std::move(outer), full.releaseInner());
     return P;
 }
 
@@ -352,7 +352,7 @@ void HessianAssembly::instancePattern(size_t k, InstancePattern& out) const {
     }
     outer[nvar] = (int)inner.size();
     out.pattern = SparseMatrix<double>(nvar, nvar);
-    out.pattern.setPattern(outer, inner);
+    out.pattern.setStructure(std::move(outer), std::move(inner));
 }
 
 Program HessianAssembly::build(const BuildOptions& opt, ProgramInputs& io, std::vector<SparseMatrix<double>>& patterns) const {

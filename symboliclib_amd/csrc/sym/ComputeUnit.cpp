@@ -141,7 +141,7 @@ void ComputeUnitBase::initFromProducts(const ProductSum& ps) {
     program.outStart.push_back(0);
     program.outLength.push_back(fusedProgram.numResults);
     SparseMatrix<double> pat(ps.n, ps.n);
-    pat.setPattern(outer, inner);
+    pat.setStructure(std::move(outer), std::move(inner));
     assemblyPatterns.clear();
     assemblyPatterns.push_back(std::move(pat));
 }

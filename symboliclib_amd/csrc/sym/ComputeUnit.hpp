@@ -146,6 +146,11 @@ struct ProductSum {
     const int* rowidx;
 };
 
+// A program saved with ComputeUnitBase::save, to construct a unit from without rebuilding it.
+struct SavedProgram {
+    std::string path;
+};
+
 // Everything of ComputeUnit that does not depend on the scalar type.
 class ComputeUnitBase {
 protected:
@@ -173,6 +178,7 @@ protected:
     void init();
     void initFromAssembly(const HessianAssembly& assembly);
     void initFromProducts(const ProductSum& ps);
+    void initFromFile(const std::string& path);
     void finalize();
     void setArgImpl(int i, const double* d, size_t n);
     void getResultImpl(int k, double* d, size_t n);
@@ -187,6 +193,9 @@ public:
     virtual ~ComputeUnitBase();
 
     void close();
+    // Write the built program (tables, templates, source, launches, tile pools, patterns) to
+    // `path`; a unit constructed from SavedProgram{path} is the same program.
+    void save(const std::string& path) const;
     bool compiled() const { return rt != nullptr || cpu != nullptr; }
     bool onCpu() const { return cpuDevice; }
 
@@ -207,6 +216,8 @@ public:
     std::vector<uint64_t> tileProfile() const;
     // output patterns of a unit built from a HessianAssembly (one per instance)
     const std::vector<SparseMatrix<double>>& outputPatterns() const { return assemblyPatterns; }
+    // record the output patterns of a unit built from expressions (kept by save())
+    void setOutputPatterns(std::vector<SparseMatrix<double>> p) { assemblyPatterns = std::move(p); }
     const std::string& tilingInfo() const { return tilingNote; }
     size_t tilePoolSize() const { return tilePool.size(); }
     symhip_t runtime() const { return rt; }
@@ -266,6 +277,11 @@ public:
 
     // Unit of B = A^T A + A in the fused product-sum form (input: A's values, output: B's).
     ComputeUnit(Device device_, const ProductSum& ps) : ComputeUnitBase(device_) { initFromProducts(ps); }
+
+    // A program written by save(): no front-end, no build; compile() finds its code object in
+    // the runtime's cache.  The device kind (UseHIP or the CPU device) must be the saved one;
+    // the other Device settings come from the file.
+    ComputeUnit(Device device_, const SavedProgram& saved) : ComputeUnitBase(device_) { initFromFile(saved.path); }
 
     ComputeUnit& compile() {
         compileProgram();

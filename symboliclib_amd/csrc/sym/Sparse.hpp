@@ -82,7 +82,7 @@ public:
 
     int rows() const { return rows_; }
     int cols() const { return cols_; }
-    size_t nonZeros() const { return values_.size(); }
+    size_t nonZeros() const { return inner_.size(); }
     size_t size() const { return values_.size(); }
 
     T* valuePtr() { return values_.data(); }
@@ -158,6 +158,15 @@ public:
         for (size_t k = 0; k < values_.size(); ++k) R.values()[k] = U(values_[k]);
         return R;
     }
+
+    // The pattern alone, without values (valuePtr() is then null): output patterns.
+    void setStructure(std::vector<int> outer, std::vector<int> inner) {
+        outer_ = std::move(outer);
+        inner_ = std::move(inner);
+        values_.clear();
+        values_.shrink_to_fit();
+    }
+    std::vector<int> releaseInner() { return std::move(inner_); }
 
     void setPattern(const std::vector<int>& outer, const std::vector<int>& inner) {
         outer_ = outer;

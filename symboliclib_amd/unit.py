@@ -9,6 +9,7 @@ These calls map one to one onto ComputeUnit<double>(Device(UseHIP(), ThreadsPerB
 in, out).compile().execute(in).getResults(out) of the reference (src/ComputeUnit.hpp:192-222).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -47,6 +48,43 @@ class Unit:
         self._h = handle
         self.input_lengths = list(input_lengths)
         self._keep = []
+
+    # ---- programs on disk
+    def save(self, path):
+        """Write the built program to `path` (symlib_unit_save)."""
+        _native.check(_native.lib().symlib_unit_save(self._h, os.fsencode(path)), "symlib_unit_save")
+        return self
+
+    @classmethod
+    def load(cls, path, cpu_threads=0):
+        """The unit saved at `path`, without front-end or build (symlib_unit_load); cpu_threads > 0
+        for a program saved from the CPU device."""
+        L = _native.lib()
+        h = ctypes.c_void_p()
+        o = _opts(cpu_threads=cpu_threads)
+        _native.check(L.symlib_unit_load(ctypes.byref(h), os.fsencode(path), ctypes.byref(o)), "symlib_unit_load")
+        lengths = []
+        for i in range(L.symlib_unit_num_inputs(h)):
+            n = ctypes.c_uint64()
+            _native.check(L.symlib_unit_input_length(h, i, ctypes.byref(n)), "symlib_unit_input_length")
+            lengths.append(n.value)
+        return cls(h, lengths)
+
+    @classmethod
+    def cached(cls, directory, key, build):
+        """Unit of the program identified by `key` (bytes: a digest of everything the program
+        is built from) from `directory` when saved there before, else build() saved there."""
+        import hashlib
+
+        os.makedirs(directory, exist_ok=True)
+        path = os.path.join(directory, hashlib.sha256(key).hexdigest()[:32] + ".symprog")
+        if os.path.exists(path):
+            return cls.load(path)
+        u = build()
+        tmp = f"{path}.{os.getpid()}.tmp"
+        u.save(tmp)
+        os.replace(tmp, path)
+        return u
 
     # ---- constructors (symbolic front-ends of the reference examples)
     @classmethod
