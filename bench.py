@@ -479,6 +479,17 @@ def run_batched(args, dev, stream, sp):
     balg = sum(b_alg(c, 0) for c in counts) + 12 * n_out
     stats = batched.fused_stats()
     flops = fp64_flops(stats)
+    # the same evaluation from host arrays: execute() copies every cloth's positions over PCIe
+    # (blocking copies, as the reference's setArg), then launches; synchronised per step
+    in_bytes = 8 * sum(a.size for a in inputs)
+    batched.execute(*inputs)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for _ in range(5):
+        batched.execute(*inputs)
+        torch.cuda.synchronize()
+    pcie_ms = (time.perf_counter() - tp) * 1e3 / 5
+    log(f"[bench] with host inputs (H2D {in_bytes / 1e6:.0f} MB + launch): {pcie_ms:.3f} ms/step")
 
     # plain single-instance form, one program per distinct side, launched once per instance
     plain_line = None
@@ -561,6 +572,9 @@ def run_batched(args, dev, stream, sp):
                  "peak_tflops": FP64_PEAK_TFLOPS, "frac": flops / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                  "note": "operations of the emitted code: element evaluations x (adds + multiplies + divisions + "
                          "square roots per element) + one add per element contribution"},
+        "pcie_inclusive": {"value": n_out / (pcie_ms * 1e-3), "unit": "nnz/s", "ms_per_step": pcie_ms,
+                           "h2d_bytes": in_bytes,
+                           "note": "execute() from host arrays: positions copied to HBM every step, then the launch"},
         "plain_per_instance": plain_line,
         "speedup_vs_plain": (value / plain_line["value"]) if plain_line else None,
         "copy_bandwidth_gbs": copy_gbs, "torch_copy_gbs": torch_copy_gbs,
