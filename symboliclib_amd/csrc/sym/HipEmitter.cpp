@@ -1282,7 +1282,10 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 for (int q = 0; q < fp.maxPairs; ++q)
                     os << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " & 0xffffu));\n"
                        << "                s" << x << " += *(const double*)((const char*)V + (w" << q << x << " >> 16));\n";
-                os << "                if (act" << x << ") res[pos" << x << "] = s" << x << " + kc" << x << ";\n";
+                // outputs are not read again by the launch: stored without caching (whole lines:
+                // a wave's 64 consecutive outputs)
+                os << "                if (act" << x << ") __builtin_nontemporal_store(s" << x << " + kc" << x << ", res + pos" << x
+                   << ");\n";
             }
             os << "            }\n";
         } else {
