@@ -400,13 +400,18 @@ def test_fused_graph_replays_match_eager(require_device, split):
                 assert np.array_equal(u.get_results(k), eager[k]), f"eager launch after replay {rep} differs"
 
 
-@pytest.mark.parametrize("subdiv,schedule,lds", [(1, "static", 0), (5, "queue", 0), (5, "strided", 65536)])
-def test_ata_plus_a_product_tiles(require_device, subdiv, schedule, lds):
+@pytest.mark.parametrize("subdiv,schedule,lds,staging", [(1, "static", 0, False), (5, "queue", 0, False),
+                                                         (5, "strided", 65536, False), (2, "static", 0, True),
+                                                         (5, "queue", 0, True)])
+def test_ata_plus_a_product_tiles(require_device, subdiv, schedule, lds, staging):
+    """Product tiles staged by direct loads to LDS and, as the fallback, through registers."""
     n, cp, ri, a = W.c1_matrix(subdiv=subdiv)
-    with Unit.ata_plus_a(cp, ri, fused=True, schedule=schedule, tile_lds_bytes=lds) as u:
+    with Unit.ata_plus_a(cp, ri, fused=True, schedule=schedule, tile_lds_bytes=lds, register_staging=staging) as u:
         assert u.info()["tiled"] == 2
+        assert ("global_load_lds" in u.source()) == (not staging)
         u.compile().execute(a)
-        _check(f"A^T A + A icosphere({subdiv}) product tiles {schedule}", u, O.ata_plus_a(cp, ri, a), u.get_results())
+        _check(f"A^T A + A icosphere({subdiv}) product tiles {schedule} staging={'registers' if staging else 'direct'}",
+               u, O.ata_plus_a(cp, ri, a), u.get_results())
         a2 = W.random_values(len(ri), seed=77)
         u.execute(a2)
         _check(f"A^T A + A icosphere({subdiv}) product tiles second values", u, O.ata_plus_a(cp, ri, a2),
