@@ -19,8 +19,8 @@ inputs are already resident in HBM.  Prints one JSON line with the throughput, t
 of the dominant kernel (algorithmic bytes per launch / its average launch time, against the
 8 TB/s HBM peak and against the measured ceiling of a hand-written STREAM copy) and the CPU
 baseline: the reference's CPU path restated (the CPU device: the same programs as host C++
-with OpenMP, compiled with the reference's flags) on the host cores, over a size-stratified
-sample of the batch (--cpu-full: the whole batch).
+with OpenMP, compiled with the reference's flags) on the host cores, over the whole batch
+(--cpu-sample: 16 cloths of evenly spaced sizes).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c3|c2|c1] [--no-cpu]
 """
@@ -364,15 +364,15 @@ def fp64_flops(stats):
 
 def cpu_device_baseline(cloths, sides, args):
     """The reference's CPU path restated (CPU device, symcpu.h) on the host cores: one CPU batch
-    program over a size-stratified sample of the cloths (16 at evenly spaced size ranks, the
-    smallest and the largest included), or over all of them (--cpu-full); execute() (inputs
-    copied, every kernel run) repeated for about args.cpu_seconds after one warm-up."""
+    program over all the cloths, or (--cpu-sample) over 16 cloths at evenly spaced size ranks,
+    the smallest and the largest included; execute() (inputs copied, every kernel run) repeated
+    for about args.cpu_seconds after one warm-up."""
     from symboliclib_amd import workloads as W
     from symboliclib_amd.unit import Unit
 
     threads = max(1, min(16, os.cpu_count() or 1))
     order = np.argsort(sides, kind="stable")
-    if args.cpu_full:
+    if not args.cpu_sample:
         pick = list(range(len(cloths)))
     else:
         ranks = np.unique(np.linspace(0, len(order) - 1, min(16, len(order))).round().astype(int))
@@ -393,8 +393,8 @@ def cpu_device_baseline(cloths, sides, args):
             break
     tc = time.perf_counter() - ts
     u.close()
-    sample = (f"{'all' if args.cpu_full else len(pick)} of the {len(cloths)} cloths"
-              f"{'' if args.cpu_full else ' at evenly spaced size ranks'} (sides {min(sides[k] for k in pick)}.."
+    sample = (f"{'all' if not args.cpu_sample else len(pick)} of the {len(cloths)} cloths"
+              f"{' at evenly spaced size ranks' if args.cpu_sample else ''} (sides {min(sides[k] for k in pick)}.."
               f"{max(sides[k] for k in pick)}, {nnz} output nonzeros) as one program of the CPU device (the "
               f"reference's VecWidth/NumThreads path restated: host C++ from the same program, OpenMP over "
               f"{threads} threads, clang with the reference's flags); {n} execute() calls in {tc:.1f} s after a "
@@ -602,7 +602,8 @@ def main():
     ap.add_argument("--untuned", action="store_true", help="c4: default options, no selection")
     ap.add_argument("--committed", action="store_true", help="c4: the committed selection, no selection in the run")
     ap.add_argument("--trials", type=int, default=3, help="c4: configurations timed in the in-run selection")
-    ap.add_argument("--cpu-full", action="store_true", help="c4: CPU baseline over the whole batch")
+    ap.add_argument("--cpu-sample", action="store_true",
+                    help="c4: CPU baseline over 16 cloths at evenly spaced size ranks instead of the whole batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--form", default="waves", choices=["tiled", "waves", "fused", "direct"],
                     help="c1-c3 execution form (direct: waves form of the direct element assembly, c2/c3)")
