@@ -39,6 +39,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TUNED = os.path.join(ROOT, "symboliclib_amd", "tuning", "c4_selected.json")
+# committed fused-form selections of the single workloads (the cloth's is the batch's)
+SINGLE_TUNED = {"c2": os.path.join(ROOT, "symboliclib_amd", "tuning", "c2_selected.json"), "c3": TUNED}
 MODEL = os.path.join(ROOT, "symboliclib_amd", "tuning", "cost_model.txt")
 FP64_PEAK_TFLOPS = 78.6
 TRAFFIC = os.path.join(ROOT, "profiles", "symlib_fused_bw_hessian_batch256_traffic.json")
@@ -139,7 +141,12 @@ def run_single(args, dev, stream, sp):
     import oracle as O
 
     if args.form == "fused":
-        opt = fused_options(args)
+        sel = None
+        path = SINGLE_TUNED.get(args.workload)
+        if path and os.path.exists(path) and not args.untuned:
+            with open(path) as f:
+                sel = json.load(f)["selected"]
+        opt = fused_options(args, sel)
     else:
         opt = {"tiled": args.form == "tiled", "tile_lds_bytes": args.tile_lds or 0,
                "threads_per_block": args.tpb or 256}
@@ -599,7 +606,8 @@ def main():
     ap.add_argument("--count", type=int, default=256, help="c4: number of cloths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="c4: skip the plain per-instance baseline")
-    ap.add_argument("--untuned", action="store_true", help="c4: default options, no selection")
+    ap.add_argument("--untuned", action="store_true",
+                    help="default options: no selection (c4), no committed selection (c2, c3 fused)")
     ap.add_argument("--committed", action="store_true", help="c4: the committed selection, no selection in the run")
     ap.add_argument("--trials", type=int, default=3, help="c4: configurations timed in the in-run selection")
     ap.add_argument("--cpu-sample", action="store_true",

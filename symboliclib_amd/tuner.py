@@ -1,5 +1,7 @@
 """Autotuner of the fused assembly form: tile shape, output groups, register target, block size
 and schedule, chosen by exhaustive timing or by the learned cost model (costmodel.py).
+`python -m symboliclib_amd.tuner tune-cotan` searches the cotangent Laplacian's fused form
+exhaustively (its selection is committed as tuning/c2_selected.json).
 
 Search space (SPACE): tile LDS budget (the whole workgroup's LDS, which sets the tile shape:
 how many owner nodes a tile takes), output groups per element, minimum waves per SIMD, threads
@@ -363,6 +365,10 @@ def _main():
     s.add_argument("--count", type=int, default=256)
     s.add_argument("--top-k", type=int, default=3)
     s.add_argument("--no-exhaustive", action="store_true")
+    c2 = sub.add_parser("tune-cotan", help="exhaustive search of the cotangent Laplacian's fused form (512 x 512 grid)")
+    c2.add_argument("--out", required=True)
+    c2.add_argument("--side", type=int, default=512)
+    c2.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
 
@@ -441,6 +447,27 @@ def _main():
                        regret=regret(same["ms"] if same else ms, recs), regret_across_passes=regret(ms, recs),
                        exhaustive_seconds=time.perf_counter() - t0, exhaustive_trials=len(recs))
             log(f"[select] exhaustive best {best['config']}: {best['ms']:.4f} ms; regret {res['regret']:.3f}")
+        with open(args.out, "w") as f:
+            json.dump(res, f, indent=1)
+        print(json.dumps(res))
+
+
+    elif args.cmd == "tune-cotan":
+        from . import workloads as W
+        from .unit import Unit
+
+        V, F = W.c2_mesh(side=args.side)
+        # element matrices of 6 values: one output group is in the space
+        space = dict(SPACE, groups=[1] + SPACE["groups"])
+        t = Tuner(lambda lds: Unit.cotan(len(V), F, fused=True, tile_lds_bytes=lds), [W.to_colmajor(V)], space, log)
+        t0 = time.perf_counter()
+        recs = sorted(t.exhaustive(Timer(), reps=args.reps, record_features=False), key=lambda r: r["ms"])
+        res = {"workload": f"c2 (cotangent Laplacian, {args.side} x {args.side} grid)", "selected": recs[0]["config"],
+               "selected_ms": recs[0]["ms"], "by": "exhaustive search", "seconds": time.perf_counter() - t0,
+               "space": space, "configurations": len(recs),
+               "best": [{"config": r["config"], "ms": r["ms"]} for r in recs[:8]],
+               "median_ms": recs[len(recs) // 2]["ms"]}
+        log(f"[tune-cotan] best {recs[0]['config']}: {recs[0]['ms']:.4f} ms of {len(recs)} configurations")
         with open(args.out, "w") as f:
             json.dump(res, f, indent=1)
         print(json.dumps(res))
