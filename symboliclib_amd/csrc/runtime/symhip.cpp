@@ -59,9 +59,11 @@ std::string defaultArch() {
         return "gfx950";
     }
     int dev = 0;
-    hipGetDevice(&dev);
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "gfx950";
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return "gfx950";
+    }
     std::string a = prop.gcnArchName;
     auto colon = a.find(':');
     if (colon != std::string::npos) a = a.substr(0, colon);
@@ -181,13 +183,9 @@ void symhip_destroy(symhip_t h) {
     if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->module) (void)hipModuleUnload(h->module);
-    hipFree(h->x);
-    hipFree(h->result);
-    hipFree(h->c);
-    hipFree(h->p);
-    hipFree(h->o);
-    hipFree(h->tiles);
-    hipFree(h->work);
+    // a destroy cannot report errors: the buffers are released whatever the frees return
+    for (void* ptr : {(void*)h->x, (void*)h->result, (void*)h->c, (void*)h->p, (void*)h->o, (void*)h->tiles, (void*)h->work})
+        if (ptr) (void)hipFree(ptr);
     delete h;
 }
 
@@ -288,7 +286,7 @@ int symhip_compile(symhip_t h, const char* source, const char* options) {
     }
 
     if (h->module) {
-        hipModuleUnload(h->module);
+        (void)hipModuleUnload(h->module);
         h->module = nullptr;
     }
     h->functions.clear();
