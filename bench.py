@@ -65,6 +65,8 @@ def fused_options(args, tuned=None):
                  ("threads_per_block", args.tpb), ("schedule", args.schedule), ("split", args.split)):
         if v is not None:
             o[k] = v
+    if args.output_sums:
+        o["output_sums"] = True
     return o
 
 
@@ -623,6 +625,8 @@ def main():
     ap.add_argument("--schedule", default=None, choices=["static", "queue", "largest", "interleaved", "strided"])
     ap.add_argument("--split", type=int, default=None, choices=[0, 1],
                     help="fused: 1 = element images in HBM, then sums (two launches)")
+    ap.add_argument("--output-sums", action="store_true",
+                    help="fused: one contribution list per output instead of node-block sums")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -52,6 +52,8 @@ typedef struct {
     int split;                   /* fused cloth form: 1 = two launches (element images in HBM, then sums) */
     int register_staging;        /* fused forms: 1 = stage tiles through registers instead of
                                     direct global-to-LDS loads (default 0) */
+    int output_sums;             /* fused element tiles: one contribution list per output even
+                                    when the Hessian is made of node blocks (block sums otherwise) */
     int cpu_threads;             /* > 0: the reference's CPU device instead of UseHIP, with this
                                     many OpenMP threads (waves form only; symcpu.h); 0 = GPU */
 } symlib_options;

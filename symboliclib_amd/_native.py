@@ -20,7 +20,7 @@ class Options(ctypes.Structure):
                 ("tile_lds_bytes", ctypes.c_int), ("direct_assembly", ctypes.c_int), ("fused", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("output_groups", ctypes.c_int), ("waves_per_eu", ctypes.c_int),
                 ("profile_tiles", ctypes.c_int), ("split", ctypes.c_int),
-                ("register_staging", ctypes.c_int), ("cpu_threads", ctypes.c_int)]
+                ("register_staging", ctypes.c_int), ("output_sums", ctypes.c_int), ("cpu_threads", ctypes.c_int)]
 
 
 class UnitInfo(ctypes.Structure):

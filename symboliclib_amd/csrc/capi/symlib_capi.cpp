@@ -53,6 +53,7 @@ Sym::Device makeDevice(const symlib_options* opt) {
     d.profileTiles = o.profile_tiles != 0;
     d.fusedSplit = o.split != 0;
     d.ldsDma = o.register_staging == 0;
+    d.blockSums = o.output_sums == 0;
     return d;
 }
 
@@ -182,6 +183,7 @@ void symlib_default_options(symlib_options* opt) {
     opt->profile_tiles = 0;
     opt->split = 0;
     opt->register_staging = 0;
+    opt->output_sums = 0;
     opt->cpu_threads = 0;
 }
 

@@ -108,6 +108,10 @@ private:
         SparseMatrix<double> pattern;
     };
     void instancePattern(size_t k, InstancePattern& out) const;
+    // buildFused with block sums (blocks) or one contribution list per output; with blocks it
+    // throws when the Hessian is not block-structured
+    FusedProgram buildFusedForm(const BuildOptions& opt, const FusedOptions& fopt, ProgramInputs& io,
+                                std::vector<SparseMatrix<double>>& patterns, bool blocks) const;
     // the pattern an instance reports (its first dof block for node patterns)
     SparseMatrix<double> reportedPattern(size_t k, SparseMatrix<double>&& full) const;
     void inputLayout(ProgramInputs& io, std::vector<size_t>& inOff) const;

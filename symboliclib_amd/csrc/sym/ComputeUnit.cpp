@@ -95,6 +95,7 @@ void ComputeUnitBase::initFromAssembly(const HessianAssembly& assembly) {
         fo.profile = device.profileTiles;
         fo.split = device.fusedSplit;
         fo.ldsDma = device.ldsDma;
+        fo.blockSums = device.blockSums;
         fusedProgram = assembly.buildFused(opt, fo, io, assemblyPatterns);
         fusedOptions = fo;
         code = emitHipFused(fusedProgram, fo, launches, &fusedStats);

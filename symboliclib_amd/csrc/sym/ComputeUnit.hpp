@@ -68,6 +68,8 @@ using WavesPerEU = NamedType<int, struct WavesPerEUTag>;
 struct UseSplit {};
 // Fused form: stage tiles through registers instead of direct global-to-LDS loads.
 struct UseRegisterStaging {};
+// Fused form: one contribution list per output even when the Hessian is made of node blocks.
+struct UseOutputSums {};
 
 class Device {
 public:
@@ -88,6 +90,7 @@ public:
     bool profileTiles = false;
     bool fusedSplit = false;
     bool ldsDma = true;
+    bool blockSums = true;
 
 private:
     void set(const DecompositionThreshold& nt) { decompositionThreshold = (unsigned)nt.x; }
@@ -108,6 +111,7 @@ private:
     void set(const WavesPerEU& w) { wavesPerEU = w.x; }
     void set(const UseSplit&) { fusedSplit = true; }
     void set(const UseRegisterStaging&) { ldsDma = false; }
+    void set(const UseOutputSums&) { blockSums = false; }
     void set(const UseHIP&) {
         hipDevice = true;
         cudaDevice = false;

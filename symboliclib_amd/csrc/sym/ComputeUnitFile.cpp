@@ -29,6 +29,7 @@ void writeDevice(BinWriter& w, const Device& d) {
     w.pod<uint8_t>(d.profileTiles);
     w.pod<uint8_t>(d.fusedSplit);
     w.pod<uint8_t>(d.ldsDma);
+    w.pod<uint8_t>(d.blockSums);
 }
 
 void readDevice(BinReader& r, Device& d) {
@@ -50,6 +51,7 @@ void readDevice(BinReader& r, Device& d) {
     d.profileTiles = r.pod<uint8_t>() != 0;
     d.fusedSplit = r.pod<uint8_t>() != 0;
     d.ldsDma = r.pod<uint8_t>() != 0;
+    d.blockSums = r.pod<uint8_t>() != 0;
 }
 
 void writeSizes(BinWriter& w, const std::vector<size_t>& v) {
@@ -160,10 +162,13 @@ void ComputeUnitBase::save(const std::string& path) const {
         w.pod<uint64_t>(v);
     w.pod<int32_t>(F.stagedDofs);
     w.pod<int32_t>(F.maxPairs);
+    w.pod<int32_t>((int)F.sums);
+    w.pod<int32_t>(F.blockDofs);
+    w.pod<uint64_t>(F.maxBlocks);
     const FusedOptions& O = fusedOptions;
     for (size_t v : {O.ldsBytes, O.maxOutputs}) w.pod<uint64_t>(v);
     for (int v : {O.threadsPerBlock, (int)O.schedule, O.outputGroups, O.wavesPerEU}) w.pod<int32_t>(v);
-    for (bool v : {O.profile, O.shareReciprocals, O.foldCoefficients, O.split, O.ldsDma}) w.pod<uint8_t>(v);
+    for (bool v : {O.profile, O.shareReciprocals, O.foldCoefficients, O.split, O.ldsDma, O.blockSums}) w.pod<uint8_t>(v);
     const FusedCodeStats& S = fusedStats;
     for (size_t v : {S.adds, S.muls, S.divs, S.sqrts, S.statements}) w.pod<uint64_t>(v);
     w.pod<int32_t>(S.maxLive);
@@ -279,6 +284,11 @@ void ComputeUnitBase::initFromFile(const std::string& path) {
         *v = r.pod<uint64_t>();
     F.stagedDofs = r.pod<int32_t>();
     F.maxPairs = r.pod<int32_t>();
+    const int sums = r.pod<int32_t>();
+    if (sums != (int)FusedSums::Outputs && sums != (int)FusedSums::Blocks) r.fail("unknown sum form");
+    F.sums = (FusedSums)sums;
+    F.blockDofs = r.pod<int32_t>();
+    F.maxBlocks = r.pod<uint64_t>();
     FusedOptions& O = fusedOptions;
     O.ldsBytes = r.pod<uint64_t>();
     O.maxOutputs = r.pod<uint64_t>();
@@ -288,7 +298,8 @@ void ComputeUnitBase::initFromFile(const std::string& path) {
     O.schedule = (FusedSchedule)sched;
     O.outputGroups = r.pod<int32_t>();
     O.wavesPerEU = r.pod<int32_t>();
-    for (bool* v : {&O.profile, &O.shareReciprocals, &O.foldCoefficients, &O.split, &O.ldsDma}) *v = r.pod<uint8_t>() != 0;
+    for (bool* v : {&O.profile, &O.shareReciprocals, &O.foldCoefficients, &O.split, &O.ldsDma, &O.blockSums})
+        *v = r.pod<uint8_t>() != 0;
     FusedCodeStats& S = fusedStats;
     for (size_t* v : {&S.adds, &S.muls, &S.divs, &S.sqrts, &S.statements}) *v = r.pod<uint64_t>();
     S.maxLive = r.pod<int32_t>();
@@ -313,6 +324,14 @@ void ComputeUnitBase::initFromFile(const std::string& path) {
     char tail[8];
     r.raw(tail, 8);
     if (std::string(tail, 8) != std::string(PROGRAM_FILE_MAGIC, 8)) r.fail("bad end marker");
+    // every index of a tile pool against its buffers, as after a build
+    if (fusedForm) {
+        try {
+            validateFused(F);
+        } catch (const std::logic_error& e) {
+            r.fail(e.what());
+        }
+    }
 }
 
 }  // namespace Sym

@@ -36,6 +36,11 @@ struct TemplateLayout {
     std::vector<char> uniform;
     std::vector<int> gatherNode, gatherDof;
     int numNodes = 0, numParams = 0, numOutputs = 0;
+    // block sums: the differentiated nodes (layout node of each, first-use order), the value
+    // slot of node pair (p, q), p <= q, at blockBase[p * diffNode.size() + q], and the template
+    // output held by each value slot
+    std::vector<int> diffNode, blockBase, slotOutput;
+    std::vector<int> diffArg;  // block sums: an argument of each differentiated node
     int valueStride() const { return numOutputs | 1; }
     // LDS bytes of one element besides its nodes' coordinates: parameters, values, slots
     size_t elementBytes() const { return 8 * (size_t)(numParams + valueStride()) + 2 * (size_t)numNodes; }
@@ -50,7 +55,57 @@ struct InstanceTiles {
     std::vector<size_t> values;  // per tile: element values (its image in the split form)
     size_t evaluations = 0, contributions = 0, lds = 0, elementLds = 0, stagedLds = 0, outputs = 0;
     int maxPairs = 0;  // the most slot pairs of an output
+    size_t maxBlocks = 0;
 };
+
+// The Hessian of an instance is not made of node blocks (or exceeds the block records'
+// fields): the program is built with one contribution list per output instead.
+struct NotBlockStructured : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Block value layout of a template over the differentiated arguments 0..nd-1 of `t` for D
+// dofs per node; false when the arguments are not every dof 0..D-1 of their nodes, each once.
+bool blockLayout(TemplateLayout& tl, const ExprTemplate& t, int D) {
+    const int nd = t.numDiffVars;
+    if (nd <= 0 || (int)t.entryToOutput.size() != nd * nd) return false;
+    std::vector<int> nodeIdx(nd), dofOf(nd);
+    tl.diffNode.clear();
+    tl.diffArg.clear();
+    for (int i = 0; i < nd; ++i) {
+        const int g = i < (int)tl.argGather.size() ? tl.argGather[i] : -1;
+        if (g < 0) return false;
+        const int j = tl.gatherNode[g];
+        int p = (int)(std::find(tl.diffNode.begin(), tl.diffNode.end(), j) - tl.diffNode.begin());
+        if (p == (int)tl.diffNode.size()) {
+            tl.diffNode.push_back(j);
+            tl.diffArg.push_back(i);
+        }
+        nodeIdx[i] = p;
+        dofOf[i] = tl.gatherDof[g];
+    }
+    const int n = (int)tl.diffNode.size();
+    if (n * D != nd) return false;
+    std::vector<int> argOf(nd, -1);  // (p, d) -> argument
+    for (int i = 0; i < nd; ++i) {
+        if (dofOf[i] < 0 || dofOf[i] >= D || argOf[nodeIdx[i] * D + dofOf[i]] >= 0) return false;
+        argOf[nodeIdx[i] * D + dofOf[i]] = i;
+    }
+    tl.blockBase.assign((size_t)n * n, -1);
+    tl.slotOutput.clear();
+    for (int p = 0; p < n; ++p)
+        for (int q = p; q < n; ++q) {
+            tl.blockBase[(size_t)p * n + q] = (int)tl.slotOutput.size();
+            for (int da = 0; da < D; ++da)
+                for (int db = p == q ? da : 0; db < D; ++db) {
+                    const int i = argOf[p * D + da], j = argOf[q * D + db];
+                    const int o = t.entryToOutput[(size_t)i * nd + j];
+                    if (o < 0) return false;
+                    tl.slotOutput.push_back(o);
+                }
+        }
+    return true;
+}
 
 }  // namespace
 
@@ -64,6 +119,17 @@ std::string FusedProgram::summary() const {
 
 FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOptions& fo, ProgramInputs& io,
                                          std::vector<SparseMatrix<double>>& patterns) const {
+    if (fo.blockSums) {
+        try {
+            return buildFusedForm(opt, fo, io, patterns, true);
+        } catch (const NotBlockStructured&) {
+        }
+    }
+    return buildFusedForm(opt, fo, io, patterns, false);
+}
+
+FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const FusedOptions& fo, ProgramInputs& io,
+                                             std::vector<SparseMatrix<double>>& patterns, bool blocks) const {
     PhaseTimer timer;
     const int T = fo.threadsPerBlock;
     if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("fused form: threads per block must be a multiple of 64 up to 1024");
@@ -134,6 +200,31 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             }
         }
     }
+    // ---- block sums: every instance's Hessian made of D x D node blocks (D = dofs, 1 for a
+    // node pattern): no constants besides nodal blocks, every template's differentiated
+    // arguments all dofs of its nodes, the nodes of an element distinct
+    int Db = 0;
+    if (blocks) {
+        for (auto& I : inst_) {
+            const int d = I.nodePattern ? 1 : I.dofs;
+            if (!I.cRow.empty() || (Db && d != Db)) throw NotBlockStructured("constants or dofs");
+            Db = d;
+        }
+        if (Db < 1 || Db > 3) throw NotBlockStructured("dofs per node");
+        for (auto& tl : L)
+            if (!blockLayout(tl, getTemplate(tl.tmplId), Db)) throw NotBlockStructured("element template");
+        for (size_t e = 0; e < elemTmpl_.size(); ++e) {
+            const TemplateLayout& tl = L[(size_t)elemT[e]];
+            const Instance& I = inst_[elemInst[e]];
+            const int N = I.numVars / I.dofs;
+            const Arg* a = &args_[elemArg_[e]];
+            const size_t n = tl.diffNode.size();
+            for (size_t p = 0; p < n; ++p)
+                for (size_t q = 0; q < p; ++q)
+                    if (a[tl.diffArg[p]].id % N == a[tl.diffArg[q]].id % N) throw NotBlockStructured("repeated element node");
+        }
+        for (auto& tl : L) tl.numOutputs = (int)tl.slotOutput.size();
+    }
     for (auto& tl : L) {
         const ExprTemplate& t = getTemplate(tl.tmplId);
         std::vector<Symbolic> argRef((size_t)t.numArgs);
@@ -146,8 +237,12 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             }
         }
         std::vector<Symbolic> assigns;
-        for (int s = 0; s < t.numOutputs; ++s)
-            assigns.emplace_back(ASSIGN, Symbolic(s, -2), substituteArguments(t.outputs[s], argRef));
+        if (blocks)
+            for (size_t s = 0; s < tl.slotOutput.size(); ++s)
+                assigns.emplace_back(ASSIGN, Symbolic((int)s, -2), substituteArguments(t.outputs[(size_t)tl.slotOutput[s]], argRef));
+        else
+            for (int s = 0; s < t.numOutputs; ++s)
+                assigns.emplace_back(ASSIGN, Symbolic(s, -2), substituteArguments(t.outputs[s], argRef));
         FusedTemplate ft;
         ft.code = optimizeTemplate(Symbolic(BLOCK, assigns), opt.simplifyTemplates);
         ft.numNodes = tl.numNodes;
@@ -158,6 +253,10 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         ft.valueStride = tl.valueStride();
         for (int d : tl.gatherDof) fp.stagedDofs = std::max(fp.stagedDofs, d + 1);
         fp.templates.push_back(std::move(ft));
+    }
+    if (blocks) {
+        fp.sums = FusedSums::Blocks;
+        fp.blockDofs = Db;
     }
     const size_t NT = L.size();
     const size_t nodeLds = 8 * (size_t)fp.stagedDofs;  // LDS bytes per staged tile node
@@ -227,6 +326,28 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             }
             nodeBytes[b] = bytes + outCount[b] * (1.0 + 4.0 * FUSED_CHUNK_WORDS / 64.0);
         }
+        // block sums: a node adds its runs, node record, blocks (record, chunk record share,
+        // constant, list entries plus one for odd chunk lengths) and output chunk records; the
+        // lists of a tile are padded to each chunk's longest, at most 64 entries of the longest
+        // list per block kind (listMargin)
+        double listMargin = 0.0;
+        std::vector<uint32_t> nodeBlocks(blocks ? (size_t)N : 0, 0);
+        if (blocks) {
+            uint32_t longest = 0;
+            for (int b = 0; b < N; ++b) {
+                const int c = b;
+                double nblk = 0.0, entries = 0.0;
+                for (int q = colptr[c]; q < colptr[c + 1] && P.pattern.inner()[q] < N; ++q) {
+                    nblk += 1.0;
+                    entries += (double)P.count[q];
+                    longest = std::max(longest, P.count[q]);
+                }
+                nodeBytes[b] = 8.0 * Db + 8.0 + outCount[b] * (12.0 / 64.0) + nblk * (4.0 + 8.0 / 64.0 + 8.0 + 2.0) +
+                               2.0 * entries;
+                nodeBlocks[b] = (uint32_t)nblk;
+            }
+            listMargin = 2.0 * 128.0 * (double)longest + 3.0 * 256.0;
+        }
 
         std::vector<char> assigned((size_t)N, 0);
         for (int b = 0; b < N; ++b) assigned[b] = outCount[b] == 0;
@@ -263,7 +384,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             std::fill(heads.begin(), heads.end(), 0);
             tNodes.clear();
             for (auto& te : tElems) te.clear();
-            size_t elemBytes = 0, outs = 0, valueBytes = 0;
+            size_t elemBytes = 0, outs = 0, valueBytes = 0, tileBlocks = 0;
             double recipeBytes = 0.0;
             tileNodeList.clear();
             push((uint32_t)cursor, cost((uint32_t)cursor, tileId));
@@ -310,14 +431,20 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                         newNodes.push_back(n);
                     }
                 }
-                const double need = (double)(elemBytes + newBytes + nodeLds * (tileNodeList.size() + newNodes.size()));
+                double need = (double)(elemBytes + newBytes + nodeLds * (tileNodeList.size() + newNodes.size()));
+                // block sums: the tile's outputs overlay its values and their zeros when more
+                if (blocks)
+                    need += listMargin + std::max(0.0, 8.0 * (double)(outs + outCount[v]) -
+                                                           (double)(valueBytes + newValueBytes + 8 * (size_t)Db * Db));
                 // element values and the zero after them within 64 KiB: slots are u16 byte offsets
                 const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget && valueBytes + newValueBytes + 8 <= 65536;
-                if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs)) break;
+                const bool blockCap = blocks && tileBlocks + nodeBlocks[v] > FUSED_MAX_BLOCKS;
+                if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs || blockCap)) break;
                 if (!fits) throw std::invalid_argument("fused form: one node and its elements need more LDS than a tile holds");
                 assigned[v] = 1;
                 tNodes.push_back(v);
                 outs += outCount[v];
+                if (blocks) tileBlocks += nodeBlocks[v];
                 elemBytes += newBytes;
                 valueBytes += newValueBytes;
                 recipeBytes += nodeBytes[v];
@@ -349,66 +476,6 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
             // slot ldsBase[NT] is the tile's zero (padding of odd contribution lists); slots are
             // stored as LDS byte offsets (8 x index), so the values and the zero fit in 64 KiB
             if (8 * ldsBase[NT] >= 65536) throw std::logic_error("fused form: tile values exceed 64 KiB of byte offsets");
-            std::vector<uint32_t> runs;  // (position, first output)
-            std::vector<uint8_t> counts;
-            std::vector<uint16_t> slotList;
-            std::vector<double> cvals;
-            bool anyConst = false;
-            size_t m = 0;
-            uint32_t prevEnd = NONE;
-            for (int d = 0; d < D; ++d)
-                for (uint32_t b : tNodes) {
-                    const int c = (int)b + d * N;
-                    for (int q = colptr[c]; q < colptr[c + 1]; ++q, ++m) {
-                        if ((uint32_t)q != prevEnd) {
-                            runs.push_back((uint32_t)q);
-                            runs.push_back((uint32_t)m);
-                        }
-                        prevEnd = (uint32_t)q + 1;
-                        const uint32_t cnt = P.count[q];
-                        if (cnt > 126) throw std::invalid_argument("fused form: an output has more than 126 element contributions");
-                        // lists padded to even length with the zero slot (read in pairs)
-                        counts.push_back((uint8_t)((cnt + (cnt & 1u)) | (P.hasConst[q] ? 0x80u : 0u)));
-                        out.maxPairs = std::max(out.maxPairs, (int)((cnt + 1) / 2));
-                        for (uint32_t r = 0; r < cnt; ++r) {
-                            const uint64_t s = P.contrib[P.first[q] + r];
-                            const size_t i = (size_t)(s >> 16) - e0;
-                            const size_t slot = (size_t)(s & 0xffff);
-                            if (stamp[i] != tileId) throw std::logic_error("fused form: contribution from an element outside the tile");
-                            const size_t t = (size_t)elemT[e0 + i];
-                            slotList.push_back((uint16_t)(8 * (ldsBase[t] + (size_t)local[i] * (size_t)L[t].valueStride() + slot)));
-                        }
-                        if (cnt & 1u) slotList.push_back((uint16_t)(8 * ldsBase[NT]));
-                        if (P.hasConst[q]) {
-                            cvals.push_back(P.cval[q]);
-                            anyConst = true;
-                        }
-                    }
-                }
-            runs.push_back(0);
-            runs.push_back((uint32_t)m);
-            const size_t nch = (m + 63) / 64;
-            const size_t CW = FUSED_CHUNK_WORDS;
-            std::vector<uint32_t> chunks(CW * nch);
-            {
-                size_t sb = 0, cb = 0, r = 0;
-                for (size_t ch = 0; ch < nch; ++ch) {
-                    const size_t o0 = ch * 64;
-                    while (runs[2 * (r + 1) + 1] <= o0) ++r;
-                    chunks[CW * ch] = (uint32_t)sb;
-                    chunks[CW * ch + 1] = (uint32_t)r;
-                    chunks[CW * ch + 2] = (uint32_t)cb;
-                    uint64_t mask = 0;
-                    for (size_t q = r + 1; runs[2 * q + 1] < std::min(m, o0 + 64); ++q) mask |= 1ull << (runs[2 * q + 1] - o0);
-                    chunks[CW * ch + 3] = (uint32_t)mask;
-                    chunks[CW * ch + 4] = (uint32_t)(mask >> 32);
-                    for (size_t o = o0; o < std::min(m, o0 + 64); ++o) {
-                        sb += counts[o] & 0x7fu;
-                        cb += counts[o] >> 7;
-                    }
-                }
-            }
-
             std::vector<uint32_t>& pool = out.pool;
             const size_t H = pool.size();
             out.header.push_back(H);
@@ -420,48 +487,299 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 if (bytes) std::memcpy(&pool[H + off], data, bytes);
                 return u32(off, "tile size");
             };
-            const uint32_t runsOff = appendWords(runs.data(), runs.size() * 4);
-            const uint32_t countsOff = appendWords(counts.data(), counts.size());
-            const uint32_t slotsOff = appendWords(slotList.data(), slotList.size() * 2);
-            const uint32_t chunksOff = appendWords(chunks.data(), chunks.size() * 4);
-            pool[H + 0] = u32(inOff[k], "input base");
-            pool[H + 1] = (uint32_t)N;
-            pool[H + 2] = u32(m, "outputs per tile");
-            pool[H + 3] = (uint32_t)nch;
-            pool[H + 4] = runsOff;
-            pool[H + 5] = countsOff;
-            pool[H + 6] = slotsOff;
-            pool[H + 7] = chunksOff;
-            // periodic constants: the outputs split into D equal dof blocks (columns b + d N
-            // for d = 0..D-1) and every block holds the same constant values in the same
-            // order; then one block's constants are stored and H[11] carries the period
-            uint32_t period = 0;
-            size_t stored = cvals.size();
-            if (D > 1 && D < 16 && m % (size_t)D == 0 && cvals.size() % (size_t)D == 0 && !cvals.empty()) {
-                const size_t per = cvals.size() / (size_t)D, blk = m / (size_t)D;
-                bool ok = true;
-                size_t k = 0;
-                for (int d = 0; d < D && ok; ++d) {
-                    size_t inBlock = 0;
-                    for (size_t o = (size_t)d * blk; o < (size_t)(d + 1) * blk; ++o) inBlock += counts[o] >> 7;
-                    ok = inBlock == per;
-                    for (size_t j = 0; j < per && ok; ++j, ++k) ok = cvals[k] == cvals[j];
+            size_t m = 0, stored = 0, contrib = 0, valueRegion = ldsBase[NT] + 1;
+            if (blocks) {
+                // ---- block sums: outputs in run order (dof-major columns of the owned nodes)
+                const size_t nvals = ldsBase[NT], Z = (size_t)Db * Db, nOwned = tNodes.size();
+                if (nvals + Z > 0x7fff) throw NotBlockStructured("tile values exceed 15-bit value indices");
+                if (nOwned > 0x3fff) throw NotBlockStructured("more than 16383 owned nodes in a tile");
+                const uint16_t zeroEntry = (uint16_t)nvals;
+                std::vector<uint32_t> runs;  // (position, first output)
+                std::vector<uint32_t> colStart((size_t)Db * nOwned, 0);
+                uint32_t prevEnd = NONE;
+                for (int d = 0; d < D; ++d)
+                    for (size_t ni = 0; ni < nOwned; ++ni) {
+                        const int c = (int)tNodes[ni] + d * N;
+                        if (d >= Db) {
+                            if (colptr[c + 1] != colptr[c]) throw NotBlockStructured("output column beyond the block dofs");
+                            continue;
+                        }
+                        colStart[(size_t)d * nOwned + ni] = (uint32_t)m;
+                        for (int q = colptr[c]; q < colptr[c + 1]; ++q, ++m) {
+                            if ((uint32_t)q != prevEnd) {
+                                runs.push_back((uint32_t)q);
+                                runs.push_back((uint32_t)m);
+                            }
+                            prevEnd = (uint32_t)q + 1;
+                        }
+                    }
+                runs.push_back(0);
+                runs.push_back((uint32_t)m);
+                if (m > 65536) throw std::logic_error("fused form: more than 65536 outputs in a tile");
+
+                // blocks of every owned node b: list entries from its elements, E(b) (nodes
+                // sharing an element), K(b) (the other dof-0 rows of column (b, 0)); the pattern's
+                // columns, counts and constants are checked against the block structure
+                struct Blk {
+                    uint32_t ni, kind, rE, rEK, first, len;
+                    bool hasConst;
+                    double cval;
+                };
+                std::vector<Blk> blks;
+                std::vector<uint16_t> ents;
+                std::vector<uint32_t> nodeRec(2 * nOwned, 0);
+                std::vector<std::pair<uint32_t, uint16_t>> pe;  // (node a, entry) of node b's elements
+                std::vector<uint32_t> Eb, EK;
+                const std::vector<int>& inner = P.pattern.inner();
+                for (size_t ni = 0; ni < nOwned; ++ni) {
+                    const uint32_t b = tNodes[ni];
+                    pe.clear();
+                    for (uint32_t qi = incStart[b]; qi < incStart[b + 1]; ++qi) {
+                        const uint32_t i = inc[qi];
+                        if (stamp[i] != tileId) throw std::logic_error("fused form: element of an owned node outside the tile");
+                        const size_t t = (size_t)elemT[e0 + i];
+                        const TemplateLayout& tl = L[t];
+                        const size_t n = tl.diffNode.size();
+                        size_t qb = n;
+                        for (size_t p = 0; p < n; ++p)
+                            if (en[enStart[i] + (uint32_t)tl.diffNode[p]] == b) qb = p;
+                        if (qb == n) continue;  // b only read, not differentiated
+                        const size_t vbase = ldsBase[t] + (size_t)local[i] * (size_t)tl.valueStride();
+                        for (size_t p = 0; p < n; ++p) {
+                            const size_t lo = std::min(p, qb), hi = std::max(p, qb);
+                            const size_t v = vbase + (size_t)tl.blockBase[lo * n + hi];
+                            pe.emplace_back(en[enStart[i] + (uint32_t)tl.diffNode[p]], (uint16_t)(v | (p > qb ? 0x8000u : 0u)));
+                        }
+                    }
+                    std::stable_sort(pe.begin(), pe.end(), [](const std::pair<uint32_t, uint16_t>& x, const std::pair<uint32_t, uint16_t>& y) {
+                        return x.first < y.first;
+                    });
+                    Eb.clear();
+                    for (size_t j = 0; j < pe.size(); ++j)
+                        if (j == 0 || pe[j].first != pe[j - 1].first) Eb.push_back(pe[j].first);
+                    EK.clear();
+                    for (int q = colptr[b]; q < colptr[b + 1] && inner[q] < N; ++q) EK.push_back((uint32_t)inner[q]);
+                    const size_t nE = Eb.size();
+                    if (EK.size() < nE || !std::includes(EK.begin(), EK.end(), Eb.begin(), Eb.end()))
+                        throw NotBlockStructured("element pair without an output");
+                    const size_t nK = EK.size() - nE;
+                    if (EK.size() > 127) throw NotBlockStructured("node with more than 127 blocks");
+                    // column (b, db): segment da holds E(b), and K(b) too when da == db
+                    for (int db = 0; db < Db; ++db) {
+                        const int c = (int)b + db * N;
+                        int q = colptr[c];
+                        for (int da = 0; da < Db; ++da)
+                            for (uint32_t a : (da == db ? EK : Eb)) {
+                                if (q >= colptr[c + 1] || inner[q] != (int)a + da * N) throw NotBlockStructured("output pattern");
+                                ++q;
+                            }
+                        if (q != colptr[c + 1]) throw NotBlockStructured("output pattern");
+                    }
+                    nodeRec[2 * ni] = colStart[ni] | (Db > 1 ? colStart[nOwned + ni] << 16 : 0u);
+                    nodeRec[2 * ni + 1] = (Db > 2 ? colStart[2 * nOwned + ni] : 0u) | (uint32_t)nE << 16 | (uint32_t)nK << 24;
+                    size_t je = 0, jp = 0;
+                    for (size_t r = 0; r < EK.size(); ++r) {
+                        const uint32_t a = EK[r];
+                        Blk B;
+                        B.ni = (uint32_t)ni;
+                        B.rEK = (uint32_t)r;
+                        const bool inE = je < nE && Eb[je] == a;
+                        B.rE = inE ? (uint32_t)je : 0u;
+                        B.kind = inE ? (a == b ? 0u : 1u) : 2u;
+                        B.first = (uint32_t)ents.size();
+                        if (inE) {
+                            ++je;
+                            while (jp < pe.size() && pe[jp].first == a) ents.push_back(pe[jp++].second);
+                        }
+                        B.len = (uint32_t)(ents.size() - B.first);
+                        // every output of the block: its contribution count and constant
+                        const int q00 = colptr[b] + (int)r;  // entry (0, 0)
+                        B.hasConst = P.hasConst[(size_t)q00] != 0;
+                        B.cval = P.cval[(size_t)q00];
+                        for (int db = 0; db < Db; ++db)
+                            for (int da = 0; da < Db; ++da) {
+                                if (!inE && da != db) continue;
+                                const size_t q = (size_t)colptr[(int)b + db * N] + (size_t)da * nE + (db < da ? nK : 0) +
+                                                 (da == db ? r : B.rE);
+                                const bool hc = P.hasConst[q] != 0;
+                                if (P.count[q] != B.len || hc != (da == db && B.hasConst) || (hc && P.cval[q] != B.cval))
+                                    throw NotBlockStructured("output contributions");
+                            }
+                        if (B.len > 255) throw NotBlockStructured("block list longer than 255");
+                        blks.push_back(B);
+                    }
                 }
-                if (ok) {
-                    period = (uint32_t)D;
-                    stored = per;
+                // blocks by kind, longest lists first: a chunk's lanes have near-equal lists
+                const size_t nb = blks.size(), nbch = (nb + 63) / 64, noch = (m + 63) / 64;
+                std::vector<uint32_t> order(nb);
+                std::iota(order.begin(), order.end(), 0u);
+                std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+                    return blks[x].kind != blks[y].kind ? blks[x].kind < blks[y].kind : blks[x].len > blks[y].len;
+                });
+                std::vector<uint32_t> blockRec(nb), blockChunk(2 * nbch), lists;
+                std::vector<double> cvals;
+                for (size_t ch = 0; ch < nbch; ++ch) {
+                    uint32_t Lc = 0;
+                    for (size_t bi = ch * 64; bi < std::min(nb, ch * 64 + 64); ++bi) Lc = std::max(Lc, blks[order[bi]].len);
+                    const size_t pairs = (Lc + 1) / 2, lb = lists.size();
+                    if (lb >= (1u << 24)) throw NotBlockStructured("block lists exceed 24-bit offsets");
+                    blockChunk[2 * ch] = (uint32_t)lb | Lc << 24;
+                    blockChunk[2 * ch + 1] = (uint32_t)cvals.size();
+                    lists.resize(lb + 64 * pairs, 0);
+                    for (size_t l = 0; l < 64; ++l) {
+                        const size_t bi = ch * 64 + l;
+                        const Blk* B = bi < nb ? &blks[order[bi]] : nullptr;
+                        auto entry = [&](size_t j) -> uint32_t { return B && j < B->len ? ents[B->first + j] : zeroEntry; };
+                        for (size_t j = 0; j < pairs; ++j) lists[lb + 64 * j + l] = entry(2 * j) | entry(2 * j + 1) << 16;
+                        if (!B) continue;
+                        blockRec[bi] = B->ni | B->kind << 14 | B->rE << 16 | B->rEK << 23 | (B->hasConst ? 1u << 30 : 0u);
+                        if (B->hasConst) cvals.push_back(B->cval);
+                        if (B->kind != 2) contrib += (size_t)B->len * Z;
+                    }
                 }
-            }
-            if (anyConst) {
-                pool[H + 8] = u32(out.consts.size(), "constants");
-                out.consts.insert(out.consts.end(), cvals.begin(), cvals.begin() + (long)stored);
+                // output chunk records: the run of each chunk's first output and the run starts in it
+                std::vector<uint32_t> outChunk(3 * noch);
+                for (size_t ch = 0, r = 0; ch < noch; ++ch) {
+                    const size_t o0 = ch * 64;
+                    while (runs[2 * (r + 1) + 1] <= o0) ++r;
+                    uint64_t mask = 0;
+                    for (size_t q = r + 1; runs[2 * q + 1] < std::min(m, o0 + 64); ++q) mask |= 1ull << (runs[2 * q + 1] - o0);
+                    outChunk[3 * ch] = (uint32_t)r;
+                    outChunk[3 * ch + 1] = (uint32_t)mask;
+                    outChunk[3 * ch + 2] = (uint32_t)(mask >> 32);
+                }
+                const uint32_t recipeOff = appendWords(outChunk.data(), outChunk.size() * 4);
+                appendWords(blockChunk.data(), blockChunk.size() * 4);
+                appendWords(blockRec.data(), blockRec.size() * 4);
+                const uint32_t nodeOff = appendWords(nodeRec.data(), nodeRec.size() * 4);
+                const uint32_t runsOff = appendWords(runs.data(), runs.size() * 4);
+                const uint32_t listOff = appendWords(lists.data(), lists.size() * 4);
+                pool[H + 0] = u32(inOff[k], "input base");
+                pool[H + 1] = (uint32_t)N;
+                pool[H + 2] = (uint32_t)m;
+                pool[H + 3] = (uint32_t)nb;
+                pool[H + 4] = recipeOff;
+                pool[H + 5] = runsOff;
+                pool[H + 6] = listOff;
+                pool[H + 7] = nodeOff;
+                if (!cvals.empty()) {
+                    pool[H + 8] = u32(out.consts.size(), "constants");
+                    out.consts.insert(out.consts.end(), cvals.begin(), cvals.end());
+                } else {
+                    pool[H + 8] = NONE;
+                }
+                pool[H + 9] = 0;  // result base, set at merge
+                pool[H + 10] = listOff + (uint32_t)lists.size() - recipeOff;
+                stored = cvals.size();
+                if (stored >= (1u << 28)) throw std::length_error("fused form: too many output constants in a tile");
+                pool[H + 11] = (uint32_t)stored;
+                valueRegion = std::max(nvals + Z, m);
+                out.maxBlocks = std::max(out.maxBlocks, nb);
             } else {
-                pool[H + 8] = NONE;
+                std::vector<uint32_t> runs;  // (position, first output)
+                std::vector<uint8_t> counts;
+                std::vector<uint16_t> slotList;
+                std::vector<double> cvals;
+                bool anyConst = false;
+                m = 0;
+                uint32_t prevEnd = NONE;
+                for (int d = 0; d < D; ++d)
+                    for (uint32_t b : tNodes) {
+                        const int c = (int)b + d * N;
+                        for (int q = colptr[c]; q < colptr[c + 1]; ++q, ++m) {
+                            if ((uint32_t)q != prevEnd) {
+                                runs.push_back((uint32_t)q);
+                                runs.push_back((uint32_t)m);
+                            }
+                            prevEnd = (uint32_t)q + 1;
+                            const uint32_t cnt = P.count[q];
+                            if (cnt > 126) throw std::invalid_argument("fused form: an output has more than 126 element contributions");
+                            // lists padded to even length with the zero slot (read in pairs)
+                            counts.push_back((uint8_t)((cnt + (cnt & 1u)) | (P.hasConst[q] ? 0x80u : 0u)));
+                            out.maxPairs = std::max(out.maxPairs, (int)((cnt + 1) / 2));
+                            for (uint32_t r = 0; r < cnt; ++r) {
+                                const uint64_t s = P.contrib[P.first[q] + r];
+                                const size_t i = (size_t)(s >> 16) - e0;
+                                const size_t slot = (size_t)(s & 0xffff);
+                                if (stamp[i] != tileId) throw std::logic_error("fused form: contribution from an element outside the tile");
+                                const size_t t = (size_t)elemT[e0 + i];
+                                slotList.push_back((uint16_t)(8 * (ldsBase[t] + (size_t)local[i] * (size_t)L[t].valueStride() + slot)));
+                            }
+                            if (cnt & 1u) slotList.push_back((uint16_t)(8 * ldsBase[NT]));
+                            if (P.hasConst[q]) {
+                                cvals.push_back(P.cval[q]);
+                                anyConst = true;
+                            }
+                        }
+                    }
+                runs.push_back(0);
+                runs.push_back((uint32_t)m);
+                const size_t nch = (m + 63) / 64;
+                const size_t CW = FUSED_CHUNK_WORDS;
+                std::vector<uint32_t> chunks(CW * nch);
+                {
+                    size_t sb = 0, cb = 0, r = 0;
+                    for (size_t ch = 0; ch < nch; ++ch) {
+                        const size_t o0 = ch * 64;
+                        while (runs[2 * (r + 1) + 1] <= o0) ++r;
+                        chunks[CW * ch] = (uint32_t)sb;
+                        chunks[CW * ch + 1] = (uint32_t)r;
+                        chunks[CW * ch + 2] = (uint32_t)cb;
+                        uint64_t mask = 0;
+                        for (size_t q = r + 1; runs[2 * q + 1] < std::min(m, o0 + 64); ++q) mask |= 1ull << (runs[2 * q + 1] - o0);
+                        chunks[CW * ch + 3] = (uint32_t)mask;
+                        chunks[CW * ch + 4] = (uint32_t)(mask >> 32);
+                        for (size_t o = o0; o < std::min(m, o0 + 64); ++o) {
+                            sb += counts[o] & 0x7fu;
+                            cb += counts[o] >> 7;
+                        }
+                    }
+                }
+
+                const uint32_t runsOff = appendWords(runs.data(), runs.size() * 4);
+                const uint32_t countsOff = appendWords(counts.data(), counts.size());
+                const uint32_t slotsOff = appendWords(slotList.data(), slotList.size() * 2);
+                const uint32_t chunksOff = appendWords(chunks.data(), chunks.size() * 4);
+                pool[H + 0] = u32(inOff[k], "input base");
+                pool[H + 1] = (uint32_t)N;
+                pool[H + 2] = u32(m, "outputs per tile");
+                pool[H + 3] = (uint32_t)nch;
+                pool[H + 4] = runsOff;
+                pool[H + 5] = countsOff;
+                pool[H + 6] = slotsOff;
+                pool[H + 7] = chunksOff;
+                // periodic constants: the outputs split into D equal dof blocks (columns b + d N
+                // for d = 0..D-1) and every block holds the same constant values in the same
+                // order; then one block's constants are stored and H[11] carries the period
+                uint32_t period = 0;
+                stored = cvals.size();
+                if (D > 1 && D < 16 && m % (size_t)D == 0 && cvals.size() % (size_t)D == 0 && !cvals.empty()) {
+                    const size_t per = cvals.size() / (size_t)D, blk = m / (size_t)D;
+                    bool ok = true;
+                    size_t k = 0;
+                    for (int d = 0; d < D && ok; ++d) {
+                        size_t inBlock = 0;
+                        for (size_t o = (size_t)d * blk; o < (size_t)(d + 1) * blk; ++o) inBlock += counts[o] >> 7;
+                        ok = inBlock == per;
+                        for (size_t j = 0; j < per && ok; ++j, ++k) ok = cvals[k] == cvals[j];
+                    }
+                    if (ok) {
+                        period = (uint32_t)D;
+                        stored = per;
+                    }
+                }
+                if (anyConst) {
+                    pool[H + 8] = u32(out.consts.size(), "constants");
+                    out.consts.insert(out.consts.end(), cvals.begin(), cvals.begin() + (long)stored);
+                } else {
+                    pool[H + 8] = NONE;
+                }
+                pool[H + 9] = 0;  // result base, set at merge
+                pool[H + 10] = chunksOff + (uint32_t)(CW * nch) - runsOff;
+                if (stored >= (1u << 28)) throw std::length_error("fused form: too many output constants in a tile");
+                pool[H + 11] = anyConst ? (uint32_t)stored | (period << 28) : 0u;
+                contrib = slotList.size();
             }
-            pool[H + 9] = 0;  // result base, set at merge
-            pool[H + 10] = chunksOff + (uint32_t)(CW * nch) - runsOff;
-            if (stored >= (1u << 28)) throw std::length_error("fused form: too many output constants in a tile");
-            pool[H + 11] = anyConst ? (uint32_t)stored | (period << 28) : 0u;
             pool[H + 12] = 0;  // element image, set at merge
             pool[H + 13] = (uint32_t)tileNodeList.size();
             pool[H + 14] = appendWords(tileNodeList.data(), tileNodeList.size() * 4);
@@ -488,14 +806,14 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
                 }
             }
             const size_t staged = fusedStagedDoubles(fp, &pool[H]);
-            const size_t tileLds = 8 * (ldsBase[NT] + 1 + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * stored;
+            const size_t tileLds = 8 * (valueRegion + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * stored;
             out.cost.push_back((double)m + (double)ldsBase[NT]);
             out.values.push_back(ldsBase[NT]);
             out.stagedLds = std::max(out.stagedLds, 8 * staged);
             out.evaluations += evals;
-            out.contributions += slotList.size();
+            out.contributions += contrib;
             out.lds = std::max(out.lds, tileLds);
-            out.elementLds = std::max(out.elementLds, 8 * (ldsBase[NT] + 1 + staged));
+            out.elementLds = std::max(out.elementLds, 8 * (valueRegion + staged));
             out.outputs += m;
             ++tileId;
         }
@@ -540,6 +858,7 @@ FusedProgram HessianAssembly::buildFused(const BuildOptions& opt, const FusedOpt
         fp.ldsBytes = std::max(fp.ldsBytes, tiles[k].lds);
         fp.elementLdsBytes = std::max(fp.elementLdsBytes, tiles[k].elementLds);
         fp.maxPairs = std::max(fp.maxPairs, tiles[k].maxPairs);
+        fp.maxBlocks = std::max(fp.maxBlocks, tiles[k].maxBlocks);
     }
     u32(pb, "tile pool");
     u32(cb, "constants");
@@ -605,20 +924,38 @@ std::vector<float> tileStats(const FusedProgram& fp) {
     for (size_t t = 0; t < fp.numTiles; ++t) {
         const size_t H = P[1 + t];
         const uint32_t m = P[H + 2];
-        const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[H + P[H + 5]]);
-        size_t S = 0;
-        for (size_t o = 0; o < m; ++o) S += cnts[o] & 0x7fu;
-        size_t elems = 0, staged = fp.kind == 1 ? P[H + 12] : 0, vals = 0, runs = 0;
-        if (fp.kind == 1) S = 0;
-        if (fp.kind == 1)
-            for (size_t o = 0; o < m; ++o) S += 2 * (cnts[o] & 0x7fu) + (cnts[o] >> 7);
+        const bool blockTile = fp.kind == 0 && fp.sums == FusedSums::Blocks;
+        size_t elems = 0, staged = fp.kind == 1 ? P[H + 12] : 0, vals = 0, runs = 0, S = 0;
         for (size_t q = 0; q < NT; ++q) {
             const size_t cnt = P[H + HEADER + 3 * q];
             elems += cnt;
             vals += cnt * (size_t)fp.templates[q].valueStride;
         }
+        if (blockTile) {
+            // contributions: list entries of blocks with elements (not the zero entry), D^2 each
+            const size_t nb = P[H + 3], nbch = (nb + 63) / 64, noch = ((size_t)m + 63) / 64;
+            const size_t Z = (size_t)fp.blockDofs * fp.blockDofs;
+            const size_t bch = H + P[H + 4] + 3 * noch, brec = bch + 2 * nbch, lst = H + P[H + 6];
+            for (size_t ch = 0; ch < nbch; ++ch) {
+                const size_t lb = P[bch + 2 * ch] & 0xffffffu, Lc = P[bch + 2 * ch] >> 24;
+                for (size_t l = 0; l < 64 && ch * 64 + l < nb; ++l) {
+                    if (((P[brec + ch * 64 + l] >> 14) & 3u) == 2) continue;
+                    for (size_t j = 0; j < Lc; ++j) {
+                        const uint32_t w = P[lst + lb + 64 * (j / 2) + l];
+                        if ((j % 2 ? w >> 16 : w & 0xffffu) != vals) S += Z;
+                    }
+                }
+            }
+            while (P[H + P[H + 5] + 2 * runs + 1] != m) ++runs;
+        } else {
+            const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[H + P[H + 5]]);
+            if (fp.kind == 1)
+                for (size_t o = 0; o < m; ++o) S += 2 * (cnts[o] & 0x7fu) + (cnts[o] >> 7);
+            else
+                for (size_t o = 0; o < m; ++o) S += cnts[o] & 0x7fu;
+            while (P[H + P[H + 4] + 2 * runs + 1] != m) ++runs;
+        }
         if (fp.kind == 0) staged = fusedStagedDoubles(fp, &P[H]);
-        while (P[H + P[H + 4] + 2 * runs + 1] != m) ++runs;
         float* f = &out[t * TILE_STATS];
         f[0] = (float)m;
         f[1] = (float)S;
@@ -719,6 +1056,106 @@ void validateProducts(const FusedProgram& fp) {
 
 }  // namespace
 
+namespace {
+
+// Block-sum recipe of the element tile at H (layout in Fused.hpp); nvals: the tile's element
+// values, staged: its staged doubles.  Walks every table as the kernel reads it and checks
+// that the block entries' output positions are exactly the tile's outputs, each once.
+// Returns the tile's output contributions (list entries times D^2).
+size_t validateBlockTile(const FusedProgram& fp, size_t H, size_t nvals, size_t staged) {
+    const auto& P = fp.pool;
+    auto need = [](bool ok, const char* what) {
+        if (!ok) throw std::logic_error(std::string("fused form: invalid block-sum tile: ") + what);
+    };
+    const size_t D = (size_t)fp.blockDofs, Z = D * D, ND = D * (D + 1) / 2;
+    need(D >= 1 && D <= 3, "block dofs");
+    const size_t m = P[H + 2], nb = P[H + 3], noch = (m + 63) / 64, nbch = (nb + 63) / 64;
+    need(m <= 65536 && nb <= fp.maxBlocks, "outputs or blocks");
+    const size_t rec = H + P[H + 4], runs = H + P[H + 5], lst = H + P[H + 6], nod = H + P[H + 7];
+    const size_t bch = rec + 3 * noch, brec = bch + 2 * nbch;
+    need(brec + nb == nod && nod <= runs && runs < lst && (runs - nod) % 2 == 0 && (lst - runs) % 2 == 0,
+         "recipe tables not contiguous");
+    const size_t listWords = (size_t)P[H + 4] + P[H + 10] - P[H + 6];
+    need(P[H + 4] + (size_t)P[H + 10] >= P[H + 6] && lst + listWords <= P.size(), "lists outside the pool");
+    const size_t nOwned = (runs - nod) / 2, numRuns = (lst - runs) / 2 - 1;
+    need(nOwned <= 0x3fff, "owned nodes");
+    // runs: first outputs increasing from 0 to the sentinel m; positions inside the results
+    need(numRuns >= 1 || m == 0, "runs");
+    need(P[runs + 2 * numRuns + 1] == m && (m == 0 || P[runs + 1] == 0), "run sentinel");
+    for (size_t r = 0; r < numRuns; ++r) {
+        const size_t first = P[runs + 2 * r + 1], next = P[runs + 2 * r + 3];
+        need(next > first && next <= m, "runs not increasing");
+        need((size_t)P[H + 9] + P[runs + 2 * r] + (next - first) <= fp.numResults, "run outside the results");
+    }
+    for (size_t ch = 0; ch < noch; ++ch) {
+        const size_t r = P[rec + 3 * ch];
+        need(r < numRuns && P[runs + 2 * r + 1] <= ch * 64 && P[runs + 2 * r + 3] > ch * 64, "output chunk run");
+        uint64_t mask = 0;
+        for (size_t q = r + 1; q < numRuns && P[runs + 2 * q + 1] < std::min<size_t>(m, ch * 64 + 64); ++q)
+            mask |= 1ull << (P[runs + 2 * q + 1] - ch * 64);
+        need(P[rec + 3 * ch + 1] == (uint32_t)mask && P[rec + 3 * ch + 2] == (uint32_t)(mask >> 32), "output chunk run mask");
+    }
+    // node records: columns inside the outputs
+    for (size_t ni = 0; ni < nOwned; ++ni) {
+        const uint32_t w0 = P[nod + 2 * ni], w1 = P[nod + 2 * ni + 1];
+        const size_t cs[3] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu}, nE = (w1 >> 16) & 0xffu, nK = w1 >> 24;
+        need(nE + nK <= 127, "node blocks");
+        for (size_t d = 0; d < 3; ++d) need(d < D ? cs[d] + D * nE + nK <= m : cs[d] == 0, "node columns");
+    }
+    // blocks: ranks inside their node's columns; their entries' positions cover the outputs once
+    std::vector<char> seen(m, 0);
+    size_t numConst = 0, contrib = 0, listBase = 0;
+    for (size_t ch = 0; ch < nbch; ++ch) {
+        const uint32_t c0 = P[bch + 2 * ch];
+        const size_t lb = c0 & 0xffffffu, Lc = c0 >> 24, pairs = (Lc + 1) / 2;
+        need(lb == listBase && P[bch + 2 * ch + 1] == numConst, "block chunk record");
+        listBase += 64 * pairs;
+        need(listBase <= listWords, "block lists");
+        for (size_t l = 0; l < 64; ++l) {
+            const size_t bi = ch * 64 + l;
+            size_t kind = 3;
+            if (bi < nb) {
+                const uint32_t r = P[brec + bi];
+                const size_t ni = r & 0x3fffu, rE = (r >> 16) & 0x7fu, rEK = (r >> 23) & 0x7fu;
+                kind = (r >> 14) & 3u;
+                need(ni < nOwned && kind <= 2 && (r >> 31) == 0, "block record");
+                numConst += (r >> 30) & 1u;
+                const uint32_t w0 = P[nod + 2 * ni], w1 = P[nod + 2 * ni + 1];
+                const size_t cs[3] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu}, nE = (w1 >> 16) & 0xffu, nK = w1 >> 24;
+                need(rEK < nE + nK && (kind == 2 || rE < nE), "block ranks");
+                for (size_t db = 0; db < D; ++db)
+                    for (size_t da = 0; da < D; ++da) {
+                        if (kind == 2 && da != db) continue;
+                        const size_t pos = cs[db] + da * nE + (db < da ? nK : 0) + (da == db ? rEK : rE);
+                        need(pos < m && !seen[pos], "block output position");
+                        seen[pos] = 1;
+                    }
+            }
+            // entries a lane reads: every one of its chunk's list (kind 0: D(D+1)/2 values
+            // each, kind 1: D^2); wings and lanes past the last block read none
+            for (size_t j = 0; j < Lc; ++j) {
+                const uint32_t w = P[lst + lb + 64 * (j / 2) + l];
+                const uint32_t e = j % 2 ? w >> 16 : w & 0xffffu;
+                const size_t v = e & 0x7fffu;
+                if (kind == 0) need((e >> 15) == 0 && v + ND <= nvals + Z, "diagonal block entry");
+                if (kind == 1) need(v + Z <= nvals + Z, "block entry");
+                if (kind >= 2) need(e == nvals, "entry of a block without elements");
+                if (kind <= 1 && e != nvals) contrib += Z;
+            }
+        }
+    }
+    need(listBase == listWords, "list words");
+    for (size_t o = 0; o < m; ++o) need(seen[o], "output without a block entry");
+    const size_t stored = P[H + 11];
+    need(stored == numConst, "block constant count");
+    if (stored) need(P[H + 8] != 0xffffffffu && (size_t)P[H + 8] + stored <= fp.consts.size(), "block constants outside the constants");
+    need(8 * (std::max(nvals + Z, m) + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * stored <= fp.ldsBytes,
+         "tile LDS exceeds the launch's LDS");
+    return contrib;
+}
+
+}  // namespace
+
 void validateFused(const FusedProgram& fp) {
     if (fp.kind == 1) {
         validateProducts(fp);
@@ -734,17 +1171,20 @@ void validateFused(const FusedProgram& fp) {
     for (size_t t = 0; t < fp.numTiles; ++t) {
         const size_t H = P[1 + t];
         need(H + HEADER + 3 * NT <= P.size(), "header outside the pool");
-        const uint32_t B = P[H], N = P[H + 1], m = P[H + 2], nch = P[H + 3];
+        const bool blockTile = fp.sums == FusedSums::Blocks;
+        const uint32_t B = P[H], N = P[H + 1], m = P[H + 2], nch = blockTile ? (P[H + 2] + 63) / 64 : P[H + 3];
         need(nch == (m + 63) / 64, "chunk count");
         const size_t runs = H + P[H + 4], counts = H + P[H + 5], slots = H + P[H + 6], chunks = H + P[H + 7];
         const uint32_t cOff = P[H + 8], rbase = P[H + 9];
         const size_t CW = FUSED_CHUNK_WORDS;
-        need(chunks + CW * (size_t)nch <= P.size() && counts + ((size_t)m + 3) / 4 <= P.size(), "tables outside the pool");
+        if (!blockTile) {
+            need(chunks + CW * (size_t)nch <= P.size() && counts + ((size_t)m + 3) / 4 <= P.size(), "tables outside the pool");
+            need(P[H + 10] == P[H + 7] + CW * nch - P[H + 4] && P[H + 4] <= P[H + 5] && P[H + 5] <= P[H + 6] &&
+                     P[H + 6] <= P[H + 7],
+                 "recipe tables not contiguous");
+        }
         // LDS of the tile and element data
         size_t lds = 0, staged = 0;
-        need(P[H + 10] == P[H + 7] + CW * nch - P[H + 4] && P[H + 4] <= P[H + 5] && P[H + 5] <= P[H + 6] &&
-                 P[H + 6] <= P[H + 7],
-             "recipe tables not contiguous");
         // tile nodes: sorted, inside the instance, every staged dof inside the inputs
         const size_t nn = P[H + 13], tn = H + P[H + 14];
         need(nn <= 65536 && tn + nn <= P.size(), "tile node list outside the pool");
@@ -765,6 +1205,10 @@ void validateFused(const FusedProgram& fp) {
         need((size_t)P[H + 12] >= fp.numInputs && (size_t)P[H + 12] + lds <= fp.numInputs + fp.imageDoubles,
              "element image outside the value buffer");
         need(8 * staged <= fp.stagedLdsBytes, "staged values exceed the element launch's LDS");
+        if (blockTile) {
+            totalContrib += validateBlockTile(fp, H, lds, staged);
+            continue;
+        }
         const uint8_t* cnts = reinterpret_cast<const uint8_t*>(&P[counts]);
         size_t S = 0, numConst = 0;
         for (size_t o = 0; o < m; ++o) {

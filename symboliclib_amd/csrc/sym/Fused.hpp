@@ -45,6 +45,39 @@
 //   count_t + i), node coordinates (dof d of local node s at X + d * nn + s, stagedDofs
 //   dofs), slot tables (template t's words at W_t), then the recipe (u32 words copied from
 //   H[4]), then the output constants.
+//
+// Block sums (FusedProgram::sums == FusedSums::Blocks).  When every element adds a full D x D
+// block to every pair of its nodes (D = blockDofs: all dofs of a node are differentiated) and
+// constants are nodal blocks M (x) I_D, the Hessian is made of node blocks: block (a, b) of
+// column node b is either full (a shares an element with b; a == b is the diagonal block) or
+// holds only its diagonal (a is coupled to b by a constant alone, a "wing").  A lane then sums
+// a whole block from one list of (element value base, transposed) entries, since the D x D
+// entries of one element's block sit at consecutive value slots (the template's outputs are
+// laid out block by block: pair (p, q), p < q, of the element's nodes as D x D row-major
+// values, pair (p, p) as its D (D + 1) / 2 upper-triangle values).  Column (b, db) of the
+// output holds, for da = 0..D-1, the nodes E(b) (sharing an element with b, b included) and,
+// when da == db, the wings K(b), each segment sorted by node; so entry (a, da; b, db) of a
+// block sits at tile output
+//   cs[db] + da * nE + (db < da ? nK : 0) + (da == db ? rank of a in E(b) u K(b) : rank in E(b)).
+// Lanes keep their block sums in registers (a local array beyond 8 blocks per thread), write
+// them after a barrier to the tile's outputs
+// in LDS (over the element values, which are dead by then), and a last pass stores the
+// outputs in CSC order (whole lines per wave).  Header words that differ from the form above:
+//     H[3] nb, blocks of the tile
+//     H[4] rel recipe start: output chunk records (first run, run-start mask low and high:
+//          3 words per 64 outputs), then block chunk records (list base in words | longest
+//          list << 24, first constant: 2 words per 64 blocks), then block records (1 word per block: owned
+//          node index bits 0-13, kind bits 14-15 (0 diagonal, 1 full, 2 wing), rank in E(b)
+//          bits 16-22, rank in E(b) u K(b) bits 23-29, has a constant bit 30)
+//     H[5] rel runs (as above)   H[6] rel lists   H[7] rel node records (2 words per owned node:
+//          cs[0] | cs[1] << 16, cs[2] | nE << 16 | nK << 24)
+//     H[8] block constants in c[] (one per block that has one, in block order) or 0xffffffff
+//     H[10] words from the recipe start to the end of the lists   H[11] stored constants
+//   Lists: a chunk of 64 blocks reads (longest + 1) / 2 u32 words per lane, word j of lane l at list base
+//   + 64 j + l, each two u16 entries (value index bits 0-14, transposed bit 15); entries past a
+//   block's list point at D x D zeros after the element values (V[nvals ..]).
+//   LDS: values and their zeros, or the tile's outputs when more (max(nvals + D^2, m)
+//   doubles), then the staged region, the recipe and the block constants.
 #pragma once
 
 #include <cstdint>
@@ -80,7 +113,12 @@ struct FusedOptions {
     // stage tiles with direct global-to-LDS loads (no register round trip, no wait per load);
     // false: through registers
     bool ldsDma = true;
+    // element tiles: sum node blocks when the Hessian is block-structured (FusedSums::Blocks);
+    // false: one contribution list per output
+    bool blockSums = true;
 };
+
+enum class FusedSums : int { Outputs = 0, Blocks = 1 };
 
 // `work` starts with two work-queue counters (next tile, finished workgroups) per launch of an
 // evaluation (the split form has two launches).  Profiled launches record after them, per tile
@@ -122,6 +160,9 @@ struct FusedProgram {
     size_t headerTable = 0, headerStride = 0, headerNodes = 0;
     int stagedDofs = 0;  // dofs staged per tile node: 1 + the largest dof an element reads
     int maxPairs = 0;    // element tiles: the most slot pairs (even-padded contributions / 2) of an output
+    FusedSums sums = FusedSums::Outputs;
+    int blockDofs = 0;      // block sums: D, dofs per node block
+    size_t maxBlocks = 0;   // block sums: the most blocks of a tile
     size_t elementEvaluations = 0, numElements = 0, numContributions = 0;
     double recompute() const { return numElements ? (double)elementEvaluations / (double)numElements : 0.0; }
     std::string summary() const;
@@ -144,6 +185,7 @@ constexpr int FUSED_CHUNK_WORDS = 5;  // words per 64-output chunk record of an 
 // the chunk records that follow it (at least FUSED_CHUNK_WORDS words).
 constexpr int FUSED_UNROLLED_PAIRS = FUSED_CHUNK_WORDS;
 constexpr size_t FUSED_MAX_LDS_BYTES = 160 * 1024;  // LDS of one gfx950 CU
+constexpr size_t FUSED_MAX_BLOCKS = 4096;           // block sums: node blocks of one tile
 std::vector<float> tileStats(const FusedProgram& fp);
 
 // Write the tile table pool[1..numTiles] in the order of `schedule`.

@@ -1012,6 +1012,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
     if (T <= 0 || T > 1024 || T % 64) throw std::invalid_argument("emitHipFused: threads per block must be a multiple of 64 up to 1024");
     const bool persistent = opt.schedule != FusedSchedule::Static;
     const bool counter = persistent && opt.schedule != FusedSchedule::Strided;
+    const bool blockSums = fp.sums == FusedSums::Blocks;
     static const Program noProgram;
     static const ProgramKernel noKernel;
     const size_t NT = fp.templates.size();
@@ -1316,6 +1317,155 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         }
         os << "        }\n";
         };
+    // B with block sums: B1 each lane sums one node block of the tile (sorted so a wave's
+    // blocks have near-equal lists) from its list of (value index, transposed) entries into
+    // registers; after a barrier B2 writes every block's entries to their tile outputs in LDS
+    // (over the element values); after a second barrier B3 stores the outputs in CSC order
+    auto emitBlockSums = [&]() {
+        const int D = fp.blockDofs, Z = D * D, W = T / 64;
+        const int iters = std::max<int>(1, (int)((fp.maxBlocks + (size_t)T - 1) / (size_t)T));
+        // up to 8 blocks per thread: every block's sums in registers (unrolled); more (small
+        // workgroups on large tiles): a loop over an indexed local array, which may live in
+        // scratch memory
+        const bool unrolled = iters <= 8;
+        auto acc = [&](int it, int k) { return (unrolled ? "a" + std::to_string(it) : std::string("a")) + "_" + std::to_string(k); };
+        auto recv = [&](int it) { return unrolled ? "r" + std::to_string(it) : std::string("rb"); };
+        os << "        const unsigned nb = nch, noch = (m + 63u) >> 6, nbch = (nb + 63u) >> 6;\n"
+           << "        const INDEX* __restrict__ BC = R + 3u * noch;  // block chunk records\n"
+           << "        const INDEX* __restrict__ BR = BC + 2u * nbch;  // block records\n"
+           << "        const INDEX* __restrict__ NR = R + (hchk - hrec);  // node records\n"
+           << "        const INDEX* __restrict__ RN = R + (hcnt - hrec);  // runs\n"
+           << "        const unsigned* __restrict__ LW = R + (hslt - hrec);  // lists\n";
+        auto declare = [&](int it, const std::string& ind) {
+            os << ind << "double";
+            for (int k = 0; k < Z; ++k) os << (k ? ", " : " ") << acc(it, k) << " = 0.0";
+            os << ";\n" << ind << "unsigned " << recv(it) << " = 0xc000u;  // no block\n";
+        };
+        if (unrolled)
+            for (int it = 0; it < iters; ++it) declare(it, "        ");
+        else
+            os << "        double acc[" << iters << "][" << Z << "];\n        unsigned recs[" << iters << "];\n";
+        // one entry of a diagonal block (D (D + 1) / 2 upper-triangle values) or of a full block
+        // (D x D row-major, transposed when the element holds the pair the other way round)
+        auto diagEntry = [&](int it, const std::string& e, const std::string& ind) {
+            os << ind << "{\n" << ind << "    const double* __restrict__ p = V + (" << e << " & 0x7fffu);\n";
+            int t = 0;
+            for (int da = 0; da < D; ++da)
+                for (int db = da; db < D; ++db, ++t) os << ind << "    " << acc(it, da * D + db) << " += p[" << t << "];\n";
+            os << ind << "}\n";
+        };
+        auto fullEntry = [&](int it, const std::string& e, const std::string& ind) {
+            os << ind << "{\n" << ind << "    const double* __restrict__ p = V + (" << e << " & 0x7fffu);\n"
+               << ind << "    const bool tr = ((" << e << " >> 15) & 1u) != 0u;\n";
+            for (int k = 0; k < Z; ++k) os << ind << "    const double v" << k << " = p[" << k << "];\n";
+            for (int da = 0; da < D; ++da)
+                for (int db = 0; db < D; ++db) {
+                    const int k = da * D + db, kt = db * D + da;
+                    if (k == kt) os << ind << "    " << acc(it, k) << " += v" << k << ";\n";
+                    else os << ind << "    " << acc(it, k) << " += tr ? v" << kt << " : v" << k << ";\n";
+                }
+            os << ind << "}\n";
+        };
+        // B1 for iteration `it` (unrolled: a literal; looped: the loop's variable)
+        auto b1 = [&](int it) {
+            const std::string chx = unrolled ? std::to_string(it * W) + "u + wv" : "it * " + std::to_string(W) + "u + wv";
+            os << "        {\n"
+               << "            const unsigned ch = " << chx << ";\n"
+               << "            if (ch < nbch) {  // uniform\n"
+               << "                const unsigned bi = ch * 64u + lane;\n"
+               << "                const unsigned rec = bi < nb ? BR[bi] : 0xc000u;\n"
+               << "                " << recv(it) << " = rec;\n"
+               << "                const unsigned c0 = BC[2u * ch], kb = BC[2u * ch + 1u];\n"
+               << "                const unsigned L = c0 >> 24, kind = (rec >> 14) & 3u;\n"
+               << "                const unsigned* __restrict__ lw = LW + (c0 & 0xffffffu) + lane;\n";
+            for (int kind = 0; kind < 2; ++kind) {
+                os << "                " << (kind ? "else if" : "if") << " (kind == " << kind << "u) {\n"
+                   << "                    for (unsigned j = 0; j < L; j += 2u) {\n"
+                   << "                        const unsigned w = lw[(j >> 1) * 64u];\n";
+                if (kind == 0) diagEntry(it, "w", "                        ");
+                else fullEntry(it, "w", "                        ");
+                os << "                        if (j + 1u < L) {\n"
+                   << "                            const unsigned w1 = w >> 16;\n";
+                if (kind == 0) diagEntry(it, "w1", "                            ");
+                else fullEntry(it, "w1", "                            ");
+                os << "                        }\n"
+                   << "                    }\n"
+                   << "                }\n";
+            }
+            // a block's constant adds to its diagonal entries; constants are stored per block
+            // in block order, so a lane's is the chunk's first plus the lanes below with one
+            os << "                const unsigned hc = (rec >> 30) & 1u;\n"
+               << "                const unsigned long long cm = __ballot(hc);\n"
+               << "                const unsigned ci = kb + __builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32), "
+                  "__builtin_amdgcn_mbcnt_lo((unsigned)cm, 0u));\n"
+               << "                if (hc) {\n"
+               << "                    const double kc = K[ci];\n";
+            for (int d = 0; d < D; ++d) os << "                    " << acc(it, d * D + d) << " += kc;\n";
+            os << "                }\n";
+            if (D > 1) {
+                os << "                if (kind == 0u) {  // the diagonal block's lower triangle\n";
+                for (int da = 0; da < D; ++da)
+                    for (int db = 0; db < da; ++db) os << "                    " << acc(it, da * D + db) << " = " << acc(it, db * D + da) << ";\n";
+                os << "                }\n";
+            }
+            os << "            }\n"
+               << "        }\n";
+        };
+        // B2 for iteration `it`: the block's entries to their tile outputs
+        auto b2 = [&](int it) {
+            const std::string r = recv(it);
+            os << "        {\n"
+               << "            const unsigned kind = (" << r << " >> 14) & 3u;\n"
+               << "            if (kind < 3u) {\n"
+               << "                const unsigned ni = " << r << " & 0x3fffu, rE = (" << r << " >> 16) & 0x7fu, rEK = (" << r
+               << " >> 23) & 0x7fu;\n"
+               << "                const unsigned w0 = NR[2u * ni], w1 = NR[2u * ni + 1u];\n"
+               << "                const unsigned nE = (w1 >> 16) & 0xffu, nK = w1 >> 24;\n"
+               << "                (void)nK;\n";
+            const char* cs[3] = {"(w0 & 0xffffu)", "(w0 >> 16)", "(w1 & 0xffffu)"};
+            for (int db = 0; db < D; ++db)
+                for (int da = 0; da < D; ++da) {
+                    std::string pos = std::string(cs[db]) + (da ? " + " + std::to_string(da) + "u * nE" : "") +
+                                      (db < da ? " + nK" : "") + (da == db ? " + rEK" : " + rE");
+                    if (da == db) os << "                V[" << pos << "] = " << acc(it, da * D + db) << ";\n";
+                    else os << "                if (kind != 2u) V[" << pos << "] = " << acc(it, da * D + db) << ";\n";
+                }
+            os << "            }\n"
+               << "        }\n";
+        };
+        if (unrolled) {
+            for (int it = 0; it < iters; ++it) b1(it);
+            os << "        __syncthreads();  // every element value read: the outputs overlay them\n";
+            for (int it = 0; it < iters; ++it) b2(it);
+        } else {
+            os << "        for (unsigned it = 0; it < " << iters << "u; ++it) {\n";
+            declare(0, "        ");
+            b1(0);
+            for (int k = 0; k < Z; ++k) os << "        acc[it][" << k << "] = " << acc(0, k) << ";\n";
+            os << "        recs[it] = " << recv(0) << ";\n"
+               << "        }\n"
+               << "        __syncthreads();  // every element value read: the outputs overlay them\n"
+               << "        for (unsigned it = 0; it < " << iters << "u; ++it) {\n"
+               << "        const double";
+            for (int k = 0; k < Z; ++k) os << (k ? ", " : " ") << acc(0, k) << " = acc[it][" << k << "]";
+            os << ";\n        const unsigned " << recv(0) << " = recs[it];\n";
+            b2(0);
+            os << "        }\n";
+        }
+        os << "        __syncthreads();\n";
+        // B3: 64 consecutive outputs per wave, whole lines, stored without caching
+        os << "        {\n"
+           << "            double* __restrict__ res = out + hres;\n"
+           << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
+           << "            for (unsigned ch = wv; ch < noch; ch += " << W << "u) {\n"
+           << "                const unsigned oi = ch * 64u + lane;\n"
+           << "                const unsigned long long mk = ((unsigned long long)R[3u * ch + 2u] << 32) | R[3u * ch + 1u];\n"
+           << "                const unsigned r = R[3u * ch] + (unsigned)__popcll(mk & below);\n"
+           << "                const unsigned pos = RN[2u * r] + (oi - RN[2u * r + 1u]);\n"
+           << "                if (oi < m) __builtin_nontemporal_store(V[oi], res + pos);\n"
+           << "            }\n"
+           << "        }\n";
+    };
     // Kernel variants: 0 fused (stage, evaluate into LDS, sum), 1 split elements (stage, evaluate
     // into the tile's image in HBM), 2 split sums (copy image, recipe and constants to LDS, sum).
     auto kernel = [&](const char* name, int variant) {
@@ -1344,16 +1494,30 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         emitTileWords("        ");
         emitTemplateWords("        ");
         // V[nvals]: the tile's zero (odd contribution lists are padded with its slot)
-        if (variant == 0) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
+        if (variant == 0 && blockSums)
+            os << "        // values and their zeros, or the tile's outputs when more\n"
+               << "        const unsigned vreg = nvals + " << fp.blockDofs * fp.blockDofs << "u > m ? nvals + " << fp.blockDofs * fp.blockDofs
+               << "u : m;\n"
+               << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + vreg;\n";
+        else if (variant == 0)
+            os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
         if (variant == 1) os << "        double* __restrict__ V = y + H[12];\n        double* __restrict__ Ast = lds;\n";
-        if (variant == 2) os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
+        if (variant == 2 && blockSums)
+            os << "        const unsigned vreg = nvals + " << fp.blockDofs * fp.blockDofs << "u > m ? nvals + " << fp.blockDofs * fp.blockDofs
+               << "u : m;\n"
+               << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + vreg;\n";
+        else if (variant == 2)
+            os << "        double* __restrict__ V = lds;\n        double* __restrict__ Ast = lds + nvals + 1u;\n";
         os << "        INDEX* __restrict__ R = (INDEX*)(Ast + " << (variant == 2 ? "0u" : "nstaged") << ");\n";
         os << "        double* __restrict__ K = (double*)(R + ((words + 1u) & ~1u));\n";
         os << "        double* __restrict__ Xs = Ast + np;\n"
            << "        unsigned* __restrict__ Es = (unsigned*)(Xs + " << fp.stagedDofs << "u * nn);\n";
         os << "        (void)nvals; (void)nstaged; (void)R; (void)K; (void)V; (void)Xs; (void)Es;\n";
         os << "        {\n";
-        if (sums) os << "            if (threadIdx.x == 0u) V[nvals] = 0.0;\n";
+        if (sums && blockSums)
+            os << "            if (threadIdx.x < " << fp.blockDofs * fp.blockDofs << "u) V[nvals + threadIdx.x] = 0.0;\n";
+        else if (sums)
+            os << "            if (threadIdx.x == 0u) V[nvals] = 0.0;\n";
         if (stage && opt.ldsDma) {
             emitStageDma(earlyNodes, "node");
         } else if (stage) {
@@ -1413,7 +1577,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             stamp(os, 2);
 
         }
-        if (sums) emitSums();
+        if (sums && blockSums) emitBlockSums();
+        else if (sums) emitSums();
         os << "        __syncthreads();\n";
         if (variant != 1) stamp(os, 3);
         if (counter) {

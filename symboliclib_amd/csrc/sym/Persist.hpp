@@ -88,6 +88,6 @@ private:
 };
 
 constexpr char PROGRAM_FILE_MAGIC[9] = "SYMLIBPG";
-constexpr uint32_t PROGRAM_FILE_VERSION = 1;
+constexpr uint32_t PROGRAM_FILE_VERSION = 2;
 
 }  // namespace Sym

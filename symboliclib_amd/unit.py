@@ -22,7 +22,7 @@ SCHEDULES = {"static": 0, "queue": 1, "largest": 2, "interleaved": 3, "strided":
 
 def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, share_templates=True, tiled=False,
           tile_lds_bytes=0, direct_assembly=False, fused=False, schedule="queue", output_groups=0, waves_per_eu=0,
-          profile_tiles=False, split=False, register_staging=False, cpu_threads=0):
+          profile_tiles=False, split=False, register_staging=False, output_sums=False, cpu_threads=0):
     o = _native.Options()
     _native.lib().symlib_default_options(ctypes.byref(o))
     o.threads_per_block = int(threads_per_block)
@@ -39,6 +39,7 @@ def _opts(threads_per_block=256, decomposition_threshold=-1, simplify=True, shar
     o.profile_tiles = int(bool(profile_tiles))
     o.split = int(bool(split))
     o.register_staging = int(bool(register_staging))
+    o.output_sums = int(bool(output_sums))
     o.cpu_threads = int(cpu_threads)
     return o
 

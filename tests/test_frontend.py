@@ -233,14 +233,15 @@ def test_fused_form_errors():
 
 
 def test_fused_sums_unrolled_or_looped():
-    # contribution lists of at most 5 slot pairs: every lane reads the longest length at once;
-    # the center of an 18-triangle fan sums 18 contributions (9 pairs), so lists loop per lane
+    # one list per output (output_sums): contribution lists of at most 5 slot pairs: every lane
+    # reads the longest length at once; the center of an 18-triangle fan sums 18 contributions
+    # (9 pairs), so lists loop per lane
     rest, F, _ = W.c3_cloth(side=16)
-    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2, output_sums=True) as u:
         src = u.source()
         assert "np2" in src and "q < k; q += 2u" not in src
     rest, F, _ = W.wheel(18, seed=5)
-    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2, output_sums=True) as u:
         src = u.source()
         assert "q < k; q += 2u" in src and "np2" not in src
         cp, ri = O.bw_pattern(len(rest), F)
@@ -249,8 +250,26 @@ def test_fused_sums_unrolled_or_looped():
     # the bound: 10 contributions (5 pairs) unrolled, 12 (6 pairs) looped
     for n, looped in [(10, False), (12, True)]:
         rest, F, _ = W.wheel(n, seed=5)
-        with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
+        with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2, output_sums=True) as u:
             assert ("q < k; q += 2u" in u.source()) == looped
+
+
+def test_fused_block_sums_chosen_for_node_blocks():
+    # cloth Hessians are made of 3 x 3 node blocks: the fused form sums whole blocks (one list per
+    # block, the outputs through LDS); the 18-triangle fan's center block sums 18 entries
+    for rest, F in [W.c3_cloth(side=16)[:2], W.wheel(18, seed=5)[:2]]:
+        with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, output_groups=2) as u:
+            src = u.source()
+            assert "block records" in src and "np2" not in src and "q < k; q += 2u" not in src
+            cp, ri = O.bw_pattern(len(rest), F)
+            outer, inner = u.output_pattern()
+            assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
+            # every face adds its 9 x 9 Hessian once per owning tile: 81 contributions per face
+            assert u.fused_stats()["contributions"] == 81 * len(F)
+    # product tiles (A^T A + A) and general constants keep one list per output
+    n, cp, ri, a = W.c1_matrix(subdiv=2)
+    with Unit.ata_plus_a(cp, ri, fused=True) as u:
+        assert "block records" not in u.source()
 
 
 def test_fused_values_within_byte_offsets():

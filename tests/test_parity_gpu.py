@@ -283,6 +283,24 @@ def test_bw_hessian_fused(require_device, side, schedule, lds, groups, wpe, spli
         _check_cloth(f"{name} second positions", u, rest, F, X2, kb, u.get_results())
 
 
+# One contribution list per output (output_sums: the form for Hessians that are not made of
+# node blocks), in one and two launches, staged directly and through registers.
+OUTPUT_SUM_CASES = [(16, "static", 0, 0, False, False), (64, "queue", 49152, 3, True, False),
+                    (33, "strided", 81408, 2, False, True), (256, "static", 81408, 2, False, False)]
+
+
+@pytest.mark.parametrize("kb", BENDS)
+@pytest.mark.parametrize("side,schedule,lds,groups,split,staging", OUTPUT_SUM_CASES)
+def test_bw_hessian_fused_output_sums(require_device, side, schedule, lds, groups, split, staging, kb):
+    rest, F, X = W.c3_cloth(side=side)
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, kb, fused=True, schedule=schedule, tile_lds_bytes=lds,
+                         output_groups=groups, split=split, register_staging=staging, output_sums=True) as u:
+        assert "block records" not in u.source()
+        u.compile().execute(W.to_colmajor(X))
+        _check_cloth(f"Baraff-Witkin {side}x{side} fused output sums {schedule} lds={lds} split={split} "
+                     f"staging={staging}", u, rest, F, X, kb, u.get_results())
+
+
 # Staging through registers instead of direct global-to-LDS loads (the fallback path): element
 # tiles in one and two launches, several schedules and tile budgets.
 STAGING_CASES = [(16, "static", 0, 0, False), (33, "queue", 16384, 3, False), (64, "strided", 49152, 5, True),
@@ -343,20 +361,25 @@ def test_bw_hessian_fused_tiny_cloths(require_device, schedule):
                          u.get_results(k), k)
 
 
+@pytest.mark.parametrize("sums", ["blocks", "outputs"])
 @pytest.mark.parametrize("n,schedule", [(10, "static"), (18, "static"), (18, "queue")])
-def test_bw_hessian_fused_long_contribution_lists(require_device, n, schedule):
-    # the center of an n-triangle fan sums n element contributions: 5 slot pairs (n = 10, the
-    # most the unrolled sums take) or 9 (n = 18, the per-lane loop); then fans batched with a
-    # cloth (the loop)
+def test_bw_hessian_fused_long_contribution_lists(require_device, n, schedule, sums):
+    # the center of an n-triangle fan sums n element contributions.  One list per output: 5 slot
+    # pairs (n = 10, the most the unrolled sums take) or 9 (n = 18, the per-lane loop); block
+    # sums: the center's diagonal block reads n entries.  Then fans batched with a cloth.
+    outputs = sums == "outputs"
     rest, F, X = W.wheel(n, seed=5)
     with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, schedule=schedule,
-                         output_groups=2) as u:
-        assert ("q < k; q += 2u" in u.source()) == (n > 10)
+                         output_groups=2, output_sums=outputs) as u:
+        if outputs:
+            assert ("q < k; q += 2u" in u.source()) == (n > 10)
+        else:
+            assert "block records" in u.source()
         u.compile().execute(W.to_colmajor(X))
-        _check_cloth(f"fan of {n} fused {schedule}", u, rest, F, X, W.K_BEND, u.get_results())
+        _check_cloth(f"fan of {n} fused {schedule} {sums}", u, rest, F, X, W.K_BEND, u.get_results())
     meshes = [W.wheel(18, seed=6), W.cloth(9, seed=8), W.wheel(7, seed=9)]
     with Unit.bw_hessian_batch([(r, F) for r, F, _ in meshes], W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True,
-                               schedule=schedule, output_groups=2) as u:
+                               schedule=schedule, output_groups=2, output_sums=outputs) as u:
         u.compile().execute(*[W.to_colmajor(X) for _, _, X in meshes])
         for k, (rest, F, X) in enumerate(meshes):
             _check_cloth(f"fans and a cloth fused {schedule}, mesh {k}", u, rest, F, X, W.K_BEND, u.get_results(k), k)
