@@ -433,8 +433,9 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                 }
                 double need = (double)(elemBytes + newBytes + nodeLds * (tileNodeList.size() + newNodes.size()));
                 // block sums: the tile's outputs overlay its values and their zeros when more
+                // (plus a padding slot before each run: at most Db per owned node, one at the end)
                 if (blocks)
-                    need += listMargin + std::max(0.0, 8.0 * (double)(outs + outCount[v]) -
+                    need += listMargin + std::max(0.0, 8.0 * (double)(outs + outCount[v] + (size_t)Db * (tNodes.size() + 1) + 1) -
                                                            (double)(valueBytes + newValueBytes + 8 * (size_t)Db * Db));
                 // element values and the zero after them within 64 KiB: slots are u16 byte offsets
                 const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget && valueBytes + newValueBytes + 8 <= 65536;
@@ -494,8 +495,13 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                 if (nvals + Z > 0x7fff) throw NotBlockStructured("tile values exceed 15-bit value indices");
                 if (nOwned > 0x3fff) throw NotBlockStructured("more than 16383 owned nodes in a tile");
                 const uint16_t zeroEntry = (uint16_t)nvals;
-                std::vector<uint32_t> runs;  // (position, first output)
+                // output slots of the tile in run order; every run starts at an even slot (a
+                // padding slot before it when needed) and the slot count is even, so the outputs
+                // are stored in pairs of slots
+                std::vector<uint32_t> runs;  // (position, first slot)
                 std::vector<uint32_t> colStart((size_t)Db * nOwned, 0);
+                std::vector<char> padSlot;
+                size_t sl = 0;
                 uint32_t prevEnd = NONE;
                 for (int d = 0; d < D; ++d)
                     for (size_t ni = 0; ni < nOwned; ++ni) {
@@ -504,18 +510,28 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                             if (colptr[c + 1] != colptr[c]) throw NotBlockStructured("output column beyond the block dofs");
                             continue;
                         }
-                        colStart[(size_t)d * nOwned + ni] = (uint32_t)m;
-                        for (int q = colptr[c]; q < colptr[c + 1]; ++q, ++m) {
+                        for (int q = colptr[c]; q < colptr[c + 1]; ++q, ++m, ++sl) {
                             if ((uint32_t)q != prevEnd) {
+                                if (sl % 2) {
+                                    padSlot.resize(sl + 1, 0);
+                                    padSlot[sl++] = 1;
+                                }
                                 runs.push_back((uint32_t)q);
-                                runs.push_back((uint32_t)m);
+                                runs.push_back((uint32_t)sl);
                             }
+                            if (q == colptr[c]) colStart[(size_t)d * nOwned + ni] = (uint32_t)sl;
                             prevEnd = (uint32_t)q + 1;
                         }
                     }
+                if (sl % 2) {
+                    padSlot.resize(sl + 1, 0);
+                    padSlot[sl++] = 1;
+                }
+                padSlot.resize(sl, 0);
+                const size_t m2 = sl;
                 runs.push_back(0);
-                runs.push_back((uint32_t)m);
-                if (m > 65536) throw std::logic_error("fused form: more than 65536 outputs in a tile");
+                runs.push_back((uint32_t)m2);
+                if (m2 > 65536) throw NotBlockStructured("more than 65536 output slots in a tile");
 
                 // blocks of every owned node b: list entries from its elements, E(b) (nodes
                 // sharing an element), K(b) (the other dof-0 rows of column (b, 0)); the pattern's
@@ -610,7 +626,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                     }
                 }
                 // blocks by kind, longest lists first: a chunk's lanes have near-equal lists
-                const size_t nb = blks.size(), nbch = (nb + 63) / 64, noch = (m + 63) / 64;
+                const size_t nb = blks.size(), nbch = (nb + 63) / 64, noch = (m2 + 127) / 128;
                 std::vector<uint32_t> order(nb);
                 std::iota(order.begin(), order.end(), 0u);
                 std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
@@ -637,16 +653,21 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                         if (B->kind != 2) contrib += (size_t)B->len * Z;
                     }
                 }
-                // output chunk records: the run of each chunk's first output and the run starts in it
-                std::vector<uint32_t> outChunk(3 * noch);
+                // output chunk records, per 128 slots (64 pairs): the run of the first pair, the
+                // pairs after it where a run starts, the pairs whose second slot is padding
+                std::vector<uint32_t> outChunk(5 * noch);
                 for (size_t ch = 0, r = 0; ch < noch; ++ch) {
-                    const size_t o0 = ch * 64;
-                    while (runs[2 * (r + 1) + 1] <= o0) ++r;
-                    uint64_t mask = 0;
-                    for (size_t q = r + 1; runs[2 * q + 1] < std::min(m, o0 + 64); ++q) mask |= 1ull << (runs[2 * q + 1] - o0);
-                    outChunk[3 * ch] = (uint32_t)r;
-                    outChunk[3 * ch + 1] = (uint32_t)mask;
-                    outChunk[3 * ch + 2] = (uint32_t)(mask >> 32);
+                    const size_t s0 = ch * 128;
+                    while (runs[2 * (r + 1) + 1] <= s0) ++r;
+                    uint64_t mask = 0, pad = 0;
+                    for (size_t q = r + 1; runs[2 * q + 1] < std::min(m2, s0 + 128); ++q) mask |= 1ull << ((runs[2 * q + 1] - s0) / 2);
+                    for (size_t i = 0; i < 64 && s0 + 2 * i < m2; ++i)
+                        if (padSlot[s0 + 2 * i + 1]) pad |= 1ull << i;
+                    outChunk[5 * ch] = (uint32_t)r;
+                    outChunk[5 * ch + 1] = (uint32_t)mask;
+                    outChunk[5 * ch + 2] = (uint32_t)(mask >> 32);
+                    outChunk[5 * ch + 3] = (uint32_t)pad;
+                    outChunk[5 * ch + 4] = (uint32_t)(pad >> 32);
                 }
                 const uint32_t recipeOff = appendWords(outChunk.data(), outChunk.size() * 4);
                 appendWords(blockChunk.data(), blockChunk.size() * 4);
@@ -656,7 +677,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                 const uint32_t listOff = appendWords(lists.data(), lists.size() * 4);
                 pool[H + 0] = u32(inOff[k], "input base");
                 pool[H + 1] = (uint32_t)N;
-                pool[H + 2] = (uint32_t)m;
+                pool[H + 2] = (uint32_t)m2;
                 pool[H + 3] = (uint32_t)nb;
                 pool[H + 4] = recipeOff;
                 pool[H + 5] = runsOff;
@@ -673,7 +694,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                 stored = cvals.size();
                 if (stored >= (1u << 28)) throw std::length_error("fused form: too many output constants in a tile");
                 pool[H + 11] = (uint32_t)stored;
-                valueRegion = std::max(nvals + Z, m);
+                valueRegion = std::max(nvals + Z, m2);
                 out.maxBlocks = std::max(out.maxBlocks, nb);
             } else {
                 std::vector<uint32_t> runs;  // (position, first output)
@@ -925,7 +946,7 @@ std::vector<float> tileStats(const FusedProgram& fp) {
         const size_t H = P[1 + t];
         const uint32_t m = P[H + 2];
         const bool blockTile = fp.kind == 0 && fp.sums == FusedSums::Blocks;
-        size_t elems = 0, staged = fp.kind == 1 ? P[H + 12] : 0, vals = 0, runs = 0, S = 0;
+        size_t elems = 0, staged = fp.kind == 1 ? P[H + 12] : 0, vals = 0, runs = 0, S = 0, pads = 0;
         for (size_t q = 0; q < NT; ++q) {
             const size_t cnt = P[H + HEADER + 3 * q];
             elems += cnt;
@@ -933,9 +954,11 @@ std::vector<float> tileStats(const FusedProgram& fp) {
         }
         if (blockTile) {
             // contributions: list entries of blocks with elements (not the zero entry), D^2 each
-            const size_t nb = P[H + 3], nbch = (nb + 63) / 64, noch = ((size_t)m + 63) / 64;
+            const size_t nb = P[H + 3], nbch = (nb + 63) / 64, noch = ((size_t)m + 127) / 128;
             const size_t Z = (size_t)fp.blockDofs * fp.blockDofs;
-            const size_t bch = H + P[H + 4] + 3 * noch, brec = bch + 2 * nbch, lst = H + P[H + 6];
+            const size_t bch = H + P[H + 4] + 5 * noch, brec = bch + 2 * nbch, lst = H + P[H + 6];
+            for (size_t ch = 0; ch < noch; ++ch)
+                pads += (size_t)__builtin_popcount(P[H + P[H + 4] + 5 * ch + 3]) + (size_t)__builtin_popcount(P[H + P[H + 4] + 5 * ch + 4]);
             for (size_t ch = 0; ch < nbch; ++ch) {
                 const size_t lb = P[bch + 2 * ch] & 0xffffffu, Lc = P[bch + 2 * ch] >> 24;
                 for (size_t l = 0; l < 64 && ch * 64 + l < nb; ++l) {
@@ -957,7 +980,7 @@ std::vector<float> tileStats(const FusedProgram& fp) {
         }
         if (fp.kind == 0) staged = fusedStagedDoubles(fp, &P[H]);
         float* f = &out[t * TILE_STATS];
-        f[0] = (float)m;
+        f[0] = (float)(m - pads);
         f[1] = (float)S;
         f[2] = (float)elems;
         f[3] = (float)staged;
@@ -1060,8 +1083,9 @@ namespace {
 
 // Block-sum recipe of the element tile at H (layout in Fused.hpp); nvals: the tile's element
 // values, staged: its staged doubles.  Walks every table as the kernel reads it and checks
-// that the block entries' output positions are exactly the tile's outputs, each once.
-// Returns the tile's output contributions (list entries times D^2).
+// that the blocks' entries fill exactly the tile's output slots that are not padding, each
+// once, and that the slot pairs store inside their runs.  Returns the tile's output
+// contributions (list entries times D^2).
 size_t validateBlockTile(const FusedProgram& fp, size_t H, size_t nvals, size_t staged) {
     const auto& P = fp.pool;
     auto need = [](bool ok, const char* what) {
@@ -1069,41 +1093,32 @@ size_t validateBlockTile(const FusedProgram& fp, size_t H, size_t nvals, size_t 
     };
     const size_t D = (size_t)fp.blockDofs, Z = D * D, ND = D * (D + 1) / 2;
     need(D >= 1 && D <= 3, "block dofs");
-    const size_t m = P[H + 2], nb = P[H + 3], noch = (m + 63) / 64, nbch = (nb + 63) / 64;
-    need(m <= 65536 && nb <= fp.maxBlocks, "outputs or blocks");
+    const size_t m2 = P[H + 2], nb = P[H + 3], noch = (m2 + 127) / 128, nbch = (nb + 63) / 64;
+    need(m2 <= 65536 && m2 % 2 == 0 && nb <= fp.maxBlocks, "output slots or blocks");
     const size_t rec = H + P[H + 4], runs = H + P[H + 5], lst = H + P[H + 6], nod = H + P[H + 7];
-    const size_t bch = rec + 3 * noch, brec = bch + 2 * nbch;
+    const size_t bch = rec + 5 * noch, brec = bch + 2 * nbch;
     need(brec + nb == nod && nod <= runs && runs < lst && (runs - nod) % 2 == 0 && (lst - runs) % 2 == 0,
          "recipe tables not contiguous");
     const size_t listWords = (size_t)P[H + 4] + P[H + 10] - P[H + 6];
     need(P[H + 4] + (size_t)P[H + 10] >= P[H + 6] && lst + listWords <= P.size(), "lists outside the pool");
     const size_t nOwned = (runs - nod) / 2, numRuns = (lst - runs) / 2 - 1;
     need(nOwned <= 0x3fff, "owned nodes");
-    // runs: first outputs increasing from 0 to the sentinel m; positions inside the results
-    need(numRuns >= 1 || m == 0, "runs");
-    need(P[runs + 2 * numRuns + 1] == m && (m == 0 || P[runs + 1] == 0), "run sentinel");
+    // runs: first slots even and increasing from 0 to the sentinel m2
+    need(numRuns >= 1 || m2 == 0, "runs");
+    need(P[runs + 2 * numRuns + 1] == m2 && (m2 == 0 || P[runs + 1] == 0), "run sentinel");
     for (size_t r = 0; r < numRuns; ++r) {
         const size_t first = P[runs + 2 * r + 1], next = P[runs + 2 * r + 3];
-        need(next > first && next <= m, "runs not increasing");
-        need((size_t)P[H + 9] + P[runs + 2 * r] + (next - first) <= fp.numResults, "run outside the results");
+        need(first % 2 == 0 && next > first && next <= m2, "runs not increasing from even slots");
     }
-    for (size_t ch = 0; ch < noch; ++ch) {
-        const size_t r = P[rec + 3 * ch];
-        need(r < numRuns && P[runs + 2 * r + 1] <= ch * 64 && P[runs + 2 * r + 3] > ch * 64, "output chunk run");
-        uint64_t mask = 0;
-        for (size_t q = r + 1; q < numRuns && P[runs + 2 * q + 1] < std::min<size_t>(m, ch * 64 + 64); ++q)
-            mask |= 1ull << (P[runs + 2 * q + 1] - ch * 64);
-        need(P[rec + 3 * ch + 1] == (uint32_t)mask && P[rec + 3 * ch + 2] == (uint32_t)(mask >> 32), "output chunk run mask");
-    }
-    // node records: columns inside the outputs
+    // node records: columns inside the slots
     for (size_t ni = 0; ni < nOwned; ++ni) {
         const uint32_t w0 = P[nod + 2 * ni], w1 = P[nod + 2 * ni + 1];
         const size_t cs[3] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu}, nE = (w1 >> 16) & 0xffu, nK = w1 >> 24;
         need(nE + nK <= 127, "node blocks");
-        for (size_t d = 0; d < 3; ++d) need(d < D ? cs[d] + D * nE + nK <= m : cs[d] == 0, "node columns");
+        for (size_t d = 0; d < 3; ++d) need(d < D ? cs[d] + D * nE + nK <= m2 : cs[d] == 0, "node columns");
     }
-    // blocks: ranks inside their node's columns; their entries' positions cover the outputs once
-    std::vector<char> seen(m, 0);
+    // blocks: ranks inside their node's columns; their entries' slots, each once
+    std::vector<char> seen(m2, 0);
     size_t numConst = 0, contrib = 0, listBase = 0;
     for (size_t ch = 0; ch < nbch; ++ch) {
         const uint32_t c0 = P[bch + 2 * ch];
@@ -1127,7 +1142,7 @@ size_t validateBlockTile(const FusedProgram& fp, size_t H, size_t nvals, size_t 
                     for (size_t da = 0; da < D; ++da) {
                         if (kind == 2 && da != db) continue;
                         const size_t pos = cs[db] + da * nE + (db < da ? nK : 0) + (da == db ? rEK : rE);
-                        need(pos < m && !seen[pos], "block output position");
+                        need(pos < m2 && !seen[pos], "block output slot");
                         seen[pos] = 1;
                     }
             }
@@ -1145,11 +1160,33 @@ size_t validateBlockTile(const FusedProgram& fp, size_t H, size_t nvals, size_t 
         }
     }
     need(listBase == listWords, "list words");
-    for (size_t o = 0; o < m; ++o) need(seen[o], "output without a block entry");
+    // slot pairs: the run of each from the chunk records; a pair's second slot is padding
+    // exactly when no block fills it, and a stored pair lies inside its run
+    for (size_t ch = 0; ch < noch; ++ch) {
+        const size_t r0 = P[rec + 5 * ch];
+        const uint64_t mask = (uint64_t)P[rec + 5 * ch + 2] << 32 | P[rec + 5 * ch + 1];
+        const uint64_t pad = (uint64_t)P[rec + 5 * ch + 4] << 32 | P[rec + 5 * ch + 3];
+        need(r0 < numRuns && P[runs + 2 * r0 + 1] <= ch * 128 && P[runs + 2 * r0 + 3] > ch * 128, "output chunk run");
+        for (size_t i = 0; i < 64; ++i) {
+            const size_t s = ch * 128 + 2 * i;
+            if (s >= m2) {
+                need(!((mask >> i) & 1) && !((pad >> i) & 1), "chunk masks past the slots");
+                continue;
+            }
+            const size_t r = r0 + (size_t)__builtin_popcountll(mask & (i == 63 ? ~0ull : ((2ull << i) - 1)));
+            need(r < numRuns && P[runs + 2 * r + 1] <= s && s < P[runs + 2 * r + 3], "slot pair run");
+            need(((mask >> i) & 1) == (i > 0 && P[runs + 2 * r + 1] == s), "run-start mask");
+            const bool hiPad = (pad >> i) & 1;
+            need(seen[s] && seen[s + 1] == !hiPad, "padding mask");
+            need((size_t)P[H + 9] + P[runs + 2 * r] + (s - P[runs + 2 * r + 1]) + (hiPad ? 0 : 1) < fp.numResults,
+                 "slot pair outside the results");
+            need(hiPad || s + 1 < P[runs + 2 * r + 3], "slot pair across runs");
+        }
+    }
     const size_t stored = P[H + 11];
     need(stored == numConst, "block constant count");
     if (stored) need(P[H + 8] != 0xffffffffu && (size_t)P[H + 8] + stored <= fp.consts.size(), "block constants outside the constants");
-    need(8 * (std::max(nvals + Z, m) + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * stored <= fp.ldsBytes,
+    need(8 * (std::max(nvals + Z, m2) + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * stored <= fp.ldsBytes,
          "tile LDS exceeds the launch's LDS");
     return contrib;
 }

@@ -1022,7 +1022,12 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
     };
     std::ostringstream os;
     os << "// generated by symboliclib_amd: fused assembly form (LDS-staged element tiles), FP64\n";
-    os << "typedef unsigned int INDEX;\n\n";
+    os << "typedef unsigned int INDEX;\n";
+    // two consecutive output slots: 16-byte aligned in LDS, 8-byte aligned in the results
+    if (fp.sums == FusedSums::Blocks)
+        os << "typedef double symlib_pair __attribute__((ext_vector_type(2)));\n"
+           << "typedef double symlib_pair_u __attribute__((ext_vector_type(2), aligned(8)));\n";
+    os << "\n";
     if (opt.ldsDma) emitLdsDmaHelpers(os, T);
     std::vector<int> groupsOf(NT);
     for (size_t t = 0; t < NT; ++t) {
@@ -1330,8 +1335,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         const bool unrolled = iters <= 8;
         auto acc = [&](int it, int k) { return (unrolled ? "a" + std::to_string(it) : std::string("a")) + "_" + std::to_string(k); };
         auto recv = [&](int it) { return unrolled ? "r" + std::to_string(it) : std::string("rb"); };
-        os << "        const unsigned nb = nch, noch = (m + 63u) >> 6, nbch = (nb + 63u) >> 6;\n"
-           << "        const INDEX* __restrict__ BC = R + 3u * noch;  // block chunk records\n"
+        os << "        const unsigned nb = nch, noch = (m + 127u) >> 7, nbch = (nb + 63u) >> 6;\n"
+           << "        const INDEX* __restrict__ BC = R + 5u * noch;  // block chunk records\n"
            << "        const INDEX* __restrict__ BR = BC + 2u * nbch;  // block records\n"
            << "        const INDEX* __restrict__ NR = R + (hchk - hrec);  // node records\n"
            << "        const INDEX* __restrict__ RN = R + (hcnt - hrec);  // runs\n"
@@ -1367,8 +1372,13 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             os << ind << "}\n";
         };
         // B1 for iteration `it` (unrolled: a literal; looped: the loop's variable)
+        // waves take the chunks of 64 blocks in snake order (odd rounds reversed), so the first,
+        // heaviest chunks (diagonal blocks, the longest lists) and the last, lightest ones
+        // (blocks of a constant alone) fall to the same waves
         auto b1 = [&](int it) {
-            const std::string chx = unrolled ? std::to_string(it * W) + "u + wv" : "it * " + std::to_string(W) + "u + wv";
+            const std::string rev = std::to_string(W - 1) + "u - wv";
+            const std::string chx = unrolled ? std::to_string(it * W) + "u + " + (it % 2 ? "(" + rev + ")" : std::string("wv"))
+                                             : "it * " + std::to_string(W) + "u + ((it & 1u) ? " + rev + " : wv)";
             os << "        {\n"
                << "            const unsigned ch = " << chx << ";\n"
                << "            if (ch < nbch) {  // uniform\n"
@@ -1453,16 +1463,23 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             os << "        }\n";
         }
         os << "        __syncthreads();\n";
-        // B3: 64 consecutive outputs per wave, whole lines, stored without caching
+        // B3: each lane stores a pair of consecutive output slots (16 bytes: one LDS read, one
+        // store) and a wave 128 slots, whole lines, without caching; runs start at even slots,
+        // so a pair never spans two runs, and a pair whose second slot is padding stores one
         os << "        {\n"
            << "            double* __restrict__ res = out + hres;\n"
            << "            const unsigned long long below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);\n"
            << "            for (unsigned ch = wv; ch < noch; ch += " << W << "u) {\n"
-           << "                const unsigned oi = ch * 64u + lane;\n"
-           << "                const unsigned long long mk = ((unsigned long long)R[3u * ch + 2u] << 32) | R[3u * ch + 1u];\n"
-           << "                const unsigned r = R[3u * ch] + (unsigned)__popcll(mk & below);\n"
-           << "                const unsigned pos = RN[2u * r] + (oi - RN[2u * r + 1u]);\n"
-           << "                if (oi < m) __builtin_nontemporal_store(V[oi], res + pos);\n"
+           << "                const unsigned s = ch * 128u + 2u * lane;\n"
+           << "                const unsigned long long mk = ((unsigned long long)R[5u * ch + 2u] << 32) | R[5u * ch + 1u];\n"
+           << "                const unsigned long long pd = ((unsigned long long)R[5u * ch + 4u] << 32) | R[5u * ch + 3u];\n"
+           << "                const unsigned r = R[5u * ch] + (unsigned)__popcll(mk & below);\n"
+           << "                const unsigned pos = RN[2u * r] + (s - RN[2u * r + 1u]);\n"
+           << "                if (s < m) {\n"
+           << "                    const symlib_pair v = *(const symlib_pair*)(V + s);\n"
+           << "                    if ((pd >> lane) & 1ull) __builtin_nontemporal_store(v.x, res + pos);\n"
+           << "                    else __builtin_nontemporal_store(v, (symlib_pair_u*)(res + pos));\n"
+           << "                }\n"
            << "            }\n"
            << "        }\n";
     };
