@@ -210,7 +210,9 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
             if (!I.cRow.empty() || (Db && d != Db)) throw NotBlockStructured("constants or dofs");
             Db = d;
         }
-        if (Db < 1 || Db > 3) throw NotBlockStructured("dofs per node");
+        // one dof per node: a block is one output, so per-output lists are as short and the
+        // block form's pass through LDS only costs
+        if (Db < 2 || Db > 3) throw NotBlockStructured("dofs per node");
         for (auto& tl : L)
             if (!blockLayout(tl, getTemplate(tl.tmplId), Db)) throw NotBlockStructured("element template");
         for (size_t e = 0; e < elemTmpl_.size(); ++e) {

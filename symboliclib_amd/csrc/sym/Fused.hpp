@@ -60,24 +60,30 @@
 // block sits at tile output
 //   cs[db] + da * nE + (db < da ? nK : 0) + (da == db ? rank of a in E(b) u K(b) : rank in E(b)).
 // Lanes keep their block sums in registers (a local array beyond 8 blocks per thread), write
-// them after a barrier to the tile's outputs
-// in LDS (over the element values, which are dead by then), and a last pass stores the
-// outputs in CSC order (whole lines per wave).  Header words that differ from the form above:
-//     H[3] nb, blocks of the tile
-//     H[4] rel recipe start: output chunk records (first run, run-start mask low and high:
-//          3 words per 64 outputs), then block chunk records (list base in words | longest
-//          list << 24, first constant: 2 words per 64 blocks), then block records (1 word per block: owned
-//          node index bits 0-13, kind bits 14-15 (0 diagonal, 1 full, 2 wing), rank in E(b)
-//          bits 16-22, rank in E(b) u K(b) bits 23-29, has a constant bit 30)
-//     H[5] rel runs (as above)   H[6] rel lists   H[7] rel node records (2 words per owned node:
-//          cs[0] | cs[1] << 16, cs[2] | nE << 16 | nK << 24)
+// them after a barrier to the tile's output slots in LDS (over the element values, which are
+// dead by then), and a last pass stores the slots in CSC order, a pair of consecutive slots per
+// lane (16 bytes: one LDS read, one store; a wave stores whole lines).  Slots follow the
+// tile's runs; every run starts at an even slot (a padding slot before it when needed, and
+// one at the end when the count is odd), so a pair never spans two runs; cs[] above counts
+// slots.  Waves take the 64-block chunks in snake order (odd rounds reversed).  Header words
+// that differ from the form above:
+//     H[2] m2, output slots of the tile (outputs and padding, even)   H[3] nb, blocks
+//     H[4] rel recipe start: output chunk records per 128 slots (run of the first pair, pairs
+//          after it where a run starts, pairs whose second slot is padding: 5 words), then
+//          block chunk records (list base in words | longest list << 24, first constant: 2 words
+//          per 64 blocks), then block records (1 word per block: owned node index bits 0-13,
+//          kind bits 14-15 (0 diagonal, 1 full, 2 wing), rank in E(b) bits 16-22, rank in
+//          E(b) u K(b) bits 23-29, has a constant bit 30)
+//     H[5] rel runs: (result position, first slot) pairs, one sentinel (first = m2)
+//     H[6] rel lists   H[7] rel node records (2 words per owned node: cs[0] | cs[1] << 16,
+//          cs[2] | nE << 16 | nK << 24)
 //     H[8] block constants in c[] (one per block that has one, in block order) or 0xffffffff
 //     H[10] words from the recipe start to the end of the lists   H[11] stored constants
-//   Lists: a chunk of 64 blocks reads (longest + 1) / 2 u32 words per lane, word j of lane l at list base
-//   + 64 j + l, each two u16 entries (value index bits 0-14, transposed bit 15); entries past a
-//   block's list point at D x D zeros after the element values (V[nvals ..]).
-//   LDS: values and their zeros, or the tile's outputs when more (max(nvals + D^2, m)
-//   doubles), then the staged region, the recipe and the block constants.
+//   Lists: a chunk of 64 blocks reads (longest + 1) / 2 u32 words per lane, word j of lane l at
+//   list base + 64 j + l, each two u16 entries (value index bits 0-14, transposed bit 15);
+//   entries past a block's list point at D x D zeros after the element values (V[nvals ..]).
+//   LDS: values and their zeros, or the tile's slots when more (max(nvals + D^2, m2) doubles),
+//   then the staged region, the recipe and the block constants.
 #pragma once
 
 #include <cstdint>
