@@ -56,8 +56,12 @@ def test_noalloc_matches_run_then_smaller_program(require_device):
 
 
 def test_noalloc_rejects_programs_beyond_the_scratch_array(require_device):
+    # a profiled launch needs 4 + 8 words per tile: small tiles of a 256 x 256 cloth pass the
+    # 16,384 words of the array
     rest, F, X = W.c3_cloth(side=256)
-    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, profile_tiles=True) as u:
+    with Unit.bw_hessian(rest, F, W.K_STRETCH, W.K_SHEAR, W.K_BEND, fused=True, profile_tiles=True,
+                         tile_lds_bytes=40448) as u:
+        assert 4 + 8 * u.fused_stats()["num_tiles"] > 16384
         u.compile().execute(W.to_colmajor(X))
         rc = _native.lib().symhip_run_noalloc(u.runtime(), None)
         msg = _native.lib().symhip_last_error().decode()
