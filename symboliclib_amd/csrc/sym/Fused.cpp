@@ -329,9 +329,10 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
             nodeBytes[b] = bytes + outCount[b] * (1.0 + 4.0 * FUSED_CHUNK_WORDS / 64.0);
         }
         // block sums: a node adds its runs, node record, blocks (record, chunk record share,
-        // constant, list entries plus one for odd chunk lengths) and output chunk records; the
-        // lists of a tile are padded to each chunk's longest, at most 64 entries of the longest
-        // list per block kind (listMargin)
+        // constant, list entries) and output chunk records, and the tile a margin for the
+        // padding of its lists to each chunk's longest (listMargin).  This estimate is not a
+        // bound (a chunk whose longest list is odd pads a word per lane); a tile whose exact
+        // LDS passes the budget is grown again with one node less.
         double listMargin = 0.0;
         std::vector<uint32_t> nodeBlocks(blocks ? (size_t)N : 0, 0);
         if (blocks) {
@@ -344,11 +345,10 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                     entries += (double)P.count[q];
                     longest = std::max(longest, P.count[q]);
                 }
-                nodeBytes[b] = 8.0 * Db + 8.0 + outCount[b] * (12.0 / 64.0) + nblk * (4.0 + 8.0 / 64.0 + 8.0 + 2.0) +
-                               2.0 * entries;
+                nodeBytes[b] = 8.0 * Db + 8.0 + outCount[b] * (20.0 / 128.0) + nblk * (4.0 + 8.0 / 64.0 + 8.0) + 2.0 * entries;
                 nodeBlocks[b] = (uint32_t)nblk;
             }
-            listMargin = 2.0 * 128.0 * (double)longest + 3.0 * 256.0;
+            listMargin = 128.0 * ((double)longest + 2.0);
         }
 
         std::vector<char> assigned((size_t)N, 0);
@@ -363,6 +363,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
         InstanceTiles& out = tiles[k];
         int64_t tileId = 0;
         size_t cursor = 0;
+        size_t nodeCap = std::numeric_limits<size_t>::max();  // owned nodes of the tile being grown
         std::vector<uint32_t> tNodes;
         std::vector<std::vector<uint32_t>> tElems(NT);
 
@@ -442,7 +443,8 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                 // element values and the zero after them within 64 KiB: slots are u16 byte offsets
                 const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget && valueBytes + newValueBytes + 8 <= 65536;
                 const bool blockCap = blocks && tileBlocks + nodeBlocks[v] > FUSED_MAX_BLOCKS;
-                if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs || blockCap)) break;
+                if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs || blockCap || tNodes.size() >= nodeCap))
+                    break;
                 if (!fits) throw std::invalid_argument("fused form: one node and its elements need more LDS than a tile holds");
                 assigned[v] = 1;
                 tNodes.push_back(v);
@@ -479,6 +481,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
             // slot ldsBase[NT] is the tile's zero (padding of odd contribution lists); slots are
             // stored as LDS byte offsets (8 x index), so the values and the zero fit in 64 KiB
             if (8 * ldsBase[NT] >= 65536) throw std::logic_error("fused form: tile values exceed 64 KiB of byte offsets");
+            const size_t constsBefore = out.consts.size();
             std::vector<uint32_t>& pool = out.pool;
             const size_t H = pool.size();
             out.header.push_back(H);
@@ -830,6 +833,18 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
             }
             const size_t staged = fusedStagedDoubles(fp, &pool[H]);
             const size_t tileLds = 8 * (valueRegion + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * stored;
+            if (tileLds > fo.ldsBytes) {
+                if (!blocks || tNodes.size() <= 1) throw std::logic_error("fused form: a tile passes the LDS budget");
+                // the estimate let the tile grow past the budget: drop it, grow it again smaller
+                pool.resize(H);
+                out.header.pop_back();
+                out.consts.resize(constsBefore);
+                for (uint32_t v : tNodes) assigned[v] = 0;
+                nodeCap = tNodes.size() - 1;
+                ++tileId;
+                continue;
+            }
+            nodeCap = std::numeric_limits<size_t>::max();
             out.cost.push_back((double)m + (double)ldsBase[NT]);
             out.values.push_back(ldsBase[NT]);
             out.stagedLds = std::max(out.stagedLds, 8 * staged);

@@ -312,3 +312,18 @@ def test_synthetic_patterns_are_symmetric_with_power_law_rows():
         B = (A.T @ A + A).tocsc()
         B.sort_indices()
         assert np.array_equal(outer, B.indptr) and np.array_equal(inner, B.indices)
+
+
+@pytest.mark.parametrize("lds", [12288, 16384, 24576, 40448, 81408])
+def test_block_sum_tiles_within_the_lds_budget(lds):
+    # tiles grow on an estimate of their list padding; a tile whose exact LDS passes the budget
+    # is grown again smaller, so every tile fits (fans: long, odd diagonal lists)
+    meshes = [W.c3_cloth(side=48)[:2], W.wheel(18, seed=5)[:2], W.wheel(7, seed=3)[:2], W.cloth(23, seed=9)[:2]]
+    with Unit.bw_hessian_batch(meshes, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=lds) as u:
+        f = _fields(u.info())
+        assert "block records" in u.source()
+        assert int(f["lds"]) <= lds, f
+        for k, (rest, F) in enumerate(meshes):
+            cp, ri = O.bw_pattern(len(rest), F)
+            outer, inner = u.output_pattern(k)
+            assert np.array_equal(outer, cp) and np.array_equal(inner, ri)
