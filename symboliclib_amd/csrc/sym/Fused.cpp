@@ -329,26 +329,22 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
             nodeBytes[b] = bytes + outCount[b] * (1.0 + 4.0 * FUSED_CHUNK_WORDS / 64.0);
         }
         // block sums: a node adds its runs, node record, blocks (record, chunk record share,
-        // constant, list entries) and output chunk records, and the tile a margin for the
-        // padding of its lists to each chunk's longest (listMargin).  This estimate is not a
-        // bound (a chunk whose longest list is odd pads a word per lane); a tile whose exact
-        // LDS passes the budget is grown again with one node less.
-        double listMargin = 0.0;
+        // constant, list entries) and output chunk records.  The lists' padding to each
+        // chunk's longest is not counted, so this is an estimate, not a bound; a tile whose
+        // exact LDS passes the budget is grown again with one node less (a few in ten
+        // thousand tiles on cloths).
         std::vector<uint32_t> nodeBlocks(blocks ? (size_t)N : 0, 0);
         if (blocks) {
-            uint32_t longest = 0;
             for (int b = 0; b < N; ++b) {
                 const int c = b;
                 double nblk = 0.0, entries = 0.0;
                 for (int q = colptr[c]; q < colptr[c + 1] && P.pattern.inner()[q] < N; ++q) {
                     nblk += 1.0;
                     entries += (double)P.count[q];
-                    longest = std::max(longest, P.count[q]);
                 }
                 nodeBytes[b] = 8.0 * Db + 8.0 + outCount[b] * (20.0 / 128.0) + nblk * (4.0 + 8.0 / 64.0 + 8.0) + 2.0 * entries;
                 nodeBlocks[b] = (uint32_t)nblk;
             }
-            listMargin = 128.0 * ((double)longest + 2.0);
         }
 
         std::vector<char> assigned((size_t)N, 0);
@@ -438,10 +434,13 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                 // block sums: the tile's outputs overlay its values and their zeros when more
                 // (plus a padding slot before each run: at most Db per owned node, one at the end)
                 if (blocks)
-                    need += listMargin + std::max(0.0, 8.0 * (double)(outs + outCount[v] + (size_t)Db * (tNodes.size() + 1) + 1) -
+                    need += std::max(0.0, 8.0 * (double)(outs + outCount[v] + (size_t)Db * (tNodes.size() + 1) + 1) -
                                                            (double)(valueBytes + newValueBytes + 8 * (size_t)Db * Db));
-                // element values and the zero after them within 64 KiB: slots are u16 byte offsets
-                const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget && valueBytes + newValueBytes + 8 <= 65536;
+                // output form: element values and the zero after them within 64 KiB (slots are u16
+                // byte offsets); block form: values within 15-bit double indices
+                const bool fits = need + recipeBytes + nodeBytes[v] <= ldsBudget &&
+                                  (blocks ? valueBytes + newValueBytes + 8 * (size_t)Db * Db <= 8 * 0x7fff
+                                          : valueBytes + newValueBytes + 8 <= 65536);
                 const bool blockCap = blocks && tileBlocks + nodeBlocks[v] > FUSED_MAX_BLOCKS;
                 if (!tNodes.empty() && (!fits || outs + outCount[v] > fo.maxOutputs || blockCap || tNodes.size() >= nodeCap))
                     break;
@@ -480,7 +479,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
             }
             // slot ldsBase[NT] is the tile's zero (padding of odd contribution lists); slots are
             // stored as LDS byte offsets (8 x index), so the values and the zero fit in 64 KiB
-            if (8 * ldsBase[NT] >= 65536) throw std::logic_error("fused form: tile values exceed 64 KiB of byte offsets");
+            if (!blocks && 8 * ldsBase[NT] >= 65536) throw std::logic_error("fused form: tile values exceed 64 KiB of byte offsets");
             const size_t constsBefore = out.consts.size();
             std::vector<uint32_t>& pool = out.pool;
             const size_t H = pool.size();
@@ -1255,7 +1254,7 @@ void validateFused(const FusedProgram& fp) {
             lds += cnt * (size_t)ft.valueStride;
         }
         staged = fusedStagedDoubles(fp, &P[H]);
-        need(8 * (lds + 1) <= 65536, "tile values exceed 16-bit byte offsets");
+        need(blockTile || 8 * (lds + 1) <= 65536, "tile values exceed 16-bit byte offsets");
         need((size_t)P[H + 12] >= fp.numInputs && (size_t)P[H + 12] + lds <= fp.numInputs + fp.imageDoubles,
              "element image outside the value buffer");
         need(8 * staged <= fp.stagedLdsBytes, "staged values exceed the element launch's LDS");

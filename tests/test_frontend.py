@@ -273,13 +273,18 @@ def test_fused_block_sums_chosen_for_node_blocks():
 
 
 def test_fused_values_within_byte_offsets():
-    # slots are u16 LDS byte offsets: with a whole CU's LDS, tiles stop growing when their element
-    # values and the zero slot would pass 64 KiB
+    # one list per output: slots are u16 LDS byte offsets, so with a whole CU's LDS tiles stop
+    # growing when their element values and the zero slot would pass 64 KiB
     rest, F, _ = W.c3_cloth(side=64)
-    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=160 * 1024) as u:
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=160 * 1024, output_sums=True) as u:
         values = u.tile_stats()[:, 4]
         assert 8 * (values.max() + 1) <= 65536
         assert values.max() > 8000  # the cap, not the LDS budget, bounds the largest tiles
+    # block sums index values in doubles (15 bits): the LDS budget bounds the tiles
+    with Unit.bw_hessian(rest, F, 1e3, 1e2, 1e-2, fused=True, tile_lds_bytes=160 * 1024) as u:
+        assert "block records" in u.source()
+        values = u.tile_stats()[:, 4]
+        assert 8 * values.max() > 65536 and values.max() + 9 <= 0x7fff
 
 
 @pytest.mark.parametrize("subdiv", [0, 2, 4])
