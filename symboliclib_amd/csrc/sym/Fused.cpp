@@ -630,31 +630,74 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                     }
                 }
                 // blocks by kind, longest lists first: a chunk's lanes have near-equal lists
-                const size_t nb = blks.size(), nbch = (nb + 63) / 64, noch = (m2 + 127) / 128;
-                std::vector<uint32_t> order(nb);
+                const size_t nblk = blks.size(), noch = (m2 + 127) / 128;
+                std::vector<uint32_t> order(nblk);
                 std::iota(order.begin(), order.end(), 0u);
                 std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
                     return blks[x].kind != blks[y].kind ? blks[x].kind < blks[y].kind : blks[x].len > blks[y].len;
                 });
-                std::vector<uint32_t> blockRec(nb), blockChunk(2 * nbch), lists;
+                // diagonal blocks whose lists are at least twice the longest full block's (a node's
+                // every element against the elements of one edge) are summed by two lanes each,
+                // l and l + 32 of a halved chunk, half of the list each; without this the chunk of
+                // diagonal blocks takes its wave several times as long as any other
+                size_t nd = 0;
+                uint32_t Ld = 0, Lf = 0;
+                for (const Blk& B : blks) {
+                    if (B.kind == 0) {
+                        ++nd;
+                        Ld = std::max(Ld, B.len);
+                    } else if (B.kind == 1) {
+                        Lf = std::max(Lf, B.len);
+                    }
+                }
+                size_t nhalf = Ld >= 4 && Ld >= 2 * Lf ? (nd + 31) / 32 : 0;
+                if (64 * nhalf + nblk - nd > FUSED_MAX_BLOCKS) nhalf = 0;
+                const size_t nsplit = nhalf ? nd : 0;  // blocks in halved chunks
+                const size_t nb = 64 * nhalf + nblk - nsplit, nbch = (nb + 63) / 64;
+                std::vector<uint32_t> blockRec(nb, 0xc000u), blockChunk(2 * nbch), lists;
                 std::vector<double> cvals;
                 for (size_t ch = 0; ch < nbch; ++ch) {
+                    const bool halved = ch < nhalf;
+                    // lane l: its block and the part [lo, hi) of the block's list it sums
+                    struct LanePart {
+                        const Blk* B;
+                        uint32_t lo, hi;
+                        bool partner;
+                    };
+                    LanePart lp[64];
                     uint32_t Lc = 0;
-                    for (size_t bi = ch * 64; bi < std::min(nb, ch * 64 + 64); ++bi) Lc = std::max(Lc, blks[order[bi]].len);
+                    for (size_t l = 0; l < 64; ++l) {
+                        lp[l] = {nullptr, 0, 0, false};
+                        if (halved) {
+                            const size_t k = ch * 32 + (l & 31);
+                            if (k < nsplit) {
+                                const Blk* B = &blks[order[k]];
+                                const uint32_t h = (B->len + 1) / 2;
+                                lp[l] = l < 32 ? LanePart{B, 0, h, false} : LanePart{B, h, B->len, true};
+                            }
+                        } else {
+                            const size_t k = nsplit + (ch - nhalf) * 64 + l;
+                            if (k < nblk) lp[l] = {&blks[order[k]], 0, blks[order[k]].len, false};
+                        }
+                        Lc = std::max(Lc, lp[l].hi - lp[l].lo);
+                    }
                     const size_t pairs = (Lc + 1) / 2, lb = lists.size();
                     if (lb >= (1u << 24)) throw NotBlockStructured("block lists exceed 24-bit offsets");
                     blockChunk[2 * ch] = (uint32_t)lb | Lc << 24;
-                    blockChunk[2 * ch + 1] = (uint32_t)cvals.size();
+                    blockChunk[2 * ch + 1] = (uint32_t)cvals.size() | (halved ? 1u << 31 : 0u);
                     lists.resize(lb + 64 * pairs, 0);
                     for (size_t l = 0; l < 64; ++l) {
-                        const size_t bi = ch * 64 + l;
-                        const Blk* B = bi < nb ? &blks[order[bi]] : nullptr;
-                        auto entry = [&](size_t j) -> uint32_t { return B && j < B->len ? ents[B->first + j] : zeroEntry; };
+                        const LanePart& q = lp[l];
+                        auto entry = [&](size_t j) -> uint32_t { return q.B && q.lo + j < q.hi ? ents[q.B->first + q.lo + j] : zeroEntry; };
                         for (size_t j = 0; j < pairs; ++j) lists[lb + 64 * j + l] = entry(2 * j) | entry(2 * j + 1) << 16;
-                        if (!B) continue;
-                        blockRec[bi] = B->ni | B->kind << 14 | B->rE << 16 | B->rEK << 23 | (B->hasConst ? 1u << 30 : 0u);
-                        if (B->hasConst) cvals.push_back(B->cval);
-                        if (B->kind != 2) contrib += (size_t)B->len * Z;
+                        if (!q.B) continue;
+                        const Blk* B = q.B;
+                        // a partner lane sums the second half of a diagonal block; its record
+                        // has no constant, and it writes no outputs
+                        const bool hc = B->hasConst && !q.partner;
+                        blockRec[ch * 64 + l] = B->ni | B->kind << 14 | B->rE << 16 | B->rEK << 23 | (hc ? 1u << 30 : 0u);
+                        if (hc) cvals.push_back(B->cval);
+                        if (B->kind != 2 && !q.partner) contrib += (size_t)B->len * Z;
                     }
                 }
                 // output chunk records, per 128 slots (64 pairs): the run of the first pair, the
@@ -1137,24 +1180,30 @@ size_t validateBlockTile(const FusedProgram& fp, size_t H, size_t nvals, size_t 
     std::vector<char> seen(m2, 0);
     size_t numConst = 0, contrib = 0, listBase = 0;
     for (size_t ch = 0; ch < nbch; ++ch) {
-        const uint32_t c0 = P[bch + 2 * ch];
+        const uint32_t c0 = P[bch + 2 * ch], kw = P[bch + 2 * ch + 1];
         const size_t lb = c0 & 0xffffffu, Lc = c0 >> 24, pairs = (Lc + 1) / 2;
-        need(lb == listBase && P[bch + 2 * ch + 1] == numConst, "block chunk record");
+        const bool halved = (kw >> 31) != 0;
+        need(lb == listBase && (kw & 0x7fffffffu) == numConst, "block chunk record");
         listBase += 64 * pairs;
         need(listBase <= listWords, "block lists");
         for (size_t l = 0; l < 64; ++l) {
             const size_t bi = ch * 64 + l;
             size_t kind = 3;
-            if (bi < nb) {
+            // a halved chunk: lane l + 32 sums the rest of lane l's diagonal block (the same
+            // record without a constant) and writes nothing; lanes without a block are 0xc000
+            if (halved && l >= 32) need(bi < nb && P[brec + bi] == (P[brec + bi - 32] & ~(1u << 30)), "partner record");
+            if (bi < nb && P[brec + bi] == 0xc000u) need(halved, "empty block record outside a halved chunk");
+            if (bi < nb && P[brec + bi] != 0xc000u) {
                 const uint32_t r = P[brec + bi];
                 const size_t ni = r & 0x3fffu, rE = (r >> 16) & 0x7fu, rEK = (r >> 23) & 0x7fu;
                 kind = (r >> 14) & 3u;
                 need(ni < nOwned && kind <= 2 && (r >> 31) == 0, "block record");
+                need(!halved || kind == 0, "halved chunk of other than diagonal blocks");
                 numConst += (r >> 30) & 1u;
                 const uint32_t w0 = P[nod + 2 * ni], w1 = P[nod + 2 * ni + 1];
                 const size_t cs[3] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu}, nE = (w1 >> 16) & 0xffu, nK = w1 >> 24;
                 need(rEK < nE + nK && (kind == 2 || rE < nE), "block ranks");
-                for (size_t db = 0; db < D; ++db)
+                for (size_t db = 0; db < D && !(halved && l >= 32); ++db)
                     for (size_t da = 0; da < D; ++da) {
                         if (kind == 2 && da != db) continue;
                         const size_t pos = cs[db] + da * nE + (db < da ? nK : 0) + (da == db ? rEK : rE);

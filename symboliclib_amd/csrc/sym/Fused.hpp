@@ -70,10 +70,11 @@
 //     H[2] m2, output slots of the tile (outputs and padding, even)   H[3] nb, blocks
 //     H[4] rel recipe start: output chunk records per 128 slots (run of the first pair, pairs
 //          after it where a run starts, pairs whose second slot is padding: 5 words), then
-//          block chunk records (list base in words | longest list << 24, first constant: 2 words
-//          per 64 blocks), then block records (1 word per block: owned node index bits 0-13,
-//          kind bits 14-15 (0 diagonal, 1 full, 2 wing), rank in E(b) bits 16-22, rank in
-//          E(b) u K(b) bits 23-29, has a constant bit 30)
+//          block chunk records (list base in words | longest list << 24, first constant |
+//          halved << 31: 2 words per 64 block records), then block records (1 word per lane of
+//          a chunk: owned node index bits 0-13, kind bits 14-15 (0 diagonal, 1 full, 2 wing,
+//          3 none: 0xc000), rank in E(b) bits 16-22, rank in E(b) u K(b) bits 23-29, has a
+//          constant bit 30)
 //     H[5] rel runs: (result position, first slot) pairs, one sentinel (first = m2)
 //     H[6] rel lists   H[7] rel node records (2 words per owned node: cs[0] | cs[1] << 16,
 //          cs[2] | nE << 16 | nK << 24)
@@ -82,6 +83,10 @@
 //   Lists: a chunk of 64 blocks reads (longest + 1) / 2 u32 words per lane, word j of lane l at
 //   list base + 64 j + l, each two u16 entries (value index bits 0-14, transposed bit 15);
 //   entries past a block's list point at D x D zeros after the element values (V[nvals ..]).
+//   Halved chunks come first when the diagonal blocks' lists are at least 4 long and twice the
+//   full blocks' longest: 32 diagonal blocks each, lane l summing the first (len + 1) / 2
+//   entries of block l and lane l + 32 the rest (its record: lane l's without the constant);
+//   the halves meet in lane l through a cross-lane add, and lane l + 32 writes nothing.
 //   LDS: values and their zeros, or the tile's slots when more (max(nvals + D^2, m2) doubles),
 //   then the staged region, the recipe and the block constants.
 #pragma once

@@ -1385,7 +1385,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "                const unsigned bi = ch * 64u + lane;\n"
                << "                const unsigned rec = bi < nb ? BR[bi] : 0xc000u;\n"
                << "                " << recv(it) << " = rec;\n"
-               << "                const unsigned c0 = BC[2u * ch], kb = BC[2u * ch + 1u];\n"
+               << "                const unsigned c0 = BC[2u * ch], kw = BC[2u * ch + 1u], kb = kw & 0x7fffffffu;\n"
                << "                const unsigned L = c0 >> 24, kind = (rec >> 14) & 3u;\n"
                << "                const unsigned* __restrict__ lw = LW + (c0 & 0xffffffu) + lane;\n";
             for (int kind = 0; kind < 2; ++kind) {
@@ -1402,6 +1402,14 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                    << "                    }\n"
                    << "                }\n";
             }
+            // a halved chunk (diagonal blocks with long lists): lane l + 32 summed the second half
+            // of lane l's list; the halves meet in lane l, and lane l + 32 writes nothing
+            os << "                if (kw >> 31) {  // uniform\n";
+            for (int da = 0; da < D; ++da)
+                for (int db = da; db < D; ++db)
+                    os << "                    " << acc(it, da * D + db) << " += __shfl_xor(" << acc(it, da * D + db) << ", 32);\n";
+            os << "                    if (lane >= 32u) " << recv(it) << " = 0xc000u;\n"
+               << "                }\n";
             // a block's constant adds to its diagonal entries; constants are stored per block
             // in block order, so a lane's is the chunk's first plus the lanes below with one
             os << "                const unsigned hc = (rec >> 30) & 1u;\n"
