@@ -1371,63 +1371,73 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                 }
             os << ind << "}\n";
         };
-        // B1 for iteration `it` (unrolled: a literal; looped: the loop's variable)
-        // waves take the chunks of 64 blocks in snake order (odd rounds reversed), so the first,
+        // B1 for iteration `it` (unrolled: a literal; looped: the loop's variable), in two parts:
+        // b1pre reads the chunk's records, the lane's first list word and its block constant;
+        // b1sum adds the entries.  Unrolled, every chunk's b1pre comes first, so the reads of a
+        // wave's later chunks are in flight while its first chunk is summed.
+        // Waves take the chunks of 64 blocks in snake order (odd rounds reversed), so the first,
         // heaviest chunks (diagonal blocks, the longest lists) and the last, lightest ones
         // (blocks of a constant alone) fall to the same waves
-        auto b1 = [&](int it) {
-            const std::string rev = std::to_string(W - 1) + "u - wv";
+        auto sfx = [&](int it) { return unrolled ? std::to_string(it) : std::string("b"); };
+        auto b1pre = [&](int it) {
+            const std::string rev = std::to_string(W - 1) + "u - wv", x = sfx(it);
             const std::string chx = unrolled ? std::to_string(it * W) + "u + " + (it % 2 ? "(" + rev + ")" : std::string("wv"))
                                              : "it * " + std::to_string(W) + "u + ((it & 1u) ? " + rev + " : wv)";
-            os << "        {\n"
-               << "            const unsigned ch = " << chx << ";\n"
-               << "            if (ch < nbch) {  // uniform\n"
-               << "                const unsigned bi = ch * 64u + lane;\n"
-               << "                const unsigned rec = bi < nb ? BR[bi] : 0xc000u;\n"
-               << "                " << recv(it) << " = rec;\n"
-               << "                const unsigned c0 = BC[2u * ch], kw = BC[2u * ch + 1u], kb = kw & 0x7fffffffu;\n"
-               << "                const unsigned L = c0 >> 24, kind = (rec >> 14) & 3u;\n"
-               << "                const unsigned* __restrict__ lw = LW + (c0 & 0xffffffu) + lane;\n";
+            os << "        const unsigned bch" << x << " = " << chx << ";\n"
+               << "        unsigned bc" << x << " = 0u, bkw" << x << " = 0u, bw" << x << " = 0u;\n"
+               << "        double bkc" << x << " = 0.0;  // the block's constant, added to its diagonal\n"
+               << "        if (bch" << x << " < nbch) {  // uniform\n"
+               << "            const unsigned bi = bch" << x << " * 64u + lane;\n"
+               << "            const unsigned rec = bi < nb ? BR[bi] : 0xc000u;\n"
+               << "            " << recv(it) << " = rec;\n"
+               << "            bc" << x << " = BC[2u * bch" << x << "];\n"
+               << "            bkw" << x << " = BC[2u * bch" << x << " + 1u];\n"
+               << "            if ((bc" << x << " >> 24) != 0u) bw" << x << " = LW[(bc" << x << " & 0xffffffu) + lane];  // uniform\n"
+            // constants are stored per block in block order, so a lane's is the chunk's first
+            // plus the lanes below with one
+               << "            const unsigned hc = (rec >> 30) & 1u;\n"
+               << "            const unsigned long long cm = __ballot(hc);\n"
+               << "            const unsigned ci = (bkw" << x << " & 0x7fffffffu) + __builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32), "
+                  "__builtin_amdgcn_mbcnt_lo((unsigned)cm, 0u));\n"
+               << "            if (hc) bkc" << x << " = K[ci];\n"
+               << "        }\n";
+        };
+        auto b1sum = [&](int it) {
+            const std::string x = sfx(it);
+            os << "        if (bch" << x << " < nbch) {  // uniform\n"
+               << "            const unsigned L = bc" << x << " >> 24, kind = (" << recv(it) << " >> 14) & 3u;\n"
+               << "            const unsigned* __restrict__ lw = LW + (bc" << x << " & 0xffffffu) + lane;\n";
             for (int kind = 0; kind < 2; ++kind) {
-                os << "                " << (kind ? "else if" : "if") << " (kind == " << kind << "u) {\n"
-                   << "                    for (unsigned j = 0; j < L; j += 2u) {\n"
-                   << "                        const unsigned w = lw[(j >> 1) * 64u];\n";
-                if (kind == 0) diagEntry(it, "w", "                        ");
-                else fullEntry(it, "w", "                        ");
-                os << "                        if (j + 1u < L) {\n"
-                   << "                            const unsigned w1 = w >> 16;\n";
-                if (kind == 0) diagEntry(it, "w1", "                            ");
-                else fullEntry(it, "w1", "                            ");
-                os << "                        }\n"
-                   << "                    }\n"
-                   << "                }\n";
+                os << "            " << (kind ? "else if" : "if") << " (kind == " << kind << "u) {\n"
+                   << "                for (unsigned j = 0; j < L; j += 2u) {\n"
+                   << "                    const unsigned w = j == 0u ? bw" << x << " : lw[(j >> 1) * 64u];\n";
+                if (kind == 0) diagEntry(it, "w", "                    ");
+                else fullEntry(it, "w", "                    ");
+                os << "                    if (j + 1u < L) {\n"
+                   << "                        const unsigned w1 = w >> 16;\n";
+                if (kind == 0) diagEntry(it, "w1", "                        ");
+                else fullEntry(it, "w1", "                        ");
+                os << "                    }\n"
+                   << "                }\n"
+                   << "            }\n";
             }
             // a halved chunk (diagonal blocks with long lists): lane l + 32 summed the second half
-            // of lane l's list; the halves meet in lane l, and lane l + 32 writes nothing
-            os << "                if (kw >> 31) {  // uniform\n";
+            // of lane l's list; the halves meet in lane l, and lane l + 32 writes nothing (its
+            // record has no constant)
+            os << "            if (bkw" << x << " >> 31) {  // uniform\n";
             for (int da = 0; da < D; ++da)
                 for (int db = da; db < D; ++db)
-                    os << "                    " << acc(it, da * D + db) << " += __shfl_xor(" << acc(it, da * D + db) << ", 32);\n";
-            os << "                    if (lane >= 32u) " << recv(it) << " = 0xc000u;\n"
-               << "                }\n";
-            // a block's constant adds to its diagonal entries; constants are stored per block
-            // in block order, so a lane's is the chunk's first plus the lanes below with one
-            os << "                const unsigned hc = (rec >> 30) & 1u;\n"
-               << "                const unsigned long long cm = __ballot(hc);\n"
-               << "                const unsigned ci = kb + __builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32), "
-                  "__builtin_amdgcn_mbcnt_lo((unsigned)cm, 0u));\n"
-               << "                if (hc) {\n"
-               << "                    const double kc = K[ci];\n";
-            for (int d = 0; d < D; ++d) os << "                    " << acc(it, d * D + d) << " += kc;\n";
-            os << "                }\n";
+                    os << "                " << acc(it, da * D + db) << " += __shfl_xor(" << acc(it, da * D + db) << ", 32);\n";
+            os << "                if (lane >= 32u) " << recv(it) << " = 0xc000u;\n"
+               << "            }\n";
+            for (int d = 0; d < D; ++d) os << "            " << acc(it, d * D + d) << " += bkc" << x << ";\n";
             if (D > 1) {
-                os << "                if (kind == 0u) {  // the diagonal block's lower triangle\n";
+                os << "            if (kind == 0u) {  // the diagonal block's lower triangle\n";
                 for (int da = 0; da < D; ++da)
-                    for (int db = 0; db < da; ++db) os << "                    " << acc(it, da * D + db) << " = " << acc(it, db * D + da) << ";\n";
-                os << "                }\n";
+                    for (int db = 0; db < da; ++db) os << "                " << acc(it, da * D + db) << " = " << acc(it, db * D + da) << ";\n";
+                os << "            }\n";
             }
-            os << "            }\n"
-               << "        }\n";
+            os << "        }\n";
         };
         // B2 for iteration `it`: the block's entries to their tile outputs
         auto b2 = [&](int it) {
@@ -1452,13 +1462,15 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "        }\n";
         };
         if (unrolled) {
-            for (int it = 0; it < iters; ++it) b1(it);
+            for (int it = 0; it < iters; ++it) b1pre(it);
+            for (int it = 0; it < iters; ++it) b1sum(it);
             os << "        __syncthreads();  // every element value read: the outputs overlay them\n";
             for (int it = 0; it < iters; ++it) b2(it);
         } else {
             os << "        for (unsigned it = 0; it < " << iters << "u; ++it) {\n";
             declare(0, "        ");
-            b1(0);
+            b1pre(0);
+            b1sum(0);
             for (int k = 0; k < Z; ++k) os << "        acc[it][" << k << "] = " << acc(0, k) << ";\n";
             os << "        recs[it] = " << recv(0) << ";\n"
                << "        }\n"
