@@ -329,7 +329,8 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
             nodeBytes[b] = bytes + outCount[b] * (1.0 + 4.0 * FUSED_CHUNK_WORDS / 64.0);
         }
         // block sums: a node adds its runs, node record, blocks (record, chunk record share,
-        // constant, list entries) and output chunk records.  The lists' padding to each
+        // list entries) and output chunk records; block constants are read from global memory,
+        // not staged.  The lists' padding to each
         // chunk's longest is not counted, so this is an estimate, not a bound; a tile whose
         // exact LDS passes the budget is grown again with one node less (a few in ten
         // thousand tiles on cloths).
@@ -342,7 +343,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                     nblk += 1.0;
                     entries += (double)P.count[q];
                 }
-                nodeBytes[b] = 8.0 * Db + 8.0 + outCount[b] * (20.0 / 128.0) + nblk * (4.0 + 8.0 / 64.0 + 8.0) + 2.0 * entries;
+                nodeBytes[b] = 8.0 * Db + 8.0 + outCount[b] * (20.0 / 128.0) + nblk * (4.0 + 8.0 / 64.0) + 2.0 * entries;
                 nodeBlocks[b] = (uint32_t)nblk;
             }
         }
@@ -874,7 +875,7 @@ FusedProgram HessianAssembly::buildFusedForm(const BuildOptions& opt, const Fuse
                 }
             }
             const size_t staged = fusedStagedDoubles(fp, &pool[H]);
-            const size_t tileLds = 8 * (valueRegion + staged) + 8 * ((pool[H + 10] + 1) / 2) + 8 * stored;
+            const size_t tileLds = 8 * (valueRegion + staged) + 8 * ((pool[H + 10] + 1) / 2) + (blocks ? 0 : 8 * stored);
             if (tileLds > fo.ldsBytes) {
                 if (!blocks || tNodes.size() <= 1) throw std::logic_error("fused form: a tile passes the LDS budget");
                 // the estimate let the tile grow past the budget: drop it, grow it again smaller
@@ -1251,7 +1252,7 @@ size_t validateBlockTile(const FusedProgram& fp, size_t H, size_t nvals, size_t 
     const size_t stored = P[H + 11];
     need(stored == numConst, "block constant count");
     if (stored) need(P[H + 8] != 0xffffffffu && (size_t)P[H + 8] + stored <= fp.consts.size(), "block constants outside the constants");
-    need(8 * (std::max(nvals + Z, m2) + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) + 8 * stored <= fp.ldsBytes,
+    need(8 * (std::max(nvals + Z, m2) + staged) + 8 * (((size_t)P[H + 10] + 1) / 2) <= fp.ldsBytes,
          "tile LDS exceeds the launch's LDS");
     return contrib;
 }

@@ -88,7 +88,8 @@
 //   entries of block l and lane l + 32 the rest (its record: lane l's without the constant);
 //   the halves meet in lane l through a cross-lane add, and lane l + 32 writes nothing.
 //   LDS: values and their zeros, or the tile's slots when more (max(nvals + D^2, m2) doubles),
-//   then the staged region, the recipe and the block constants.
+//   then the staged region and the recipe; a lane reads its block's constant from c[] (one
+//   load, issued before the chunk's sums), so constants take no LDS.
 #pragma once
 
 #include <cstdint>

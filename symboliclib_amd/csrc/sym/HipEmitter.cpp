@@ -1190,8 +1190,9 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         }
         os << "        {\n";
         if (copyRecipe)
-            os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n"
-               << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
+            os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n";
+        if (copyRecipe && !blockSums)
+            os << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
         os << "            for (unsigned w = threadIdx.x; w < " << total << "; w += " << T << "u) {\n"
            << "                unsigned j = w, Lb = 0;\n";
         for (size_t t = 0; t < NT; ++t) {
@@ -1340,7 +1341,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
            << "        const INDEX* __restrict__ BR = BC + 2u * nbch;  // block records\n"
            << "        const INDEX* __restrict__ NR = R + (hchk - hrec);  // node records\n"
            << "        const INDEX* __restrict__ RN = R + (hcnt - hrec);  // runs\n"
-           << "        const unsigned* __restrict__ LW = R + (hslt - hrec);  // lists\n";
+           << "        const unsigned* __restrict__ LW = R + (hslt - hrec);  // lists\n"
+           << "        const double* __restrict__ KG = c + (nconst ? hcst : 0u);  // block constants, in c[]\n";
         auto declare = [&](int it, const std::string& ind) {
             os << ind << "double";
             for (int k = 0; k < Z; ++k) os << (k ? ", " : " ") << acc(it, k) << " = 0.0";
@@ -1399,7 +1401,7 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                << "            const unsigned long long cm = __ballot(hc);\n"
                << "            const unsigned ci = (bkw" << x << " & 0x7fffffffu) + __builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32), "
                   "__builtin_amdgcn_mbcnt_lo((unsigned)cm, 0u));\n"
-               << "            if (hc) bkc" << x << " = K[ci];\n"
+               << "            if (hc) bkc" << x << " = KG[ci];\n"
                << "        }\n";
         };
         auto b1sum = [&](int it) {
@@ -1580,9 +1582,9 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
             if (variant == 2) os << "            symlib_lds_copy32((const unsigned*)(x + H[12]), (unsigned*)V, 2u * nvals, wv, lane);\n";
             // with element evaluation in the launch, the recipe and constants are copied during
             // it instead (A1), off the staging phase's critical path
-            if (sums && !eval)
-                os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n"
-                   << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
+            if (sums && !eval) os << "            symlib_lds_copy32(tiles + hrec, R, words, wv, lane);\n";
+            if (sums && !eval && !blockSums)
+                os << "            if (nconst) symlib_lds_copy32((const unsigned*)(c + hcst), (unsigned*)K, 2u * nconst, wv, lane);\n";
             os << "            __builtin_amdgcn_s_waitcnt(0);\n";
             // the node numbers stay live to here, so no register that held one is rewritten
             // while its load may be in flight (which would wait for every load issued so far)
@@ -1594,8 +1596,9 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
                    << "            for (unsigned q = threadIdx.x; q < nvals; q += " << T << "u) V[q] = img[q];\n";
             if (sums)
                 os << "            const INDEX* __restrict__ src = tiles + hrec;\n"
-                   << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n"
-                   << "            if (nconst) {\n"
+                   << "            for (unsigned w = threadIdx.x; w < words; w += " << T << "u) R[w] = src[w];\n";
+            if (sums && !blockSums)
+                os << "            if (nconst) {\n"
                    << "                const double* __restrict__ cs = c + hcst;\n"
                    << "                for (unsigned q = threadIdx.x; q < nconst; q += " << T << "u) K[q] = cs[q];\n"
                    << "            }\n";
