@@ -65,8 +65,8 @@
 // lane (16 bytes: one LDS read, one store; a wave stores whole lines).  Slots follow the
 // tile's runs; every run starts at an even slot (a padding slot before it when needed, and
 // one at the end when the count is odd), so a pair never spans two runs; cs[] above counts
-// slots.  Waves take the 64-block chunks in snake order (odd rounds reversed).  Header words
-// that differ from the form above:
+// slots.  Waves take the 64-block chunks in snake order (every second pass reversed).  Header
+// words that differ from the form above:
 //     H[2] m2, output slots of the tile (outputs and padding, even)   H[3] nb, blocks
 //     H[4] rel recipe start: output chunk records per 128 slots (run of the first pair, pairs
 //          after it where a run starts, pairs whose second slot is padding: 5 words), then

@@ -1377,8 +1377,8 @@ std::string emitHipFused(const FusedProgram& fp, const FusedOptions& opt, std::v
         // b1pre reads the chunk's records, the lane's first list word and its block constant;
         // b1sum adds the entries.  Unrolled, every chunk's b1pre comes first, so the reads of a
         // wave's later chunks are in flight while its first chunk is summed.
-        // Waves take the chunks of 64 blocks in snake order (odd rounds reversed), so the first,
-        // heaviest chunks (diagonal blocks, the longest lists) and the last, lightest ones
+        // Waves take the chunks of 64 blocks in snake order (every second pass reversed), so the
+        // first, heaviest chunks (diagonal blocks, the longest lists) and the last, lightest ones
         // (blocks of a constant alone) fall to the same waves
         auto sfx = [&](int it) { return unrolled ? std::to_string(it) : std::string("b"); };
         auto b1pre = [&](int it) {
